@@ -1,0 +1,442 @@
+// C ABI of the DEAC engine (include/deac_engine.h): device memory, uploads,
+// the batch schedule of the RNG walk / mask / evolution kernels, and the
+// host-side epilogue (best member, output transform, final statistics).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "deac_engine.h"
+#include "host_model.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_message;
+
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct deac_handle {
+    deac::Model m;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int MB = 0, TB = 0, Kcp = 0, nwg = 0, chunk = 0, batch = 0;
+    std::vector<void *> allocations;
+    double *kmat = nullptr, *nterm = nullptr, *fterm = nullptr, *tterm = nullptr, *obs = nullptr, *sig = nullptr;
+    double *pop0 = nullptr, *pop1 = nullptr, *fitness = nullptr, *cr_old = nullptr, *f_old = nullptr;
+    double *emod = nullptr, *mom = nullptr;
+    int *cur = nullptr, *status = nullptr;
+    unsigned *bar = nullptr;
+    deac::MemberCtl *ctl = nullptr;
+    uint64_t *raw = nullptr, *gen_start = nullptr, *state = nullptr, *skip_rows = nullptr, *lane_tab = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_loop_ms = 0.0;
+    size_t last_generations = 0;
+
+    ~deac_handle()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : allocations) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <class T>
+    deac_status alloc(T *&out, size_t count)
+    {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return DEAC_ERR_OUT_OF_MEMORY;
+        allocations.push_back(p);
+        out = static_cast<T *>(p);
+        return DEAC_OK;
+    }
+};
+
+namespace {
+
+#define DEAC_TRY(expr)                         \
+    do {                                       \
+        const deac_status st_ = (expr);        \
+        if (st_ != DEAC_OK) return st_;        \
+    } while (0)
+#define DEAC_HIP(expr)                         \
+    do {                                       \
+        if ((expr) != hipSuccess) {            \
+            (void)hipGetLastError();           \
+            return DEAC_ERR_DEVICE;            \
+        }                                      \
+    } while (0)
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Interleaved population layout [member-block][pair][8][2] from [P][Kc].
+void interleave_population(const std::vector<double> &pop, size_t P, size_t Kc, int Kcp, std::vector<double> &out)
+{
+    const int MB = static_cast<int>((P + deac::kMemberBlock - 1) / deac::kMemberBlock);
+    out.assign(static_cast<size_t>(MB) * Kcp * 16, 0.0);
+    for (size_t i = 0; i < P; i++)
+        for (size_t k = 0; k < Kc; k++)
+            out[(((i >> 3) * Kcp + (k >> 1)) * 8 + (i & 7)) * 2 + (k & 1)] = pop[i * Kc + k];
+}
+
+void extract_member(const std::vector<double> &lay, size_t i, size_t Kc, int Kcp, double *out)
+{
+    for (size_t k = 0; k < Kc; k++) out[k] = lay[(((i >> 3) * Kcp + (k >> 1)) * 8 + (i & 7)) * 2 + (k & 1)];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *deac_status_string(deac_status s)
+{
+    switch (s) {
+    case DEAC_OK: return "success";
+    case DEAC_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case DEAC_ERR_NO_DEVICE: return "no usable gfx950 device";
+    case DEAC_ERR_OUT_OF_MEMORY: return "out of memory";
+    case DEAC_ERR_DEVICE: return "device error";
+    case DEAC_ERR_UNSUPPORTED: return "unsupported";
+    }
+    return "unknown status";
+}
+
+size_t deac_output_length(const deac_problem *p)
+{
+    if (!p) return 0;
+    return p->zero_temperature ? p->genome_size : 2 * p->genome_size - 1;
+}
+
+void deac_seed_state(uint64_t seed, uint64_t state[4]) { deac::seed_state(seed, state); }
+
+void deac_rng_advance(uint64_t state[4], uint64_t n)
+{
+    const deac::Gf2Matrix m = deac::transition_power(n);
+    uint64_t out[4];
+    m.apply(state, out);
+    std::memcpy(state, out, sizeof out);
+}
+
+deac_status deac_frequency_grid(size_t G, double omega_max, double *f)
+{
+    if (G < 2 || !f) return DEAC_ERR_INVALID_ARGUMENT;
+    const double d = omega_max / static_cast<double>(G - 1);  // deac.cpp:2121-2125
+    for (size_t i = 0; i < G; i++) f[i] = static_cast<double>(i) * d;
+    return DEAC_OK;
+}
+
+deac_status deac_create(const deac_problem *p, deac_handle **out, const char **message)
+{
+    if (message) *message = nullptr;
+    if (!p || !out) return DEAC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    const deac_status v = deac::validate(*p, g_message);
+    if (v != DEAC_OK) {
+        if (message) *message = g_message.c_str();
+        return v;
+    }
+
+    int device = 0, count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return DEAC_ERR_NO_DEVICE;
+    }
+    DEAC_HIP(hipGetDevice(&device));
+    hipDeviceProp_t prop;
+    DEAC_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DEAC_ERR_NO_DEVICE;
+
+    deac_handle *h = new deac_handle();
+    h->m = deac::build_model(*p);
+    h->device = device;
+    const deac::Model &m = h->m;
+    auto fail = [&](deac_status s) {
+        delete h;
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(DEAC_ERR_DEVICE);
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return fail(DEAC_ERR_DEVICE);
+
+    const int T = static_cast<int>(m.T), P = static_cast<int>(m.P), Kc = static_cast<int>(m.Kc);
+    h->MB = (P + deac::kMemberBlock - 1) / deac::kMemberBlock;
+    h->TB = (T + deac::kTauBlock - 1) / deac::kTauBlock;
+    h->Kcp = round_up((Kc + 1) / 2, deac::kPairBatch);
+    h->chunk = (Kc + 63) / 64;
+    // RNG batch: generations whose mask draws are buffered at once (<= 256 MiB).
+    const size_t per_gen = static_cast<size_t>(P) * Kc * sizeof(uint64_t);
+    h->batch = static_cast<int>(std::clamp<size_t>((size_t(256) << 20) / per_gen, 1, 1024));
+
+    const int maxwg = deac::evolve_max_workgroups(device);
+    if (maxwg <= 0) return fail(DEAC_ERR_DEVICE);
+    int nwg = std::max(h->MB * h->TB, 32);
+    if (const char *e = std::getenv("DEAC_EVOLVE_WORKGROUPS")) nwg = std::atoi(e);
+    h->nwg = std::clamp(nwg, 1, maxwg);
+
+    // Device buffers.
+    const size_t kmat_n = static_cast<size_t>(h->TB) * h->Kcp * 16;
+    const size_t pop_n = static_cast<size_t>(h->MB) * h->Kcp * 16;
+    deac_status st = DEAC_OK;
+    auto A = [&](auto *&ptr, size_t n) {
+        if (st == DEAC_OK) st = h->alloc(ptr, n);
+    };
+    A(h->kmat, kmat_n);
+    A(h->obs, T);
+    A(h->sig, T);
+    A(h->pop0, pop_n);
+    A(h->pop1, pop_n);
+    A(h->cur, P);
+    A(h->fitness, P);
+    A(h->cr_old, static_cast<size_t>(P) * m.nsign);
+    A(h->f_old, static_cast<size_t>(P) * m.nsign);
+    A(h->emod, static_cast<size_t>(P) * T);
+    A(h->mom, static_cast<size_t>(P) * 4);
+    A(h->status, 3);
+    A(h->bar, 1);
+    A(h->ctl, static_cast<size_t>(h->batch) * P);
+    A(h->raw, static_cast<size_t>(h->batch) * P * Kc);
+    A(h->gen_start, static_cast<size_t>(h->batch) * 4);
+    A(h->state, 4);
+    A(h->skip_rows, 256 * 4);
+    A(h->lane_tab, static_cast<size_t>(64) * 64 * 16 * 4);
+    if (!m.nterm.empty()) A(h->nterm, Kc);
+    if (!m.fterm.empty()) A(h->fterm, Kc);
+    if (!m.tterm.empty()) A(h->tterm, Kc);
+    if (st != DEAC_OK) return fail(st);
+
+    // K in the chain layout [tau-block][pair][8][2], zero padded.
+    std::vector<double> kl(kmat_n, 0.0);
+    for (size_t t = 0; t < m.T; t++)
+        for (size_t k = 0; k < m.Kc; k++)
+            kl[(((t >> 3) * h->Kcp + (k >> 1)) * 8 + (t & 7)) * 2 + (k & 1)] = m.kernel[t * m.Kc + k];
+    // Jump tables: T^(nsign*G) by rows for the walk, T^(l*chunk) by nibbles
+    // for the mask generator.
+    std::vector<uint64_t> rows(256 * 4), tabs(static_cast<size_t>(64) * 64 * 16 * 4);
+    deac::matrix_rows(deac::transition_power(static_cast<uint64_t>(Kc)), rows.data());
+    const deac::Gf2Matrix step = deac::transition_power(static_cast<uint64_t>(h->chunk));
+    deac::Gf2Matrix acc = deac::Gf2Matrix::identity();
+    for (int l = 0; l < 64; l++) {
+        deac::nibble_table(acc, tabs.data() + static_cast<size_t>(l) * 64 * 16 * 4);
+        acc = step * acc;
+    }
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        if (st == DEAC_OK && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) st = DEAC_ERR_DEVICE;
+    };
+    up(h->kmat, kl.data(), kl.size() * sizeof(double));
+    up(h->obs, m.isf.data(), m.T * sizeof(double));
+    up(h->sig, m.err.data(), m.T * sizeof(double));
+    up(h->skip_rows, rows.data(), rows.size() * sizeof(uint64_t));
+    up(h->lane_tab, tabs.data(), tabs.size() * sizeof(uint64_t));
+    if (h->nterm) up(h->nterm, m.nterm.data(), m.Kc * sizeof(double));
+    if (h->fterm) up(h->fterm, m.fterm.data(), m.Kc * sizeof(double));
+    if (h->tterm) up(h->tterm, m.tterm.data(), m.Kc * sizeof(double));
+    if (st != DEAC_OK) return fail(st);
+    *out = h;
+    return DEAC_OK;
+}
+
+void deac_destroy(deac_handle *h) { delete h; }
+
+deac_status deac_last_run_timing(const deac_handle *h, double *loop_ms, size_t *generations)
+{
+    if (!h) return DEAC_ERR_INVALID_ARGUMENT;
+    if (loop_ms) *loop_ms = h->last_loop_ms;
+    if (generations) *generations = h->last_generations;
+    return DEAC_OK;
+}
+
+deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_options *o, deac_result *r)
+{
+    if (!h || !rng_state || !o || !r || !r->dsf || !r->frequency) return DEAC_ERR_INVALID_ARGUMENT;
+    if (o->number_of_generations < 2) {
+        g_message = "number_of_generations must be at least 2";
+        return DEAC_ERR_INVALID_ARGUMENT;
+    }
+    const deac::Model &m = h->m;
+    const size_t N = o->number_of_generations, P = m.P, Kc = m.Kc;
+    const int nsign = m.nsign;
+    hipStream_t s = h->stream;
+    DEAC_HIP(hipSetDevice(h->device));
+
+    // Initial population and normalization on the host (deac.cpp:430-603).
+    deac::Xoshiro rng{{rng_state[0], rng_state[1], rng_state[2], rng_state[3]}};
+    std::vector<double> pop, lay;
+    deac::initial_population(m, rng, pop);
+    interleave_population(pop, P, Kc, h->Kcp, lay);
+    std::vector<double> zeros(lay.size(), 0.0);
+    std::vector<int> cur(P, 0);
+    std::vector<double> cr(P * nsign, m.cr0), fw(P * nsign, m.f0);  // deac.cpp:990-1032
+
+    double *stats = nullptr;
+    const bool track = o->track_stats != 0;
+    if (track && h->alloc(stats, 3 * N) != DEAC_OK) return DEAC_ERR_OUT_OF_MEMORY;
+    void *stats_alloc = stats;
+
+    DEAC_HIP(hipMemcpyAsync(h->pop0, lay.data(), lay.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    DEAC_HIP(hipMemcpyAsync(h->pop1, zeros.data(), zeros.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    DEAC_HIP(hipMemcpyAsync(h->cur, cur.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
+    DEAC_HIP(hipMemcpyAsync(h->cr_old, cr.data(), cr.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    DEAC_HIP(hipMemcpyAsync(h->f_old, fw.data(), fw.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    DEAC_HIP(hipMemcpyAsync(h->state, rng.s, sizeof rng.s, hipMemcpyHostToDevice, s));
+    const int zero3[3] = {0, 0, 0};
+    DEAC_HIP(hipMemcpyAsync(h->status, zero3, sizeof zero3, hipMemcpyHostToDevice, s));
+
+    deac::EvoArgs ea{};
+    ea.P = static_cast<int>(P);
+    ea.T = static_cast<int>(m.T);
+    ea.G = static_cast<int>(m.G);
+    ea.nsign = nsign;
+    ea.Kc = static_cast<int>(Kc);
+    ea.Kcp = h->Kcp;
+    ea.MB = h->MB;
+    ea.TB = h->TB;
+    ea.use_first = m.use_first;
+    ea.use_third = m.use_third;
+    ea.renorm = m.normalize && m.renormalize;
+    ea.allow_neg = m.allow_negative;
+    ea.track_stats = track;
+    ea.stop = o->stop_minimum_fitness;
+    ea.zeroth = m.zeroth_moment;
+    ea.first_moment = m.first_moment;
+    ea.third_moment = m.third_moment;
+    ea.third_moment_error = m.third_moment_error;
+    ea.kmat = h->kmat;
+    ea.nterm = h->nterm;
+    ea.fterm = h->fterm;
+    ea.tterm = h->tterm;
+    ea.obs = h->obs;
+    ea.sig = h->sig;
+    ea.pop0 = h->pop0;
+    ea.pop1 = h->pop1;
+    ea.cur = h->cur;
+    ea.fitness = h->fitness;
+    ea.cr_old = h->cr_old;
+    ea.f_old = h->f_old;
+    ea.emod = h->emod;
+    ea.mom = h->mom;
+    ea.ctl = h->ctl;
+    ea.raw = h->raw;
+    ea.stats = stats;
+    ea.stats_len = N;
+    ea.bar = h->bar;
+    ea.status = h->status;
+
+    // Initial fitness (deac.cpp:857-976).
+    ea.init_mode = 1;
+    ea.gen0 = 0;
+    ea.ngen = 0;
+    DEAC_HIP(deac::launch_zero_u32(h->bar, 1, s));
+    DEAC_HIP(deac::launch_evolve(ea, h->nwg, s));
+    ea.init_mode = 0;
+
+    deac::WalkArgs wa{};
+    wa.state = h->state;
+    wa.gen_start = h->gen_start;
+    wa.ctl = h->ctl;
+    wa.skip_rows = h->skip_rows;
+    wa.P = static_cast<int>(P);
+    wa.nsign = nsign;
+    wa.mod_magic = ~UINT64_C(0) / P;
+    wa.tau_cr = m.tau_cr;
+    wa.tau_f = m.tau_f;
+    deac::MaskArgs ma{};
+    ma.ctl = h->ctl;
+    ma.lane_tab = h->lane_tab;
+    ma.raw = h->raw;
+    ma.P = static_cast<int>(P);
+    ma.Kc = static_cast<int>(Kc);
+    ma.chunk = h->chunk;
+
+    // Generation loop: ii = 0 .. N-2 (deac.cpp:1138), in batches.
+    DEAC_HIP(hipEventRecord(h->ev0, s));
+    size_t done = 0;          // generations evolved
+    long stopped_at = -1;     // generation index of the stop, if any
+    const size_t total = N - 1;
+    while (done < total) {
+        const int nb = static_cast<int>(std::min<size_t>(h->batch, total - done));
+        wa.ngen = nb;
+        ma.ngen = nb;
+        ea.gen0 = static_cast<long>(done);
+        ea.ngen = nb;
+        DEAC_HIP(deac::launch_walk(wa, s));
+        DEAC_HIP(deac::launch_mask(ma, s));
+        DEAC_HIP(deac::launch_zero_u32(h->bar, 1, s));
+        DEAC_HIP(deac::launch_evolve(ea, h->nwg, s));
+        int status[3];
+        DEAC_HIP(hipMemcpyAsync(status, h->status, sizeof status, hipMemcpyDeviceToHost, s));
+        DEAC_HIP(hipStreamSynchronize(s));
+        if (status[2]) return DEAC_ERR_DEVICE;  // a grid barrier timed out
+        if (status[0]) {
+            stopped_at = static_cast<long>(done) + status[1];
+            uint64_t st4[4];
+            DEAC_HIP(hipMemcpy(st4, h->gen_start + static_cast<size_t>(status[1]) * 4, sizeof st4,
+                               hipMemcpyDeviceToHost));
+            std::memcpy(rng.s, st4, sizeof st4);
+            done += static_cast<size_t>(status[1]);
+            break;
+        }
+        done += static_cast<size_t>(nb);
+    }
+    DEAC_HIP(hipEventRecord(h->ev1, s));
+    DEAC_HIP(hipEventSynchronize(h->ev1));
+    float ms = 0.0f;
+    DEAC_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_loop_ms = ms;
+    h->last_generations = done;
+    if (stopped_at < 0) DEAC_HIP(hipMemcpy(rng.s, h->state, sizeof rng.s, hipMemcpyDeviceToHost));
+    std::memcpy(rng_state, rng.s, sizeof rng.s);
+
+    // Epilogue on the host (deac.cpp:1576-1685).
+    std::vector<double> fit(P), l0(lay.size()), l1(lay.size());
+    DEAC_HIP(hipMemcpy(fit.data(), h->fitness, P * sizeof(double), hipMemcpyDeviceToHost));
+    DEAC_HIP(hipMemcpy(cur.data(), h->cur, P * sizeof(int), hipMemcpyDeviceToHost));
+    DEAC_HIP(hipMemcpy(l0.data(), h->pop0, l0.size() * sizeof(double), hipMemcpyDeviceToHost));
+    DEAC_HIP(hipMemcpy(l1.data(), h->pop1, l1.size() * sizeof(double), hipMemcpyDeviceToHost));
+    size_t generation = stopped_at >= 0 ? static_cast<size_t>(stopped_at) : N - 2;
+    size_t best = 0;
+    double minimum = fit[0];
+    for (size_t i = 0; i < P; i++)
+        if (fit[i] < minimum) {
+            minimum = fit[i];
+            best = i;
+        }
+    std::vector<double> genome(Kc);
+    extract_member(cur[best] ? l1 : l0, best, Kc, h->Kcp, genome.data());
+    deac::output_transform(m, genome.data(), r->dsf, r->frequency);
+
+    if (track) {
+        std::vector<double> host_stats(3 * N, 0.0);
+        const size_t n = generation + 1;
+        DEAC_HIP(hipMemcpy(host_stats.data(), stats, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+        if (r->stats_mean) std::memcpy(r->stats_mean, host_stats.data(), n * sizeof(double));
+        if (r->stats_minimum) std::memcpy(r->stats_minimum, host_stats.data() + N, n * sizeof(double));
+        if (r->stats_squared_mean) std::memcpy(r->stats_squared_mean, host_stats.data() + 2 * N, n * sizeof(double));
+    }
+    if (generation + 2 == N) {  // deac.cpp:1677-1685
+        generation += 1;
+        if (track) {
+            if (r->stats_mean) r->stats_mean[generation] = deac::mean_of(fit.data(), P);
+            if (r->stats_minimum) r->stats_minimum[generation] = minimum;
+            if (r->stats_squared_mean) r->stats_squared_mean[generation] = deac::squared_mean_of(fit.data(), P);
+        }
+    }
+    r->generation = generation;
+    r->minimum_fitness = minimum;
+    r->best_index = best;
+    if (r->final_fitness) std::memcpy(r->final_fitness, fit.data(), P * sizeof(double));
+    if (stats_alloc) {
+        h->allocations.erase(std::find(h->allocations.begin(), h->allocations.end(), stats_alloc));
+        (void)hipFree(stats_alloc);
+    }
+    return DEAC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Device-side interface of the DEAC engine: argument blocks and launchers of
+// the gfx950 kernels in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace deac {
+
+// Members per member-block and timeslices per tau-block of the exact FP64
+// model-ISF contraction: one wavefront = 8 members x 8 timeslices, one
+// sequential FMA chain per lane.
+constexpr int kMemberBlock = 8;
+constexpr int kTauBlock = 8;
+// Genome pairs processed per step of a chain (loads are issued one batch
+// ahead); the genome is padded to a multiple of this many pairs.
+constexpr int kPairBatch = 8;
+
+// Random choices of one member in one generation, produced by walk_kernel.
+struct MemberCtl {
+    uint64_t mask_state[4];  // xoshiro256+ state at the member's first crossover-mask draw
+    uint32_t mutant[3];      // donor indices r1, r2, r3 (deac.cpp:200-216)
+    uint32_t flags;          // bit 2s: CR of sign s redrawn; bit 2s+1: F of sign s redrawn
+    double cr_value[2];      // redrawn CR per sign
+    double f_value[2];       // redrawn F per sign
+};
+
+struct WalkArgs {
+    uint64_t *state;             // in/out, 4 words
+    uint64_t *gen_start;         // [ngen][4] state at the top of each generation
+    MemberCtl *ctl;              // [ngen][P]
+    const uint64_t *skip_rows;   // rows of T^(nsign*G), 256 x 4 words
+    int ngen, P, nsign;
+    uint64_t mod_magic;          // floor((2^64-1)/P)
+    double tau_cr, tau_f;
+};
+
+struct MaskArgs {
+    const MemberCtl *ctl;        // [ngen][P]
+    const uint64_t *lane_tab;    // [64][64][16][4] nibble tables of T^(lane*chunk)
+    uint64_t *raw;               // [ngen][P][Kc] 53-bit draws (x >> 11)
+    int ngen, P, Kc, chunk;
+};
+
+struct EvoArgs {
+    int P, T, G, nsign, Kc, Kcp, MB, TB;   // Kcp: genome pairs, padded to kPairBatch
+    int use_first, use_third, renorm, allow_neg, init_mode, track_stats;
+    double stop, zeroth, first_moment, third_moment, third_moment_error;
+    const double *kmat;          // [TB][Kcp][8][2]
+    const double *nterm, *fterm, *tterm;   // [Kc] or null
+    const double *obs, *sig;     // [T]
+    double *pop0, *pop1;         // [MB][Kcp][8][2] each
+    int *cur;                    // [P] buffer holding each member's current genome
+    double *fitness;             // [P]
+    double *cr_old, *f_old;      // [P][nsign]
+    double *emod;                // [P][T]
+    double *mom;                 // [P][4]: normalization, first, third moment
+    const MemberCtl *ctl;        // [ngen][P]
+    const uint64_t *raw;         // [ngen][P][Kc]
+    double *stats;               // [3][stats_len]: mean, minimum, squared mean
+    size_t stats_len;
+    unsigned *bar;               // grid barrier counter, zero at launch
+    int *status;                 // [0] stopped, [1] generation index within the batch, [2] barrier timeout
+    long gen0;                   // generation index of the batch's first generation
+    int ngen;
+};
+
+hipError_t launch_walk(const WalkArgs &a, hipStream_t s);
+hipError_t launch_mask(const MaskArgs &a, hipStream_t s);
+hipError_t launch_evolve(const EvoArgs &a, int nwg, hipStream_t s);
+hipError_t launch_zero_u32(unsigned *p, size_t n, hipStream_t s);
+// Workgroups of the evolution kernel that fit co-resident on the device.
+int evolve_max_workgroups(int device);
+
+}  // namespace deac

@@ -1,0 +1,206 @@
+"""ctypes front end of libdeac_amd.so (the C ABI in include/deac_engine.h).
+
+Mirrors the reference's deac() call (src/deac/src/deac.cpp:218-229): one call
+runs a whole reconstruction and returns the best-member DSF, the mirrored
+frequency grid, the generation count, the best fitness and the per-generation
+statistics that --track_stats writes.  The library is required: importing
+this module without a built libdeac_amd.so raises, there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdeac_amd.so")
+
+MODELS = {"standard": 0, "hyperbolic": 1, "normalization": 2}
+SPECTRA = {"spbsf": 0, "spfsf": 1, "bfull": 2, "ffull": 3}
+STATUS = {0: "success", 1: "invalid argument", 2: "no usable gfx950 device", 3: "out of memory",
+          4: "device error", 5: "unsupported"}
+
+EXPORTED = ["deac_create", "deac_run", "deac_destroy", "deac_output_length", "deac_seed_state",
+            "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_run_timing"]
+
+_dp = C.POINTER(C.c_double)
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("number_of_timeslices", C.c_size_t),
+        ("imaginary_time", _dp),
+        ("isf", _dp),
+        ("isf_error", _dp),
+        ("genome_size", C.c_size_t),
+        ("frequency", _dp),
+        ("temperature", C.c_double),
+        ("model", C.c_int),
+        ("spectra_type", C.c_int),
+        ("zero_temperature", C.c_int),
+        ("allow_negative_spectral_weight", C.c_int),
+        ("renormalize_each_generation", C.c_int),
+        ("population_size", C.c_size_t),
+        ("normalize", C.c_int),
+        ("first_moment", C.c_double),
+        ("third_moment", C.c_double),
+        ("third_moment_error", C.c_double),
+        ("crossover_probability", C.c_double),
+        ("self_adapting_crossover_probability", C.c_double),
+        ("differential_weight", C.c_double),
+        ("self_adapting_differential_weight_probability", C.c_double),
+    ]
+
+
+class RunOptions(C.Structure):
+    _fields_ = [("number_of_generations", C.c_size_t), ("stop_minimum_fitness", C.c_double),
+                ("track_stats", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("generation", C.c_size_t),
+        ("minimum_fitness", C.c_double),
+        ("best_index", C.c_size_t),
+        ("dsf", _dp),
+        ("frequency", _dp),
+        ("stats_mean", _dp),
+        ("stats_minimum", _dp),
+        ("stats_squared_mean", _dp),
+        ("final_fitness", _dp),
+    ]
+
+
+class DeacError(RuntimeError):
+    def __init__(self, status, message=None):
+        self.status = status
+        super().__init__(f"{STATUS.get(status, status)}" + (f": {message}" if message else ""))
+
+
+_lib = None
+
+
+def lib():
+    """Load libdeac_amd.so; raise if it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libdeac_amd.so not built at {LIB_PATH}; run make -C deac-cpp_amd")
+        L = C.CDLL(LIB_PATH)
+        L.deac_create.argtypes = [C.POINTER(Problem), C.POINTER(C.c_void_p), C.POINTER(C.c_char_p)]
+        L.deac_create.restype = C.c_int
+        L.deac_run.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOptions), C.POINTER(Result)]
+        L.deac_run.restype = C.c_int
+        L.deac_destroy.argtypes = [C.c_void_p]
+        L.deac_output_length.argtypes = [C.POINTER(Problem)]
+        L.deac_output_length.restype = C.c_size_t
+        L.deac_seed_state.argtypes = [C.c_uint64, C.POINTER(C.c_uint64)]
+        L.deac_rng_advance.argtypes = [C.POINTER(C.c_uint64), C.c_uint64]
+        L.deac_frequency_grid.argtypes = [C.c_size_t, C.c_double, _dp]
+        L.deac_frequency_grid.restype = C.c_int
+        L.deac_status_string.argtypes = [C.c_int]
+        L.deac_status_string.restype = C.c_char_p
+        L.deac_last_run_timing.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_size_t)]
+        L.deac_last_run_timing.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def seed_state(seed):
+    s = (C.c_uint64 * 4)()
+    lib().deac_seed_state(C.c_uint64(int(seed)), s)
+    return s
+
+
+def rng_advance(state, n):
+    s = (C.c_uint64 * 4)(*[int(v) for v in state])
+    lib().deac_rng_advance(s, C.c_uint64(int(n)))
+    return [int(v) for v in s]
+
+
+def frequency_grid(genome_size, omega_max):
+    f = np.empty(genome_size)
+    st = lib().deac_frequency_grid(genome_size, omega_max, _ptr(f))
+    if st:
+        raise DeacError(st)
+    return f
+
+
+def make_problem(tau, isf, err, frequency, temperature, *, model="standard", spectra_type="bfull",
+                 zero_temperature=False, allow_negative=False, renormalize=False, population_size=8,
+                 normalize=False, first_moment=-1.0, third_moment=-1.0, third_moment_error=0.0,
+                 crossover_probability=0.9, self_adapting_crossover_probability=0.1,
+                 differential_weight=0.9, self_adapting_differential_weight_probability=0.1):
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (tau, isf, err, frequency)]
+    p = Problem(len(keep[0]), _ptr(keep[0]), _ptr(keep[1]), _ptr(keep[2]), len(keep[3]), _ptr(keep[3]),
+                temperature, MODELS[model], SPECTRA[spectra_type], int(zero_temperature), int(allow_negative),
+                int(renormalize), population_size, int(normalize), first_moment, third_moment,
+                third_moment_error, crossover_probability, self_adapting_crossover_probability,
+                differential_weight, self_adapting_differential_weight_probability)
+    p._keep = keep
+    return p
+
+
+class Engine:
+    """One deac_handle: problem uploaded once, any number of runs."""
+
+    def __init__(self, problem):
+        self.problem = problem
+        h = C.c_void_p()
+        msg = C.c_char_p()
+        st = lib().deac_create(C.byref(problem), C.byref(h), C.byref(msg))
+        if st:
+            raise DeacError(st, msg.value.decode() if msg.value else None)
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().deac_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, seed, number_of_generations, stop_minimum_fitness=1.0, track_stats=True):
+        """Run from --seed (int) or an explicit 4-word RNG state."""
+        p = self.problem
+        N = number_of_generations
+        nout = lib().deac_output_length(C.byref(p))
+        out = {k: np.zeros(n) for k, n in (("dsf", nout), ("frequency", nout), ("mean", N), ("minimum", N),
+                                            ("squared_mean", N), ("fitness", p.population_size))}
+        r = Result(0, 0.0, 0, _ptr(out["dsf"]), _ptr(out["frequency"]), _ptr(out["mean"]), _ptr(out["minimum"]),
+                   _ptr(out["squared_mean"]), _ptr(out["fitness"]))
+        if isinstance(seed, (int, np.integer)):
+            s = seed_state(seed)
+        else:
+            s = (C.c_uint64 * 4)(*[int(v) for v in seed])
+        o = RunOptions(N, stop_minimum_fitness, int(track_stats))
+        st = lib().deac_run(self.handle, s, C.byref(o), C.byref(r))
+        if st:
+            raise DeacError(st)
+        n = r.generation + 1
+        for k in ("mean", "minimum", "squared_mean"):
+            out[k] = out[k][:n]
+        ms = C.c_double()
+        gens = C.c_size_t()
+        lib().deac_last_run_timing(self.handle, C.byref(ms), C.byref(gens))
+        out.update(generation=r.generation, minimum_fitness=r.minimum_fitness, best_index=r.best_index,
+                   rng=[int(v) for v in s], loop_ms=ms.value, loop_generations=gens.value)
+        return out
+
+
+def run(problem, seed, number_of_generations, stop_minimum_fitness=1.0, track_stats=True):
+    with Engine(problem) as e:
+        return e.run(seed, number_of_generations, stop_minimum_fitness, track_stats)

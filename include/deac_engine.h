@@ -1,0 +1,146 @@
+/*
+ * deac_engine.h -- C ABI of the MI355X DEAC engine (libdeac_amd.so).
+ *
+ * The engine runs the DEAC generation loop on one gfx950 GPU.  It replaces
+ * two levels of nscottnichols/deac-cpp 2.0.0-rc1 at once:
+ *   - the function level: void deac(...) in src/deac/src/deac.cpp:218-229,
+ *     whose loop is deac.cpp:1137-1574 (C++ signature, std::string/fs::path
+ *     arguments, errors by exit());
+ *   - the operator level: the gpu_* launchers of src/deac/include/
+ *     deac_gpu.hip.hpp:651-761 (declared for CUDA in deac_gpu.cuh:6-28),
+ *     about 25 launches and 12 stream syncs per generation.
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.
+ *
+ * Ownership: the caller owns every input and output buffer.  The engine
+ * copies what it needs in deac_create and owns all device memory.
+ * Errors: every entry point returns a deac_status; none calls exit().
+ * Threading: a handle is used by one host thread at a time; distinct handles
+ * are independent.
+ */
+#ifndef DEAC_ENGINE_H
+#define DEAC_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DEAC_ENGINE_VERSION "0.1.0"
+
+typedef enum deac_status {
+    DEAC_OK = 0,
+    DEAC_ERR_INVALID_ARGUMENT = 1,  /* a problem field fails the reference's validation */
+    DEAC_ERR_NO_DEVICE = 2,         /* no usable gfx950 device */
+    DEAC_ERR_OUT_OF_MEMORY = 3,     /* device or host allocation failed */
+    DEAC_ERR_DEVICE = 4,            /* a HIP call or kernel launch failed */
+    DEAC_ERR_UNSUPPORTED = 5        /* a combination this build does not run */
+} deac_status;
+
+/* Spectral model (reference: compile-time -DDEAC_MODEL, CMakeLists.txt:97-109;
+ * here a per-problem runtime choice). */
+typedef enum deac_model {
+    DEAC_MODEL_STANDARD = 0,
+    DEAC_MODEL_HYPERBOLIC = 1,
+    DEAC_MODEL_NORMALIZATION = 2
+} deac_model;
+
+/* Spectra type (reference: --spectra_type, deac.cpp:2040-2058). */
+typedef enum deac_spectra_type {
+    DEAC_SPBSF = 0,
+    DEAC_SPFSF = 1,
+    DEAC_BFULL = 2,
+    DEAC_FFULL = 3
+} deac_spectra_type;
+
+/* Everything deac() takes except the RNG, the generation count, the stop
+ * threshold, stats tracking and the file names (deac.cpp:218-229). */
+typedef struct deac_problem {
+    size_t number_of_timeslices;          /* T >= 2 */
+    const double *imaginary_time;         /* T values */
+    const double *isf;                    /* T values, F(tau) */
+    const double *isf_error;              /* T values, > 0 */
+    size_t genome_size;                   /* G >= 2 */
+    const double *frequency;              /* G values, finite, >= 0, sorted */
+    double temperature;                   /* > 0 unless zero_temperature */
+    int model;                            /* deac_model */
+    int spectra_type;                     /* deac_spectra_type */
+    int zero_temperature;                 /* reference ZeroT build: kernel e^{-tau w}, one sign */
+    int allow_negative_spectral_weight;   /* reference ALLOW_NEGATIVE_SPECTRAL_WEIGHT */
+    int renormalize_each_generation;      /* reference SINGLE_PARTICLE_FERMIONIC_SPECTRAL_FUNCTION */
+    size_t population_size;               /* P >= 4 */
+    int normalize;                        /* --normalize */
+    double first_moment;                  /* --first_moment, < 0 disables */
+    double third_moment;                  /* --third_moment, < 0 disables */
+    double third_moment_error;            /* > 0 when third_moment is used */
+    double crossover_probability;         /* -r, initial CR */
+    double self_adapting_crossover_probability;           /* -u, tau_CR */
+    double differential_weight;                           /* -F, initial F */
+    double self_adapting_differential_weight_probability; /* -v, tau_F */
+} deac_problem;
+
+typedef struct deac_run_options {
+    size_t number_of_generations;         /* N >= 2; at most N-1 evolution steps */
+    double stop_minimum_fitness;          /* stop when the best fitness <= this */
+    int track_stats;                      /* fill the three stats arrays */
+} deac_run_options;
+
+/* Outputs of one reconstruction.  All arrays are caller-allocated. */
+typedef struct deac_result {
+    size_t generation;           /* as printed by the reference (deac.cpp:1761) */
+    double minimum_fitness;      /* best final fitness (deac.cpp:1592) */
+    size_t best_index;           /* member holding it (first on ties) */
+    double *dsf;                 /* deac_output_length() values */
+    double *frequency;           /* deac_output_length() values, mirrored grid */
+    double *stats_mean;          /* N values when track_stats (only generation+1 written), or NULL */
+    double *stats_minimum;       /* per-generation best fitness history, or NULL */
+    double *stats_squared_mean;  /* or NULL */
+    double *final_fitness;       /* P values, or NULL */
+} deac_result;
+
+typedef struct deac_handle deac_handle;
+
+/* Validates the problem exactly as the reference's main() does
+ * (deac.cpp:2060-2160), builds the kernel matrix K(tau,w), the normalization
+ * and moment vectors on the host in FP64 (deac.cpp:303-827) and uploads them
+ * to the current HIP device.  On DEAC_ERR_INVALID_ARGUMENT, *message (if not
+ * NULL) points to the reference's error text; it stays valid until the next
+ * call on this thread. */
+deac_status deac_create(const deac_problem *problem, deac_handle **out, const char **message);
+
+/* Runs one reconstruction: initial population from rng_state (in/out, the
+ * 4-word xoshiro256+ state of rng.hpp:21-23), optional normalization, then at
+ * most N-1 generations of the fused GPU pipeline.  Plain FP64: reproduces the
+ * reference CPU build's trajectory.  Replaces deac.cpp:430-1685. */
+deac_status deac_run(deac_handle *handle, uint64_t rng_state[4],
+                     const deac_run_options *options, deac_result *result);
+
+void deac_destroy(deac_handle *handle);
+
+/* Length of the dsf/frequency outputs: 2G-1, or G at zero temperature
+ * (deac.cpp:1583-1591). */
+size_t deac_output_length(const deac_problem *problem);
+
+/* xoshiro256+ state for a --seed value: seeded with 1407513600 + seed
+ * (deac.cpp:2089-2091, rng.hpp:46-59). */
+void deac_seed_state(uint64_t seed, uint64_t state[4]);
+
+/* Advances a xoshiro256+ state by n draws in O(log n) GF(2) steps
+ * (generalises xoshiro256p_jump, rng.hpp:64-87, to any distance). */
+void deac_rng_advance(uint64_t state[4], uint64_t n);
+
+/* The default frequency grid, frequency[i] = i*(omega_max/(G-1))
+ * (deac.cpp:2120-2125). */
+deac_status deac_frequency_grid(size_t genome_size, double omega_max, double *frequency);
+
+const char *deac_status_string(deac_status status);
+
+/* Per-run timing of the last deac_run on this handle (device time of the
+ * generation loop in milliseconds and the number of generations it ran). */
+deac_status deac_last_run_timing(const deac_handle *handle, double *loop_ms, size_t *generations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
