@@ -28,7 +28,7 @@ struct deac_handle {
     deac::Model m;
     int device = 0;
     hipStream_t stream = nullptr;
-    int MB = 0, TB = 0, Kcp = 0, nwg = 0, chunk = 0, batch = 0;
+    int MG = 0, TB = 0, stride = 0, nwg = 0, chunk = 0, batch = 0;
     std::vector<void *> allocations;
     double *kmat = nullptr, *nterm = nullptr, *fterm = nullptr, *tterm = nullptr, *obs = nullptr, *sig = nullptr;
     double *pop0 = nullptr, *pop1 = nullptr, *fitness = nullptr, *cr_old = nullptr, *f_old = nullptr;
@@ -38,8 +38,8 @@ struct deac_handle {
     deac::MemberCtl *ctl = nullptr;
     uint64_t *raw = nullptr, *gen_start = nullptr, *state = nullptr, *skip_rows = nullptr, *lane_tab = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_loop_ms = 0.0;
-    size_t last_generations = 0;
+    hipEvent_t kev[6] = {};      // walk, mask, evolve boundaries of one batch
+    deac_run_profile last{};
 
     ~deac_handle()
     {
@@ -47,6 +47,8 @@ struct deac_handle {
         for (void *p : allocations) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : kev)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -78,19 +80,12 @@ namespace {
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-// Interleaved population layout [member-block][pair][8][2] from [P][Kc].
-void interleave_population(const std::vector<double> &pop, size_t P, size_t Kc, int Kcp, std::vector<double> &out)
+// Device population layout: member-major rows of `stride` doubles, rows for
+// MG*4 members, zero padded.
+void pad_rows(const std::vector<double> &pop, size_t P, size_t Kc, int MG, int stride, std::vector<double> &out)
 {
-    const int MB = static_cast<int>((P + deac::kMemberBlock - 1) / deac::kMemberBlock);
-    out.assign(static_cast<size_t>(MB) * Kcp * 16, 0.0);
-    for (size_t i = 0; i < P; i++)
-        for (size_t k = 0; k < Kc; k++)
-            out[(((i >> 3) * Kcp + (k >> 1)) * 8 + (i & 7)) * 2 + (k & 1)] = pop[i * Kc + k];
-}
-
-void extract_member(const std::vector<double> &lay, size_t i, size_t Kc, int Kcp, double *out)
-{
-    for (size_t k = 0; k < Kc; k++) out[k] = lay[(((i >> 3) * Kcp + (k >> 1)) * 8 + (i & 7)) * 2 + (k & 1)];
+    out.assign(static_cast<size_t>(MG) * deac::kChainRows * stride, 0.0);
+    for (size_t i = 0; i < P; i++) std::memcpy(&out[i * stride], &pop[i * Kc], Kc * sizeof(double));
 }
 
 }  // namespace
@@ -165,11 +160,13 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
     };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(DEAC_ERR_DEVICE);
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return fail(DEAC_ERR_DEVICE);
+    for (hipEvent_t &e : h->kev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(DEAC_ERR_DEVICE);
 
     const int T = static_cast<int>(m.T), P = static_cast<int>(m.P), Kc = static_cast<int>(m.Kc);
-    h->MB = (P + deac::kMemberBlock - 1) / deac::kMemberBlock;
-    h->TB = (T + deac::kTauBlock - 1) / deac::kTauBlock;
-    h->Kcp = round_up((Kc + 1) / 2, deac::kPairBatch);
+    h->MG = (P + deac::kChainRows - 1) / deac::kChainRows;
+    h->TB = (T + deac::kChainTau - 1) / deac::kChainTau;
+    h->stride = round_up(Kc, deac::kChunk);
     h->chunk = (Kc + 63) / 64;
     // RNG batch: generations whose mask draws are buffered at once (<= 256 MiB).
     const size_t per_gen = static_cast<size_t>(P) * Kc * sizeof(uint64_t);
@@ -177,13 +174,13 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
 
     const int maxwg = deac::evolve_max_workgroups(device);
     if (maxwg <= 0) return fail(DEAC_ERR_DEVICE);
-    int nwg = std::max(h->MB * h->TB, 32);
+    int nwg = std::max(h->MG * h->TB, 32);
     if (const char *e = std::getenv("DEAC_EVOLVE_WORKGROUPS")) nwg = std::atoi(e);
     h->nwg = std::clamp(nwg, 1, maxwg);
 
     // Device buffers.
-    const size_t kmat_n = static_cast<size_t>(h->TB) * h->Kcp * 16;
-    const size_t pop_n = static_cast<size_t>(h->MB) * h->Kcp * 16;
+    const size_t kmat_n = static_cast<size_t>(h->TB) * deac::kChainTau * h->stride;
+    const size_t pop_n = static_cast<size_t>(h->MG) * deac::kChainRows * h->stride;
     deac_status st = DEAC_OK;
     auto A = [&](auto *&ptr, size_t n) {
         if (st == DEAC_OK) st = h->alloc(ptr, n);
@@ -212,11 +209,9 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
     if (!m.tterm.empty()) A(h->tterm, Kc);
     if (st != DEAC_OK) return fail(st);
 
-    // K in the chain layout [tau-block][pair][8][2], zero padded.
+    // K rows [tau][stride], zero padded to whole tau blocks and chunks.
     std::vector<double> kl(kmat_n, 0.0);
-    for (size_t t = 0; t < m.T; t++)
-        for (size_t k = 0; k < m.Kc; k++)
-            kl[(((t >> 3) * h->Kcp + (k >> 1)) * 8 + (t & 7)) * 2 + (k & 1)] = m.kernel[t * m.Kc + k];
+    for (size_t t = 0; t < m.T; t++) std::memcpy(&kl[t * h->stride], &m.kernel[t * m.Kc], m.Kc * sizeof(double));
     // Jump tables: T^(nsign*G) by rows for the walk, T^(l*chunk) by nibbles
     // for the mask generator.
     std::vector<uint64_t> rows(256 * 4), tabs(static_cast<size_t>(64) * 64 * 16 * 4);
@@ -245,11 +240,10 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
 
 void deac_destroy(deac_handle *h) { delete h; }
 
-deac_status deac_last_run_timing(const deac_handle *h, double *loop_ms, size_t *generations)
+deac_status deac_last_run_profile(const deac_handle *h, deac_run_profile *profile)
 {
-    if (!h) return DEAC_ERR_INVALID_ARGUMENT;
-    if (loop_ms) *loop_ms = h->last_loop_ms;
-    if (generations) *generations = h->last_generations;
+    if (!h || !profile) return DEAC_ERR_INVALID_ARGUMENT;
+    *profile = h->last;
     return DEAC_OK;
 }
 
@@ -270,7 +264,7 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
     deac::Xoshiro rng{{rng_state[0], rng_state[1], rng_state[2], rng_state[3]}};
     std::vector<double> pop, lay;
     deac::initial_population(m, rng, pop);
-    interleave_population(pop, P, Kc, h->Kcp, lay);
+    pad_rows(pop, P, Kc, h->MG, h->stride, lay);
     std::vector<double> zeros(lay.size(), 0.0);
     std::vector<int> cur(P, 0);
     std::vector<double> cr(P * nsign, m.cr0), fw(P * nsign, m.f0);  // deac.cpp:990-1032
@@ -295,8 +289,8 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
     ea.G = static_cast<int>(m.G);
     ea.nsign = nsign;
     ea.Kc = static_cast<int>(Kc);
-    ea.Kcp = h->Kcp;
-    ea.MB = h->MB;
+    ea.stride = h->stride;
+    ea.MG = h->MG;
     ea.TB = h->TB;
     ea.use_first = m.use_first;
     ea.use_third = m.use_third;
@@ -357,6 +351,7 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
 
     // Generation loop: ii = 0 .. N-2 (deac.cpp:1138), in batches.
     DEAC_HIP(hipEventRecord(h->ev0, s));
+    deac_run_profile prof{};
     size_t done = 0;          // generations evolved
     long stopped_at = -1;     // generation index of the stop, if any
     const size_t total = N - 1;
@@ -366,13 +361,26 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         ma.ngen = nb;
         ea.gen0 = static_cast<long>(done);
         ea.ngen = nb;
+        DEAC_HIP(hipEventRecord(h->kev[0], s));
         DEAC_HIP(deac::launch_walk(wa, s));
+        DEAC_HIP(hipEventRecord(h->kev[1], s));
         DEAC_HIP(deac::launch_mask(ma, s));
+        DEAC_HIP(hipEventRecord(h->kev[2], s));
         DEAC_HIP(deac::launch_zero_u32(h->bar, 1, s));
+        DEAC_HIP(hipEventRecord(h->kev[3], s));
         DEAC_HIP(deac::launch_evolve(ea, h->nwg, s));
+        DEAC_HIP(hipEventRecord(h->kev[4], s));
         int status[3];
         DEAC_HIP(hipMemcpyAsync(status, h->status, sizeof status, hipMemcpyDeviceToHost, s));
         DEAC_HIP(hipStreamSynchronize(s));
+        float t_walk = 0, t_mask = 0, t_evo = 0;
+        DEAC_HIP(hipEventElapsedTime(&t_walk, h->kev[0], h->kev[1]));
+        DEAC_HIP(hipEventElapsedTime(&t_mask, h->kev[1], h->kev[2]));
+        DEAC_HIP(hipEventElapsedTime(&t_evo, h->kev[3], h->kev[4]));
+        prof.walk_ms += t_walk;
+        prof.mask_ms += t_mask;
+        prof.evolve_ms += t_evo;
+        prof.evolve_launches += 1;
         if (status[2]) return DEAC_ERR_DEVICE;  // a grid barrier timed out
         if (status[0]) {
             stopped_at = static_cast<long>(done) + status[1];
@@ -389,8 +397,9 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
     DEAC_HIP(hipEventSynchronize(h->ev1));
     float ms = 0.0f;
     DEAC_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    h->last_loop_ms = ms;
-    h->last_generations = done;
+    prof.loop_ms = ms;
+    prof.generations = done;
+    h->last = prof;
     if (stopped_at < 0) DEAC_HIP(hipMemcpy(rng.s, h->state, sizeof rng.s, hipMemcpyDeviceToHost));
     std::memcpy(rng_state, rng.s, sizeof rng.s);
 
@@ -408,9 +417,8 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
             minimum = fit[i];
             best = i;
         }
-    std::vector<double> genome(Kc);
-    extract_member(cur[best] ? l1 : l0, best, Kc, h->Kcp, genome.data());
-    deac::output_transform(m, genome.data(), r->dsf, r->frequency);
+    const std::vector<double> &lb = cur[best] ? l1 : l0;
+    deac::output_transform(m, &lb[best * h->stride], r->dsf, r->frequency);
 
     if (track) {
         std::vector<double> host_stats(3 * N, 0.0);

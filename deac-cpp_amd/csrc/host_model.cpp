@@ -290,8 +290,10 @@ Model build_model(const deac_problem &p)
                 continue;
             }
             if (m.model == DEAC_MODEL_HYPERBOLIC) {  // deac.cpp:323-324, 363-364
-                const double bo2mtp = -0.5 * beta - t;          // -0.5*beta is exact
-                const double bo2mtn = std::fma(1.5, beta, -t);  // contracted by the reference build
+                // -0.5*beta and 1.5*beta are rounded before the subtraction
+                // (loop invariants in the reference build, never fused).
+                const double bo2mtp = -0.5 * beta - t;
+                const double bo2mtn = 1.5 * beta - t;
                 kp[j] = per * df * std::exp(bo2mtp * w) / 2;
                 km[j] = per * df * std::exp(-bo2mtn * w) / 2;
             } else if (m.model == DEAC_MODEL_NORMALIZATION) {  // deac.cpp:376-384

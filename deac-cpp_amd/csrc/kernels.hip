@@ -60,13 +60,6 @@ __device__ __forceinline__ int parity64(uint64_t x)
     return __popc(static_cast<uint32_t>(x) ^ static_cast<uint32_t>(x >> 32)) & 1;
 }
 
-// Element index of (member i, genome position k) in the interleaved
-// population layout [member-block][pair][8 members][2].
-__device__ __forceinline__ size_t pidx(int i, int k, int Kcp)
-{
-    return ((static_cast<size_t>(i >> 3) * Kcp + (k >> 1)) * 8 + (i & 7)) * 2 + (k & 1);
-}
-
 // Grid-wide barrier of the cooperative evolution kernel.  The wait is
 // bounded: after kBarrierSpins polls the workgroup records a failure and every
 // caller leaves the kernel, so a broken launch cannot hang the device.
@@ -74,17 +67,20 @@ constexpr unsigned kBarrierSpins = 1u << 24;
 
 __device__ __forceinline__ bool grid_barrier(unsigned *bar, unsigned &target, int *status)
 {
-    // Every wave publishes its stores at agent scope before the arrival, so
-    // the data is visible to workgroups on other XCDs after the wait.
+    // Each wave drains its own memory operations; then one thread per
+    // workgroup publishes the workgroup's stores at agent scope (L2 write-back,
+    // visible to the other XCDs), arrives, waits, and invalidates this CU's
+    // caches for what the other workgroups wrote.
     __shared__ int timed_out;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     target += gridDim.x;
     if (threadIdx.x == 0) {
         timed_out = 0;
-        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
             if (++spins == kBarrierSpins) {
                 timed_out = 1;
@@ -92,9 +88,9 @@ __device__ __forceinline__ bool grid_barrier(unsigned *bar, unsigned &target, in
                 break;
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return timed_out == 0;
 }
 
@@ -201,79 +197,214 @@ __global__ __launch_bounds__(256) void mask_kernel(MaskArgs a)
 
 namespace {
 
+// LDS of the chain phase: kStages buffers of the kernel-matrix chunk
+// [16 tau][kChunk] and of the genome chunk [4 members][kChunk].  Rows are
+// padded by one pair so the 16 tau rows start on different banks.
+constexpr int kRowPad = kChunk + 2;
+constexpr int kKBuf = kChainTau * kRowPad;
+constexpr int kSBuf = kChainRows * kRowPad;
+constexpr int kChainLds = kStages * (kKBuf + kSBuf);
+
 // Chain of one (member, row) pair in the order of the reference's inlined
 // matrix_multiply_MxN_by_Nx1 (deac.cpp:102-108): rounded products added one at
 // a time, except the last element of each sign when G is odd, which is fused.
-__device__ double fold_row(const double *trial, int i, const double *row, int Kc, int Kcp, int G)
+__device__ double fold_row(const double *x, const double *row, int Kc, int G)
 {
     constexpr int kBatch = 16;
     double acc = 0.0;
     const bool godd = (G & 1) != 0;
     for (int k0 = 0; k0 < Kc; k0 += kBatch) {
-        double x[kBatch], r[kBatch];
+        double xv[kBatch], rv[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; u++) {
             const int k = k0 + u;
-            x[u] = k < Kc ? trial[pidx(i, k, Kcp)] : 0.0;
-            r[u] = k < Kc ? row[k] : 0.0;
+            xv[u] = k < Kc ? x[k] : 0.0;
+            rv[u] = k < Kc ? row[k] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < kBatch; u++) {
             const int k = k0 + u;
             if (k >= Kc) break;
-            if (godd && (k == G - 1 || k == Kc - 1)) acc = fma(x[u], r[u], acc);
-            else acc = acc + x[u] * r[u];
+            if (godd && (k == G - 1 || k == Kc - 1)) acc = fma(xv[u], rv[u], acc);
+            else acc = acc + xv[u] * rv[u];
         }
     }
     return acc;
 }
 
-// E[i][t] for the 8x8 (member, tau) block of this wavefront: one sequential
-// FMA chain per lane over the concatenated genome, positive sign then
-// negative sign, starting from zero (matrix_multiply_LxM_by_MxN,
-// deac.cpp:110-118, called twice at deac.cpp:1386-1394).
-__device__ void chain_block(const EvoArgs &a, int mb, int tb, int lane)
+// Genome row of member i that the chain phase reads: the current genome for
+// the initial fitness, the trial (in the spare buffer) otherwise.
+__device__ __forceinline__ const double *chain_row(const EvoArgs &a, int i, bool init)
 {
-    const int il = lane >> 3, tl = lane & 7;
-    const int i = mb * kMemberBlock + il, t = tb * kTauBlock + tl;
-    const double *src = a.pop0;
-    if (i < a.P) {
-        const int c = a.cur[i];
-        src = (a.init_mode ? c : 1 - c) ? a.pop1 : a.pop0;
+    const int c = i < a.P ? a.cur[i] : 0;
+    const bool second = init ? c != 0 : c == 0;
+    return (second ? a.pop1 : a.pop0) + static_cast<size_t>(i) * a.stride;
+}
+
+typedef __attribute__((address_space(1))) const void global_cvoid;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// 16-byte LDS-DMA copies of one chunk: one copy of 1 KB per row (kChunk
+// doubles), 16 kernel-matrix rows then 4 genome rows.
+constexpr int kCopiesPerRow = kChunk / 128;
+constexpr int kCopiesPerChunk = kCopiesPerRow * (kChainTau + kChainRows);
+// Helper waves 1..3 share the copies round-robin: 7, 7 and 6 copies; rows
+// past the last timeslice are not copied (their lanes' sums are discarded).
+static_assert(kCopiesPerChunk == 20, "wait_copies assumes 7/7/6 copies per helper wave");
+
+// Issues (does not wait for) the copies of chunk `chunk` into LDS stage
+// `stage` owned by helper wave `wid` (1..3).  Lane l moves bytes
+// [16l, 16l+16) of each 1 KB piece straight from global memory into LDS.
+__device__ __forceinline__ void stage_chunk_async(const EvoArgs &a, double *lds, const double *const *srows, int tb,
+                                                  int chunk, int stage, int wid, int lane)
+{
+    const size_t col = static_cast<size_t>(chunk) * kChunk;
+    for (int q = wid - 1; q < kCopiesPerChunk; q += 3) {
+        const int row = q / kCopiesPerRow, half = (q % kCopiesPerRow) * 128;
+        const double *src;
+        double *dst;
+        if (row < kChainTau) {
+            if (tb * kChainTau + row >= a.T) {
+                // keep the per-wave copy count fixed: a 16-byte dummy copy
+                // of the first kernel row into this (discarded) LDS row
+                src = a.kmat + col;
+                dst = lds + stage * kKBuf + row * kRowPad + half;
+                __builtin_amdgcn_global_load_lds((global_cvoid *)(src + 2 * lane), (lds_void *)(dst), 16, 0, 0);
+                continue;
+            }
+            src = a.kmat + static_cast<size_t>(tb * kChainTau + row) * a.stride + col + half;
+            dst = lds + stage * kKBuf + row * kRowPad + half;
+        } else {
+            const int r = row - kChainTau;
+            src = srows[r] + col + half;
+            dst = lds + kStages * kKBuf + stage * kSBuf + r * kRowPad + half;
+        }
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(src + 2 * lane),
+                                         (lds_void *)(dst), 16, 0, 0);
     }
-    const double2 *kp = reinterpret_cast<const double2 *>(a.kmat) + static_cast<size_t>(tb) * a.Kcp * 8 + tl;
-    const double2 *sp = reinterpret_cast<const double2 *>(src) + static_cast<size_t>(mb) * a.Kcp * 8 + il;
+}
+
+// s_waitcnt with vmcnt <= n (n < 64) and the other counters unconstrained.
+#define DEAC_WAIT_VM(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8))
+// s_waitcnt lgkmcnt(0) (LDS reads retired), other counters unconstrained.
+#define DEAC_WAIT_LDS() __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4))
+
+// Workgroup barrier for the chunk pipeline.  Unlike __syncthreads it does not
+// drain outstanding global->LDS copies; every wave retires what the others
+// depend on before arriving (see wait_copies and DEAC_WAIT_LDS).
+__device__ __forceinline__ void chunk_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+__device__ __forceinline__ void wait_copies(int wid, int pending)
+{
+    // Copies retire in order: leave the `pending` newest chunks in flight.
+    if (wid == 3) {
+        if (pending == 5) DEAC_WAIT_VM(30);
+        else if (pending == 4) DEAC_WAIT_VM(24);
+        else if (pending == 3) DEAC_WAIT_VM(18);
+        else if (pending == 2) DEAC_WAIT_VM(12);
+        else if (pending == 1) DEAC_WAIT_VM(6);
+        else DEAC_WAIT_VM(0);
+    } else {
+        if (pending == 5) DEAC_WAIT_VM(35);
+        else if (pending == 4) DEAC_WAIT_VM(28);
+        else if (pending == 3) DEAC_WAIT_VM(21);
+        else if (pending == 2) DEAC_WAIT_VM(14);
+        else if (pending == 1) DEAC_WAIT_VM(7);
+        else DEAC_WAIT_VM(0);
+    }
+}
+
+#define DEAC_FMAC_DPP(Q, K) "v_fmac_f64_dpp %0, %1, %" #K " row_newbcast:" #Q " row_mask:0xf bank_mask:0xf\n"
+
+// 16 chain steps: acc += s[row, q] * k[q] for q = 0..15 in order, where s
+// holds the row's 16 genome values (lane q of the row) and reaches every lane
+// of the row by DPP row broadcast.  One rounding per step (fused), the
+// reference's FMA chain (deac.cpp:110-118).  The s_nop covers the VALU-write
+// to DPP-read hazard on s.
+__device__ __forceinline__ double fmac16(double acc, double s, const double (&k)[16])
+{
+    asm volatile("s_nop 1\n"
+                 DEAC_FMAC_DPP(0, 2) DEAC_FMAC_DPP(1, 3) DEAC_FMAC_DPP(2, 4) DEAC_FMAC_DPP(3, 5)
+                 DEAC_FMAC_DPP(4, 6) DEAC_FMAC_DPP(5, 7) DEAC_FMAC_DPP(6, 8) DEAC_FMAC_DPP(7, 9)
+                 DEAC_FMAC_DPP(8, 10) DEAC_FMAC_DPP(9, 11) DEAC_FMAC_DPP(10, 12) DEAC_FMAC_DPP(11, 13)
+                 DEAC_FMAC_DPP(12, 14) DEAC_FMAC_DPP(13, 15) DEAC_FMAC_DPP(14, 16) DEAC_FMAC_DPP(15, 17)
+                 : "+v"(acc)
+                 : "v"(s), "v"(k[0]), "v"(k[1]), "v"(k[2]), "v"(k[3]), "v"(k[4]), "v"(k[5]), "v"(k[6]), "v"(k[7]),
+                   "v"(k[8]), "v"(k[9]), "v"(k[10]), "v"(k[11]), "v"(k[12]), "v"(k[13]), "v"(k[14]), "v"(k[15]));
+    return acc;
+}
+
+// Model ISF E[i][t] of one (tau block, member group) task; the whole
+// workgroup takes part: wave 0 runs the chains, waves 1-3 stage chunks.
+// E = sum over the concatenated genome (positive sign, then negative sign)
+// as one sequential FMA chain per (member, tau) starting from zero
+// (matrix_multiply_LxM_by_MxN called twice, deac.cpp:1386-1394).
+__device__ void chain_task(const EvoArgs &a, double *lds, int tb, int mg, bool init)
+{
+    const int wid = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int r = lane >> 4, c = lane & 15;
+    __shared__ const double *srows[kChainRows];
+    if (threadIdx.x < kChainRows) srows[threadIdx.x] = chain_row(a, mg * kChainRows + threadIdx.x, init);
+    __syncthreads();
+    const int nchunk = a.stride / kChunk;
+    // Prologue: chunks 0 .. kStages-2 in flight, chunk 0 complete.
+    if (wid > 0) {
+        const int pre = min(kStages - 1, nchunk);
+        for (int q = 0; q < pre; q++) stage_chunk_async(a, lds, srows, tb, q, q, wid, lane);
+        wait_copies(wid, pre - 1);  // chunk 0 landed
+    }
+    chunk_barrier();
     double acc = 0.0;
-    double2 kb[kPairBatch], sb[kPairBatch];
+    for (int ch = 0; ch < nchunk; ch++) {
+        if (wid == 0) {
+            const int st = ch % kStages;
+            const double *kb = lds + st * kKBuf + c * kRowPad;
+            const double *sb = lds + kStages * kKBuf + st * kSBuf + r * kRowPad + c;
+            // Operands of group g+1 are read while group g runs.
+            double k[16], kn[16];
+            double sv = sb[0], svn;
 #pragma unroll
-    for (int u = 0; u < kPairBatch; u++) {
-        kb[u] = kp[u * 8];
-        sb[u] = sp[u * 8];
-    }
-    for (int jp = kPairBatch; jp < a.Kcp; jp += kPairBatch) {
-        double2 kn[kPairBatch], sn[kPairBatch];
+            for (int u = 0; u < 8; u++) {
+                const double2 v = *reinterpret_cast<const double2 *>(kb + 2 * u);
+                k[2 * u] = v.x;
+                k[2 * u + 1] = v.y;
+            }
 #pragma unroll
-        for (int u = 0; u < kPairBatch; u++) {
-            kn[u] = kp[(jp + u) * 8];
-            sn[u] = sp[(jp + u) * 8];
+            for (int j0 = 0; j0 < kChunk; j0 += 16) {
+                const int jn = j0 + 16 < kChunk ? j0 + 16 : j0;
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const double2 v = *reinterpret_cast<const double2 *>(kb + jn + 2 * u);
+                    kn[2 * u] = v.x;
+                    kn[2 * u + 1] = v.y;
+                }
+                svn = sb[jn];
+                acc = fmac16(acc, sv, k);
+#pragma unroll
+                for (int u = 0; u < 16; u++) k[u] = kn[u];
+                sv = svn;
+            }
+            DEAC_WAIT_LDS();  // the stage may be refilled after the barrier
+        } else {
+            // Keep kStages-1 chunks ahead of the compute wave; before the
+            // barrier, chunk ch+1 must have landed.
+            const int next = ch + kStages - 1;
+            if (next < nchunk) stage_chunk_async(a, lds, srows, tb, next, next % kStages, wid, lane);
+            // chunks ch+2 .. min(next, nchunk-1) may stay in flight
+            wait_copies(wid, max(0, min(next, nchunk - 1) - (ch + 1)));
         }
-#pragma unroll
-        for (int u = 0; u < kPairBatch; u++) {
-            acc = fma(kb[u].x, sb[u].x, acc);
-            acc = fma(kb[u].y, sb[u].y, acc);
-        }
-#pragma unroll
-        for (int u = 0; u < kPairBatch; u++) {
-            kb[u] = kn[u];
-            sb[u] = sn[u];
-        }
+        chunk_barrier();
     }
-#pragma unroll
-    for (int u = 0; u < kPairBatch; u++) {
-        acc = fma(kb[u].x, sb[u].x, acc);
-        acc = fma(kb[u].y, sb[u].y, acc);
+    if (wid == 0) {
+        const int i = mg * kChainRows + r, t = tb * kChainTau + c;
+        if (i < a.P && t < a.T) a.emod[static_cast<size_t>(i) * a.T + t] = acc;
     }
-    if (i < a.P && t < a.T) a.emod[static_cast<size_t>(i) * a.T + t] = acc;
+    __syncthreads();
 }
 
 // CR and F of member i, sign sg in the current generation.
@@ -283,6 +414,11 @@ __device__ __forceinline__ void control_of(const EvoArgs &a, const MemberCtl &c,
     f = (c.flags >> (2 * sg + 1)) & 1 ? c.f_value[sg] : a.f_old[i * a.nsign + sg];
 }
 
+__device__ __forceinline__ const double *current_row(const EvoArgs &a, int i)
+{
+    return (a.cur[i] ? a.pop1 : a.pop0) + static_cast<size_t>(i) * a.stride;
+}
+
 // Trial value at genome position k of member i (deac.cpp:1261-1304).
 __device__ __forceinline__ double trial_value(const EvoArgs &a, const MemberCtl &c, const uint64_t *raw, int i, int k)
 {
@@ -290,22 +426,43 @@ __device__ __forceinline__ double trial_value(const EvoArgs &a, const MemberCtl 
     double cr, f;
     control_of(a, c, i, sg, cr, f);
     const double u = static_cast<double>(raw[k]) * 0x1.0p-53;
-    const double *xp = a.cur[i] ? a.pop1 : a.pop0;
     if (u < cr) {
-        const uint32_t r1 = c.mutant[0], r2 = c.mutant[1], r3 = c.mutant[2];
-        const double x1 = (a.cur[r1] ? a.pop1 : a.pop0)[pidx(r1, k, a.Kcp)];
-        const double x2 = (a.cur[r2] ? a.pop1 : a.pop0)[pidx(r2, k, a.Kcp)];
-        const double x3 = (a.cur[r3] ? a.pop1 : a.pop0)[pidx(r3, k, a.Kcp)];
+        const double x1 = current_row(a, c.mutant[0])[k];
+        const double x2 = current_row(a, c.mutant[1])[k];
+        const double x3 = current_row(a, c.mutant[2])[k];
         const double v = fma(f, x2 - x3, x1);
         return a.allow_neg ? v : fabs(v);
     }
-    return xp[pidx(i, k, a.Kcp)];
+    return current_row(a, i)[k];
 }
 
 }  // namespace
 
+// Phase timestamps (s_memtime) of workgroup 0, compiled in only for
+// profiling builds: slot 0 top of generation, 1 after the trial barrier,
+// 2 after the chain barrier, 3 after the selection barrier.
+#ifdef DEAC_PHASE_TIMERS
+__device__ unsigned long long deac_phase_clock[4];
+#define DEAC_PHASE(slot)                                                                      \
+    do {                                                                                      \
+        if (blockIdx.x == 0 && threadIdx.x == 0 && !init) {                                   \
+            const unsigned long long now = clock64();                                         \
+            if (slot == 0) phase_t0 = now;                                                     \
+            else atomicAdd(&deac_phase_clock[slot], now - phase_t0), phase_t0 = now;           \
+        }                                                                                     \
+    } while (0)
+#else
+#define DEAC_PHASE(slot) \
+    do {                 \
+    } while (0)
+#endif
+
 __global__ __launch_bounds__(256) void evolve_kernel(EvoArgs a)
 {
+#ifdef DEAC_PHASE_TIMERS
+    unsigned long long phase_t0 = 0;
+#endif
+    extern __shared__ double lds[];
     __shared__ double dsh[kEvoWaves][kWave];
     __shared__ int stop_flag;
     const int lane = threadIdx.x & (kWave - 1);
@@ -318,13 +475,16 @@ __global__ __launch_bounds__(256) void evolve_kernel(EvoArgs a)
     const size_t gthread = static_cast<size_t>(blockIdx.x) * kEvoThreads + threadIdx.x;
     const size_t nthreads = static_cast<size_t>(nwg) * kEvoThreads;
     unsigned target = 0;
-    const int nchain = a.MB * a.TB;
-    const int nmom = (a.use_first || a.use_third) ? a.MB : 0;
+    const int nchain = a.MG * a.TB;
+    const int mgrp = (a.P + 7) / 8;  // moment tasks: 8 members x {first, third} per wave
+    const int nmom = (a.use_first || a.use_third) ? mgrp : 0;
+    const int half = a.stride / 2;
 
     for (int g = 0; g <= a.ngen; g++) {
         const bool init = a.init_mode != 0;
         if (!init) {
             if (g == a.ngen) break;
+            DEAC_PHASE(0);
             // Top of the generation: best fitness, statistics, stop test
             // (deac.cpp:1145-1166).
             if (threadIdx.x == 0) {
@@ -356,14 +516,14 @@ __global__ __launch_bounds__(256) void evolve_kernel(EvoArgs a)
             __syncthreads();
             if (stop_flag) break;
 
-            // Trial vectors into the member's spare buffer, two genome
-            // positions per work item.
+            // Trial vectors into each member's spare buffer, two genome
+            // positions per work item; the padding stays zero.
             const MemberCtl *ctlg = a.ctl + static_cast<size_t>(g) * a.P;
             const uint64_t *rawg = a.raw + static_cast<size_t>(g) * a.P * a.Kc;
-            const size_t items = static_cast<size_t>(a.P) * a.Kcp;
+            const size_t items = static_cast<size_t>(a.P) * half;
             for (size_t it = gthread; it < items; it += nthreads) {
-                const int i = static_cast<int>(it / a.Kcp);
-                const int kp = static_cast<int>(it % a.Kcp);
+                const int i = static_cast<int>(it / half);
+                const int kp = static_cast<int>(it % half);
                 const MemberCtl &c = ctlg[i];
                 const uint64_t *raw = rawg + static_cast<size_t>(i) * a.Kc;
                 double v[2];
@@ -373,26 +533,25 @@ __global__ __launch_bounds__(256) void evolve_kernel(EvoArgs a)
                     else if (a.nsign == 2 && k == a.G) v[e] = trial_value(a, c, raw, i, 0);  // deac.cpp:1305-1308
                     else v[e] = trial_value(a, c, raw, i, k);
                 }
-                double *dst = a.cur[i] ? a.pop0 : a.pop1;
-                *reinterpret_cast<double2 *>(dst + pidx(i, 2 * kp, a.Kcp)) = make_double2(v[0], v[1]);
+                double *dst = (a.cur[i] ? a.pop0 : a.pop1) + static_cast<size_t>(i) * a.stride;
+                *reinterpret_cast<double2 *>(dst + 2 * kp) = make_double2(v[0], v[1]);
             }
             if (!grid_barrier(a.bar, target, a.status)) return;
+            DEAC_PHASE(1);
 
             if (a.renorm) {
                 // Per-generation renormalization (deac.cpp:1342-1364).
-                for (int task = gwave; task < a.MB; task += total_waves) {
-                    const int i = task * kMemberBlock + lane;
-                    if (lane < kMemberBlock && i < a.P) {
-                        const double *src = a.cur[i] ? a.pop0 : a.pop1;
-                        a.mom[i * 4 + 0] = fold_row(src, i, a.nterm, a.Kc, a.Kcp, a.G);
+                for (int i = gwave; i < a.P; i += total_waves)
+                    if (lane == 0) {
+                        const double *src = (a.cur[i] ? a.pop0 : a.pop1) + static_cast<size_t>(i) * a.stride;
+                        a.mom[i * 4 + 0] = fold_row(src, a.nterm, a.Kc, a.G);
                     }
-                }
                 if (!grid_barrier(a.bar, target, a.status)) return;
                 for (size_t it = gthread; it < items; it += nthreads) {
-                    const int i = static_cast<int>(it / a.Kcp);
-                    const int kp = static_cast<int>(it % a.Kcp);
-                    double *dst = a.cur[i] ? a.pop0 : a.pop1;
-                    double2 *p2 = reinterpret_cast<double2 *>(dst + pidx(i, 2 * kp, a.Kcp));
+                    const int i = static_cast<int>(it / half);
+                    const int kp = static_cast<int>(it % half);
+                    double *dst = (a.cur[i] ? a.pop0 : a.pop1) + static_cast<size_t>(i) * a.stride;
+                    double2 *p2 = reinterpret_cast<double2 *>(dst + 2 * kp);
                     double2 v = *p2;
                     const double scale = a.zeroth / a.mom[i * 4 + 0];
                     v.x *= scale;
@@ -403,26 +562,19 @@ __global__ __launch_bounds__(256) void evolve_kernel(EvoArgs a)
             }
         }
 
-        // Model ISF chains and moment rows.
-        for (int task = gwave; task < nchain + nmom; task += total_waves) {
-            if (task < nchain) {
-                chain_block(a, task / a.TB, task % a.TB, lane);
-            } else {
-                const int mb = task - nchain;
-                const int il = lane & 7, r = lane >> 3;
-                const int i = mb * kMemberBlock + il;
-                if (i < a.P && r < 2) {
-                    const double *row = r == 0 ? a.fterm : a.tterm;
-                    const bool used = r == 0 ? a.use_first : a.use_third;
-                    if (used) {
-                        const int c = a.cur[i];
-                        const double *src = (init ? c : 1 - c) ? a.pop1 : a.pop0;
-                        a.mom[i * 4 + 1 + r] = fold_row(src, i, row, a.Kc, a.Kcp, a.G);
-                    }
-                }
+        // Model ISF chains (one workgroup per task) and moment rows.
+        for (int task = blockIdx.x; task < nchain; task += nwg) chain_task(a, lds, task % a.TB, task / a.TB, init);
+        for (int task = gwave; task < nmom; task += total_waves) {
+            const int il = lane & 7, rr = lane >> 3;
+            const int i = task * 8 + il;
+            if (i < a.P && rr < 2) {
+                const double *row = rr == 0 ? a.fterm : a.tterm;
+                const bool used = rr == 0 ? a.use_first : a.use_third;
+                if (used) a.mom[i * 4 + 1 + rr] = fold_row(chain_row(a, i, init), row, a.Kc, a.G);
             }
         }
         if (!grid_barrier(a.bar, target, a.status)) return;
+        DEAC_PHASE(2);
 
         // Fitness and greedy selection, one wavefront per member
         // (deac.cpp:1539-1572; initial fitness deac.cpp:957-976).
@@ -473,6 +625,7 @@ __global__ __launch_bounds__(256) void evolve_kernel(EvoArgs a)
             }
         }
         if (!grid_barrier(a.bar, target, a.status)) return;
+        DEAC_PHASE(3);
         if (init) break;
     }
 }
@@ -503,7 +656,7 @@ hipError_t launch_evolve(const EvoArgs &a, int nwg, hipStream_t s)
 {
     void *args[] = {const_cast<EvoArgs *>(&a)};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(evolve_kernel), dim3(nwg), dim3(kEvoThreads),
-                                      args, 0, s);
+                                      args, kChainLds * sizeof(double), s);
 }
 
 hipError_t launch_zero_u32(unsigned *p, size_t n, hipStream_t s)
@@ -515,8 +668,11 @@ hipError_t launch_zero_u32(unsigned *p, size_t n, hipStream_t s)
 int evolve_max_workgroups(int device)
 {
     int per_cu = 0, cus = 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(evolve_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kChainLds * sizeof(double)) != hipSuccess)
+        return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(evolve_kernel),
-                                                     kEvoThreads, 0) != hipSuccess)
+                                                     kEvoThreads, kChainLds * sizeof(double)) != hipSuccess)
         return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
     return per_cu * cus;

@@ -9,14 +9,15 @@
 
 namespace deac {
 
-// Members per member-block and timeslices per tau-block of the exact FP64
-// model-ISF contraction: one wavefront = 8 members x 8 timeslices, one
-// sequential FMA chain per lane.
-constexpr int kMemberBlock = 8;
-constexpr int kTauBlock = 8;
-// Genome pairs processed per step of a chain (loads are issued one batch
-// ahead); the genome is padded to a multiple of this many pairs.
-constexpr int kPairBatch = 8;
+// Exact FP64 model-ISF contraction: one compute wavefront holds 4 members
+// (rows of 16 lanes) x 16 timeslices (lanes of a row), one sequential FMA
+// chain per lane; the genome value reaches the row by DPP row broadcast, the
+// kernel matrix comes from LDS.  The genome is streamed through LDS in chunks
+// of kChunk positions, kStages chunks in flight.
+constexpr int kChainRows = 4;
+constexpr int kChainTau = 16;
+constexpr int kChunk = 128;
+constexpr int kStages = 7;
 
 // Random choices of one member in one generation, produced by walk_kernel.
 struct MemberCtl {
@@ -45,13 +46,15 @@ struct MaskArgs {
 };
 
 struct EvoArgs {
-    int P, T, G, nsign, Kc, Kcp, MB, TB;   // Kcp: genome pairs, padded to kPairBatch
+    int P, T, G, nsign, Kc;
+    int stride;                  // row stride of K and of the populations: Kc padded to kChunk
+    int MG, TB;                  // member groups of kChainRows, tau blocks of kChainTau
     int use_first, use_third, renorm, allow_neg, init_mode, track_stats;
     double stop, zeroth, first_moment, third_moment, third_moment_error;
-    const double *kmat;          // [TB][Kcp][8][2]
+    const double *kmat;          // [TB*16][stride], zero padded
     const double *nterm, *fterm, *tterm;   // [Kc] or null
     const double *obs, *sig;     // [T]
-    double *pop0, *pop1;         // [MB][Kcp][8][2] each
+    double *pop0, *pop1;         // [MG*4][stride] each, member-major rows, zero padded
     int *cur;                    // [P] buffer holding each member's current genome
     double *fitness;             // [P]
     double *cr_old, *f_old;      // [P][nsign]

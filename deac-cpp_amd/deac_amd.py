@@ -20,7 +20,7 @@ STATUS = {0: "success", 1: "invalid argument", 2: "no usable gfx950 device", 3: 
           4: "device error", 5: "unsupported"}
 
 EXPORTED = ["deac_create", "deac_run", "deac_destroy", "deac_output_length", "deac_seed_state",
-            "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_run_timing"]
+            "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_run_profile"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -70,6 +70,11 @@ class Result(C.Structure):
     ]
 
 
+class RunProfile(C.Structure):
+    _fields_ = [("loop_ms", C.c_double), ("walk_ms", C.c_double), ("mask_ms", C.c_double),
+                ("evolve_ms", C.c_double), ("generations", C.c_size_t), ("evolve_launches", C.c_size_t)]
+
+
 class DeacError(RuntimeError):
     def __init__(self, status, message=None):
         self.status = status
@@ -99,8 +104,8 @@ def lib():
         L.deac_frequency_grid.restype = C.c_int
         L.deac_status_string.argtypes = [C.c_int]
         L.deac_status_string.restype = C.c_char_p
-        L.deac_last_run_timing.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_size_t)]
-        L.deac_last_run_timing.restype = C.c_int
+        L.deac_last_run_profile.argtypes = [C.c_void_p, C.POINTER(RunProfile)]
+        L.deac_last_run_profile.restype = C.c_int
         _lib = L
     return _lib
 
@@ -193,11 +198,11 @@ class Engine:
         n = r.generation + 1
         for k in ("mean", "minimum", "squared_mean"):
             out[k] = out[k][:n]
-        ms = C.c_double()
-        gens = C.c_size_t()
-        lib().deac_last_run_timing(self.handle, C.byref(ms), C.byref(gens))
+        prof = RunProfile()
+        lib().deac_last_run_profile(self.handle, C.byref(prof))
         out.update(generation=r.generation, minimum_fitness=r.minimum_fitness, best_index=r.best_index,
-                   rng=[int(v) for v in s], loop_ms=ms.value, loop_generations=gens.value)
+                   rng=[int(v) for v in s], loop_ms=prof.loop_ms, loop_generations=prof.generations,
+                   profile={k: getattr(prof, k) for k, _ in RunProfile._fields_})
         return out
 
 

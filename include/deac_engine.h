@@ -136,9 +136,17 @@ deac_status deac_frequency_grid(size_t genome_size, double omega_max, double *fr
 
 const char *deac_status_string(deac_status status);
 
-/* Per-run timing of the last deac_run on this handle (device time of the
- * generation loop in milliseconds and the number of generations it ran). */
-deac_status deac_last_run_timing(const deac_handle *handle, double *loop_ms, size_t *generations);
+/* Device timing of the last deac_run on this handle, from HIP events on the
+ * handle's stream (no counterpart in the reference, which has no timers). */
+typedef struct deac_run_profile {
+    double loop_ms;              /* whole generation loop */
+    double walk_ms;              /* RNG walk kernels */
+    double mask_ms;              /* crossover-mask kernels */
+    double evolve_ms;            /* fused generation kernels */
+    size_t generations;          /* generations evolved */
+    size_t evolve_launches;      /* launches of the fused generation kernel */
+} deac_run_profile;
+deac_status deac_last_run_profile(const deac_handle *handle, deac_run_profile *profile);
 
 #ifdef __cplusplus
 }

@@ -125,10 +125,10 @@ void dor_kernel_terms(const dor_problem *p, double *kp, double *km) /* deac.cpp:
             }
             switch (p->model) {
             case DOR_MODEL_HYPERBOLIC: {                /* deac.cpp:323-324, 363-364 */
-                /* -0.5*beta is exact, so fusing the subtraction changes nothing;
-                 * 1.5*beta - t is fused by the reference build. */
+                /* The reference build hoists -0.5*beta and 1.5*beta out of the
+                 * loop: both subtractions round the product first. */
                 const double bo2mtp = -0.5 * beta - t;
-                const double bo2mtn = fma(1.5, beta, -t);
+                const double bo2mtn = 1.5 * beta - t;
                 kp[k] = per * df * exp(bo2mtp * w) / 2;
                 km[k] = per * df * exp(-bo2mtn * w) / 2;
                 break;

@@ -1,0 +1,113 @@
+"""The C ABI without a GPU: libdeac_amd.so loads, exports every entry point
+include/deac_engine.h declares, and its host-side pieces (seeding, jump-ahead,
+frequency grid, problem validation) agree with the oracle and the reference's
+messages.  No call here reaches a device."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import deac_amd as da
+import oracle_binding as ob
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "deac_engine.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(deac_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_abi():
+    assert set(declared_functions()) == set(da.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = da.lib()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_seed_state_matches_reference_seeding():
+    for seed in (0, 1, 7, 17, 123456789):
+        assert list(da.seed_state(seed)) == list(ob.seed_state(seed))
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 64, 65, 1000, 8192, 16384 * 2 + 7])
+def test_rng_advance_equals_stepping(n):
+    s = ob.seed_state(42)
+    start = [int(v) for v in s]
+    for _ in range(n):
+        ob.lib().dor_xoshiro256p_next(s)
+    assert da.rng_advance(start, n) == [int(v) for v in s]
+
+
+def test_rng_advance_composes():
+    s = [int(v) for v in ob.seed_state(5)]
+    a = da.rng_advance(da.rng_advance(s, 12345), 67890)
+    assert a == da.rng_advance(s, 12345 + 67890)
+    big = (1 << 63) + 12345
+    assert da.rng_advance(da.rng_advance(s, big), 1) == da.rng_advance(s, big + 1)
+
+
+@pytest.mark.parametrize("G,omega", [(2, 1.0), (8, 4.0), (512, 64.0), (4096, 512.0), (16384, 512.0), (1000, 3.7)])
+def test_frequency_grid_matches_reference(G, omega):
+    np.testing.assert_array_equal(da.frequency_grid(G, omega), ob.frequency_grid(G, omega))
+
+
+def _tiny(**kw):
+    tau = np.array([0, .2, .4, .6])
+    isf = np.array([-1, -.85, -.72, -.61])
+    err = np.full(4, .05)
+    args = dict(tau=tau, isf=isf, err=err, frequency=np.linspace(0, 4, 8), temperature=1.0)
+    args.update({k: kw.pop(k) for k in list(kw) if k in args})
+    return da.make_problem(args["tau"], args["isf"], args["err"], args["frequency"], args["temperature"], **kw)
+
+
+# (problem overrides, reference message) -- deac.cpp:2075-2151
+VALIDATION = [
+    (dict(tau=np.array([0.0]), isf=np.array([-1.0]), err=np.array([0.05])),
+     "ISF input file must contain at least two timeslices"),
+    (dict(isf=np.array([-1, np.nan, -.72, -.61])), "ISF input file contains non-finite values"),
+    (dict(isf=np.array([1, .85, .72, .61]), spectra_type="spfsf"),
+     "positive ISF values are not supported for single-particle spectra"),
+    (dict(isf=np.array([1, .85, .72, .61]), spectra_type="spbsf"),
+     "positive ISF values are not supported for single-particle spectra"),
+    (dict(err=np.array([.05, 0, .05, .05])), "ISF input errors must be positive"),
+    (dict(temperature=0.0), "temperature must be positive for non-ZeroT builds"),
+    (dict(temperature=float("inf")), "temperature must be positive for non-ZeroT builds"),
+    (dict(population_size=3), "population_size must be at least 4"),
+    (dict(frequency=np.array([0.0])), "genome_size must be at least 2"),
+    (dict(frequency=np.array([0.0, np.inf])), "frequencies must be finite and non-negative"),
+    (dict(frequency=np.array([0.0, -1.0])), "frequencies must be finite and non-negative"),
+    (dict(frequency=np.array([0.0, 2.0, 1.0])), "frequencies must be sorted in non-decreasing order"),
+    (dict(first_moment=np.nan), "first_moment must be finite"),
+    (dict(third_moment=1.0, third_moment_error=0.0), "third_moment_error must be positive when third_moment is used"),
+]
+
+
+@pytest.mark.parametrize("overrides,message", VALIDATION)
+def test_create_rejects_invalid_problems(overrides, message):
+    kw = dict(spectra_type="spfsf")
+    kw.update(overrides)
+    p = _tiny(**kw)
+    with pytest.raises(da.DeacError) as e:
+        da.Engine(p)
+    assert e.value.status == 1
+    assert message in str(e.value)
+
+
+def test_zero_temperature_skips_temperature_check_and_halves_output():
+    p = _tiny(temperature=0.0, zero_temperature=True, spectra_type="bfull", isf=np.array([1, .85, .72, .61]))
+    assert da.lib().deac_output_length(C.byref(p)) == 8
+    p2 = _tiny(spectra_type="bfull", isf=np.array([1, .85, .72, .61]))
+    assert da.lib().deac_output_length(C.byref(p2)) == 15
+
+
+def test_status_strings():
+    for code, text in da.STATUS.items():
+        assert da.lib().deac_status_string(code).decode() == text

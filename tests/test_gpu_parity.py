@@ -1,0 +1,128 @@
+"""Plain FP64 engine on the GPU against the reference and the oracle.
+
+Bar: bit-exact.  The engine reproduces the reference CPU build's random
+stream, evaluation order and rounding, so every per-generation statistic,
+the DSF and the frequency grid must be identical float64 values.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import deac_amd as da
+import golden_cases as gc
+import oracle_binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def engine_run(case, N=None, seed=None, stop=None):
+    a = case["meta"]["args"]
+    freq = gc.frequency_of(case, da.frequency_grid)
+    p = da.make_problem(case["tau"], case["isf"], case["err"], freq, a["T"], **gc.problem_kwargs(case))
+    with da.Engine(p) as e:
+        return e.run(a["seed"] if seed is None else seed, a["N"] if N is None else N,
+                     a["stop"] if stop is None else stop, track_stats=True)
+
+
+@pytest.mark.parametrize("name", gc.names())
+def test_engine_matches_reference_golden(name):
+    print(f"[gpu] golden {name}", flush=True)
+    case = gc.load(name)
+    got = engine_run(case)
+    errors = gc.compare_outputs(got, case)
+    assert not errors, f"{name}: " + "; ".join(errors)
+
+
+def _synthetic(fname):
+    return np.fromfile(f"{gc.GOLDEN_DIR}/{fname}").reshape(3, -1)
+
+
+@pytest.mark.parametrize("seed", [2, 3])
+def test_engine_matches_oracle_config1_other_seeds(seed):
+    print(f"[gpu] config-1 seed {seed}", flush=True)
+    t, f_, s = _synthetic("isf_c1_T33.bin")
+    freq = ob.frequency_grid(512, 64.0)
+    N = 3000
+    o = ob.run(ob.make_problem(t, f_, s, freq, 1.35, number_of_generations=N, population_size=8, normalize=True,
+                               stop_minimum_fitness=0.0), seed)
+    p = da.make_problem(t, f_, s, freq, 1.35, population_size=8, normalize=True)
+    with da.Engine(p) as e:
+        g = e.run(seed, N, 0.0)
+    for k in ("minimum", "mean", "squared_mean", "dsf", "frequency"):
+        assert gc.bit_diff(g[k], o[k]) is None, k
+    assert g["rng"] == o["rng"]
+
+
+@pytest.mark.parametrize("model,spectra", [("standard", "bfull"), ("normalization", "ffull"), ("hyperbolic", "bfull")])
+def test_zero_temperature_matches_oracle(model, spectra):
+    # no compilable reference build exists for zero temperature; the oracle's
+    # single-sign restatement is the checker
+    print(f"[gpu] zero-T {model} {spectra}", flush=True)
+    T = 17
+    tau = np.linspace(0, 2.0, T)
+    isf = np.exp(-1.5 * tau)
+    err = 1e-3 * isf
+    freq = ob.frequency_grid(37, 8.0)
+    kw = dict(model=model, spectra_type=spectra, zero_temperature=True, population_size=6, normalize=True,
+              first_moment=1.5)
+    o = ob.run(ob.make_problem(tau, isf, err, freq, 0.0, number_of_generations=400, stop_minimum_fitness=0.0, **kw), 9)
+    with da.Engine(da.make_problem(tau, isf, err, freq, 0.0, **kw)) as e:
+        g = e.run(9, 400, 0.0)
+    assert len(g["dsf"]) == 37
+    for k in ("minimum", "mean", "squared_mean", "dsf", "frequency"):
+        assert gc.bit_diff(g[k], o[k]) is None, k
+
+
+def test_runs_are_deterministic_and_handles_reusable():
+    print("[gpu] determinism", flush=True)
+    t, f_, s = _synthetic("isf_c2_T65.bin")
+    freq = da.frequency_grid(4096, 512.0)
+    with da.Engine(da.make_problem(t, f_, s, freq, 1.35, population_size=8, normalize=True)) as e:
+        a = e.run(1, 300, 0.0)
+        b = e.run(1, 300, 0.0)
+        c = e.run(2, 50, 0.0)
+        d = e.run(1, 300, 0.0)
+    for k in ("minimum", "mean", "squared_mean", "dsf"):
+        assert gc.bit_diff(a[k], b[k]) is None and gc.bit_diff(a[k], d[k]) is None, k
+    assert gc.bit_diff(a["minimum"][:50], c["minimum"][:50]) is not None
+
+
+def test_rng_continuation_matches_oracle():
+    # the engine returns the state at the top of the first generation it did
+    # not evolve; continuing from it matches the oracle's continuation
+    print("[gpu] rng continuation", flush=True)
+    case = gc.load("smoke_stop")
+    a = case["meta"]["args"]
+    got = engine_run(case)
+    freq = gc.frequency_of(case, ob.frequency_grid)
+    o = ob.run(ob.make_problem(case["tau"], case["isf"], case["err"], freq, a["T"], number_of_generations=a["N"],
+                               stop_minimum_fitness=a["stop"], **gc.problem_kwargs(case)), a["seed"])
+    assert got["rng"] == o["rng"]
+    assert got["generation"] == o["generation"] == case["meta"]["generation"]
+
+
+def test_full_size_properties_config2():
+    # config 2 at its full genome: best fitness never increases, the DSF is
+    # non-negative and mirrored, and the reported fitness is the reduced
+    # chi^2 of the reported DSF recomputed by the oracle's reductions
+    print("[gpu] config-2 full-size properties", flush=True)
+    t, f_, s = _synthetic("isf_c2_T65.bin")
+    G = 4096
+    freq = da.frequency_grid(G, 512.0)
+    with da.Engine(da.make_problem(t, f_, s, freq, 1.35, population_size=8, normalize=True)) as e:
+        g = e.run(1, 3000, 0.0)
+    assert np.all(np.diff(g["minimum"]) <= 0)
+    assert np.all(g["dsf"] >= 0)
+    np.testing.assert_array_equal(g["frequency"][G - 1:], freq)
+    sp, sm = g["dsf"][G - 1:], g["dsf"][:G][::-1]
+    p = ob.make_problem(t, f_, s, freq, 1.35)
+    kp, km = np.zeros(65 * G), np.zeros(65 * G)
+    ob.lib().dor_kernel_terms(C.byref(p), ob.ptr(kp), ob.ptr(km))
+    kp, km = kp.reshape(65, G), km.reshape(65, G)
+    L = ob.lib()
+    E = np.array([L.dor_fma_chain(L.dor_fma_chain(0.0, ob.ptr(np.ascontiguousarray(kp[i])), ob.ptr(sp), G),
+                                  ob.ptr(np.ascontiguousarray(km[i])), ob.ptr(np.ascontiguousarray(sm)), G)
+                  for i in range(65)])
+    chi = L.dor_chi_square(ob.ptr(np.ascontiguousarray(f_)), ob.ptr(E), ob.ptr(np.ascontiguousarray(s)), 65) / 65
+    assert chi == g["minimum_fitness"] == g["minimum"][-1]
