@@ -1,0 +1,540 @@
+// Batched C ABI (deac_batch_* in include/deac_engine.h): many independent
+// reconstructions run by one persistent launch, one workgroup each.
+//
+// Host work: validation and model building per distinct problem (identical
+// problems share one device copy of K), the kernel-side layouts (K in MFMA
+// tile order, the remainder rows, the walk and sub-stream jump tables), the
+// initial populations (on host threads, with each instance's own stream),
+// launch slicing with per-instance stop and compaction, and the same
+// epilogue as deac_run.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "batch_kernels.hpp"
+#include "deac_engine.h"
+#include "host_model.hpp"
+
+namespace {
+
+thread_local std::string g_batch_message;
+
+#define BATCH_HIP(expr)                        \
+    do {                                       \
+        if ((expr) != hipSuccess) {            \
+            (void)hipGetLastError();           \
+            return DEAC_ERR_DEVICE;            \
+        }                                      \
+    } while (0)
+
+// Device copy of one distinct problem, laid out for the batch kernel.
+struct BatchModel {
+    deac::Model m;
+    int Ppad = 0, NM = 0, NT = 0, C = 0, nchunk = 0, Lpad = 0, Lw = 0, ntt = 0, nrem = 0, S = 0, Lsub = 0;
+    double *ktile = nullptr, *krem = nullptr, *obs = nullptr, *sig = nullptr;
+    uint64_t *walk_rows = nullptr, *sub_tab = nullptr;
+};
+
+// Byte image of everything a problem's results depend on.
+std::string problem_key(const deac_problem &p)
+{
+    std::string k;
+    auto put = [&](const void *x, size_t n) { k.append(static_cast<const char *>(x), n); };
+    const size_t T = p.number_of_timeslices, G = p.genome_size;
+    put(&T, sizeof T);
+    put(p.imaginary_time, T * sizeof(double));
+    put(p.isf, T * sizeof(double));
+    put(p.isf_error, T * sizeof(double));
+    put(&G, sizeof G);
+    put(p.frequency, G * sizeof(double));
+    const deac_problem *q = &p;
+    put(&q->temperature, sizeof q->temperature);
+    const int ints[6] = {p.model, p.spectra_type, p.zero_temperature, p.allow_negative_spectral_weight,
+                         p.renormalize_each_generation, p.normalize};
+    put(ints, sizeof ints);
+    put(&p.population_size, sizeof p.population_size);
+    const double dbl[7] = {p.first_moment, p.third_moment, p.third_moment_error, p.crossover_probability,
+                           p.self_adapting_crossover_probability, p.differential_weight,
+                           p.self_adapting_differential_weight_probability};
+    put(dbl, sizeof dbl);
+    return k;
+}
+
+}  // namespace
+
+struct deac_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr;
+    std::vector<std::unique_ptr<BatchModel>> models;
+    std::vector<int> model_of;                 // instance -> model
+    std::vector<void *> allocations;
+    // per-instance state, packed
+    std::vector<size_t> pop_off, mask_off;     // offsets into the pools
+    double *pop = nullptr, *fitness = nullptr, *cr_old = nullptr, *f_old = nullptr;
+    uint32_t *masks = nullptr;
+    uint64_t *rng = nullptr;
+    int *status = nullptr;
+    deac::BatchDesc *descs = nullptr;          // device copy of the launch descriptors
+    deac_run_profile last{};
+
+    ~deac_batch()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : allocations) (void)hipFree(p);
+        for (hipEvent_t e : {ev0, ev1, kev0, kev1})
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <class T>
+    deac_status alloc(T *&out, size_t count)
+    {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+            (void)hipGetLastError();
+            return DEAC_ERR_OUT_OF_MEMORY;
+        }
+        allocations.push_back(p);
+        out = static_cast<T *>(p);
+        return DEAC_OK;
+    }
+    void release(void *p)
+    {
+        auto it = std::find(allocations.begin(), allocations.end(), p);
+        if (it != allocations.end()) {
+            allocations.erase(it);
+            (void)hipFree(p);
+        }
+    }
+};
+
+namespace {
+
+// Kernel-side layout of one model; DEAC_ERR_UNSUPPORTED for shapes the batch
+// kernel does not run (the single-instance engine runs them).
+deac_status plan_model(BatchModel &b, std::string &why)
+{
+    const deac::Model &m = b.m;
+    if (m.P > static_cast<size_t>(deac::kBatchMaxP)) {
+        why = "batched runs take at most 32 members per population";
+        return DEAC_ERR_UNSUPPORTED;
+    }
+    if (m.use_first || m.use_third) {
+        why = "batched runs do not take moment constraints";
+        return DEAC_ERR_UNSUPPORTED;
+    }
+    if (m.normalize && m.renormalize) {
+        why = "batched runs do not renormalize each generation";
+        return DEAC_ERR_UNSUPPORTED;
+    }
+    const int T = static_cast<int>(m.T), L = static_cast<int>(m.Kc);
+    b.Ppad = m.P <= 16 ? 16 : 32;
+    b.NM = b.Ppad / 16;
+    b.C = deac::kBatchChunkDoubles / b.Ppad;
+    b.nchunk = (L + b.C - 1) / b.C;
+    b.Lpad = b.nchunk * b.C;
+    b.Lw = b.Lpad / 32;
+    b.ntt = T / 16;
+    b.nrem = T % 16;
+    // Rows past the last full tile run as vector-ALU chains when they fit the
+    // remainder wave (2 chains per lane, one LDS ring buffer per chunk);
+    // otherwise they become one more MFMA tile padded with zero rows.
+    if (b.nrem * b.Ppad > 64 * deac::kBatchRemChains || b.nrem * b.C > 1024) {
+        b.ntt += 1;
+        b.nrem = 0;
+    }
+    const int per_wave = (b.ntt + 3) / 4;
+    b.NT = per_wave <= 1 ? 1 : per_wave <= 2 ? 2 : per_wave <= 4 ? 4 : 0;
+    if (b.NT == 0 || T * b.Ppad > deac::kBatchRegionDoubles) {
+        why = "batched runs take at most 271 timeslices";
+        return DEAC_ERR_UNSUPPORTED;
+    }
+    b.S = deac::kBatchWorkerLanes / b.Ppad;
+    b.Lsub = (L + 32 * b.S - 1) / (32 * b.S) * 32;
+    return DEAC_OK;
+}
+
+deac_status upload_model(deac_batch *h, BatchModel &b)
+{
+    const deac::Model &m = b.m;
+    const int T = static_cast<int>(m.T), L = static_cast<int>(m.Kc);
+    const int nkb2 = b.Lpad / 8;
+    std::vector<double> kt(static_cast<size_t>(std::max(b.ntt, 1)) * nkb2 * 64 * 2, -0.0);
+    for (int t = 0; t < b.ntt; t++)
+        for (int kb2 = 0; kb2 < nkb2; kb2++)
+            for (int l = 0; l < 64; l++)
+                for (int e = 0; e < 2; e++) {
+                    const int tau = t * 16 + (l & 15), slot = (2 * kb2 + e) * 4 + (l >> 4);
+                    double v = -0.0;
+                    if (tau < T && slot < L) v = m.kernel[static_cast<size_t>(tau) * L + slot];
+                    kt[((static_cast<size_t>(t) * nkb2 + kb2) * 64 + l) * 2 + e] = v;
+                }
+    std::vector<double> kr(static_cast<size_t>(std::max(b.nrem, 1)) * b.Lpad, -0.0);
+    for (int r = 0; r < b.nrem; r++)
+        std::memcpy(&kr[static_cast<size_t>(r) * b.Lpad], &m.kernel[static_cast<size_t>(b.ntt * 16 + r) * L],
+                    L * sizeof(double));
+    // Jump tables: T^L by rows for the walk, T^(k*Lsub) by nibbles for the
+    // mask sub-streams k = 1..S-1.
+    std::vector<uint64_t> rows(256 * 4), tabs(static_cast<size_t>(std::max(b.S - 1, 1)) * 64 * 16 * 4);
+    deac::matrix_rows(deac::transition_power(static_cast<uint64_t>(L)), rows.data());
+    const deac::Gf2Matrix step = deac::transition_power(static_cast<uint64_t>(b.Lsub));
+    deac::Gf2Matrix acc = step;
+    for (int k = 1; k < b.S; k++) {
+        deac::nibble_table(acc, tabs.data() + static_cast<size_t>(k - 1) * 64 * 16 * 4);
+        acc = step * acc;
+    }
+    deac_status st = DEAC_OK;
+    auto A = [&](auto *&p, size_t n) {
+        if (st == DEAC_OK) st = h->alloc(p, n);
+    };
+    A(b.ktile, kt.size());
+    A(b.krem, kr.size());
+    A(b.obs, m.T);
+    A(b.sig, m.T);
+    A(b.walk_rows, rows.size());
+    A(b.sub_tab, tabs.size());
+    if (st != DEAC_OK) return st;
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        if (st == DEAC_OK && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) st = DEAC_ERR_DEVICE;
+    };
+    up(b.ktile, kt.data(), kt.size() * sizeof(double));
+    up(b.krem, kr.data(), kr.size() * sizeof(double));
+    up(b.obs, m.isf.data(), m.T * sizeof(double));
+    up(b.sig, m.err.data(), m.T * sizeof(double));
+    up(b.walk_rows, rows.data(), rows.size() * sizeof(uint64_t));
+    up(b.sub_tab, tabs.data(), tabs.size() * sizeof(uint64_t));
+    return st;
+}
+
+// Runs fn(i) for i < n on up to 16 host threads.
+template <class F>
+void parallel_for(size_t n, F fn)
+{
+    const size_t nt = std::min<size_t>(n, std::clamp<unsigned>(std::thread::hardware_concurrency(), 1u, 16u));
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            for (size_t i = t; i < n; i += nt) fn(i);
+        });
+    for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_batch **out, const char **message)
+{
+    if (message) *message = nullptr;
+    if (!out) return DEAC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!problems || count == 0 || count > (size_t(1) << 20)) {
+        g_batch_message = "a batch holds 1 to 1048576 problems";
+        if (message) *message = g_batch_message.c_str();
+        return DEAC_ERR_INVALID_ARGUMENT;
+    }
+    for (size_t i = 0; i < count; i++) {
+        std::string why;
+        const deac_status v = deac::validate(problems[i], why);
+        if (v != DEAC_OK) {
+            g_batch_message = "problem " + std::to_string(i) + ": " + why;
+            if (message) *message = g_batch_message.c_str();
+            return v;
+        }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return DEAC_ERR_NO_DEVICE;
+    }
+    std::unique_ptr<deac_batch> h(new deac_batch());
+    BATCH_HIP(hipGetDevice(&h->device));
+    hipDeviceProp_t prop;
+    BATCH_HIP(hipGetDeviceProperties(&prop, h->device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DEAC_ERR_NO_DEVICE;
+    BATCH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&h->ev0, &h->ev1, &h->kev0, &h->kev1}) BATCH_HIP(hipEventCreate(e));
+
+    // Distinct problems.
+    std::map<std::string, int> seen;
+    h->model_of.resize(count);
+    for (size_t i = 0; i < count; i++) {
+        const std::string key = problem_key(problems[i]);
+        auto it = seen.find(key);
+        if (it != seen.end()) {
+            h->model_of[i] = it->second;
+            continue;
+        }
+        auto b = std::make_unique<BatchModel>();
+        b->m = deac::build_model(problems[i]);
+        std::string why;
+        const deac_status st = plan_model(*b, why);
+        if (st != DEAC_OK) {
+            g_batch_message = "problem " + std::to_string(i) + ": " + why;
+            if (message) *message = g_batch_message.c_str();
+            return st;
+        }
+        const deac_status su = upload_model(h.get(), *b);
+        if (su != DEAC_OK) return su;
+        seen.emplace(key, static_cast<int>(h->models.size()));
+        h->model_of[i] = static_cast<int>(h->models.size());
+        h->models.push_back(std::move(b));
+    }
+
+    // Per-instance state pools.
+    size_t npop = 0, nmask = 0;
+    h->pop_off.resize(count);
+    h->mask_off.resize(count);
+    for (size_t i = 0; i < count; i++) {
+        const BatchModel &b = *h->models[h->model_of[i]];
+        h->pop_off[i] = npop;
+        h->mask_off[i] = nmask;
+        npop += b.m.P * static_cast<size_t>(b.Lpad);
+        nmask += static_cast<size_t>(3) * b.Ppad * 2 * b.Lw;
+    }
+    deac_status st = DEAC_OK;
+    auto A = [&](auto *&p, size_t n) {
+        if (st == DEAC_OK) st = h->alloc(p, n);
+    };
+    A(h->pop, npop);
+    A(h->masks, nmask);
+    A(h->fitness, count * deac::kBatchMaxP);
+    A(h->cr_old, count * deac::kBatchMaxP * 2);
+    A(h->f_old, count * deac::kBatchMaxP * 2);
+    A(h->rng, count * 4);
+    A(h->status, count * 2);
+    A(h->descs, count);
+    if (st != DEAC_OK) return st;
+    *out = h.release();
+    return DEAC_OK;
+}
+
+void deac_batch_destroy(deac_batch *h) { delete h; }
+
+size_t deac_batch_size(const deac_batch *h) { return h ? h->model_of.size() : 0; }
+
+deac_status deac_batch_last_profile(const deac_batch *h, deac_run_profile *profile)
+{
+    if (!h || !profile) return DEAC_ERR_INVALID_ARGUMENT;
+    *profile = h->last;
+    return DEAC_OK;
+}
+
+deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_options *options, deac_result *results)
+{
+    if (!h || !rng_states || !options || !results) return DEAC_ERR_INVALID_ARGUMENT;
+    const size_t n = h->model_of.size();
+    for (size_t i = 0; i < n; i++) {
+        if (options[i].number_of_generations < 2 || !results[i].dsf || !results[i].frequency) {
+            g_batch_message = "instance " + std::to_string(i) + ": number_of_generations must be at least 2 and the "
+                              "dsf and frequency outputs are required";
+            return DEAC_ERR_INVALID_ARGUMENT;
+        }
+    }
+    BATCH_HIP(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+
+    // Initial populations on the host (deac.cpp:430-603), each from its own
+    // stream; the state after them is where the device walk starts.
+    std::vector<double> pop_host(h->pop_off.back() + h->models[h->model_of[n - 1]]->m.P *
+                                                         static_cast<size_t>(h->models[h->model_of[n - 1]]->Lpad),
+                                 0.0);
+    std::vector<uint64_t> rng_host(4 * n);
+    std::vector<double> cr_host(n * deac::kBatchMaxP * 2, 0.0), f_host(n * deac::kBatchMaxP * 2, 0.0);
+    parallel_for(n, [&](size_t i) {
+        const BatchModel &b = *h->models[h->model_of[i]];
+        deac::Xoshiro rng{{rng_states[4 * i], rng_states[4 * i + 1], rng_states[4 * i + 2], rng_states[4 * i + 3]}};
+        std::vector<double> pop;
+        deac::initial_population(b.m, rng, pop);
+        for (size_t m = 0; m < b.m.P; m++)
+            std::memcpy(&pop_host[h->pop_off[i] + m * b.Lpad], &pop[m * b.m.Kc], b.m.Kc * sizeof(double));
+        std::memcpy(&rng_host[4 * i], rng.s, sizeof rng.s);
+        for (size_t m = 0; m < b.m.P; m++)
+            for (int sg = 0; sg < 2; sg++) {
+                cr_host[(i * deac::kBatchMaxP + m) * 2 + sg] = b.m.cr0;  // deac.cpp:990-1032
+                f_host[(i * deac::kBatchMaxP + m) * 2 + sg] = b.m.f0;
+            }
+    });
+    BATCH_HIP(hipMemcpyAsync(h->pop, pop_host.data(), pop_host.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    BATCH_HIP(hipMemcpyAsync(h->rng, rng_host.data(), rng_host.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    BATCH_HIP(hipMemcpyAsync(h->cr_old, cr_host.data(), cr_host.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    BATCH_HIP(hipMemcpyAsync(h->f_old, f_host.data(), f_host.size() * sizeof(double), hipMemcpyHostToDevice, s));
+
+    // Statistics buffers for the instances that track them.
+    std::vector<double *> stats(n, nullptr);
+    struct Freer {
+        deac_batch *h;
+        std::vector<double *> &v;
+        ~Freer()
+        {
+            for (double *p : v)
+                if (p) h->release(p);
+        }
+    } freer{h, stats};
+    for (size_t i = 0; i < n; i++)
+        if (options[i].track_stats && h->alloc(stats[i], 3 * options[i].number_of_generations) != DEAC_OK)
+            return DEAC_ERR_OUT_OF_MEMORY;
+
+    // Launch slices: every active instance evolves up to `slice` generations
+    // per launch; stopped or finished instances leave the next launch.
+    long slice = 4096;
+    if (const char *e = std::getenv("DEAC_BATCH_SLICE")) slice = std::max(1L, std::atol(e));
+    std::vector<long> next_gen(n, 0), stopped_at(n, -1);
+    std::vector<char> active(n, 1);
+    std::vector<deac::BatchDesc> descs(n);
+    deac_run_profile prof{};
+    BATCH_HIP(hipEventRecord(h->ev0, s));
+    bool first = true;
+    for (;;) {
+        // Group the active instances by kernel instantiation.
+        std::map<std::pair<int, int>, std::vector<size_t>> groups;
+        for (size_t i = 0; i < n; i++)
+            if (active[i]) {
+                const BatchModel &b = *h->models[h->model_of[i]];
+                groups[{b.NT, b.NM}].push_back(i);
+            }
+        if (groups.empty()) break;
+        size_t slot = 0;
+        std::vector<std::pair<std::pair<int, int>, std::pair<size_t, size_t>>> launches;
+        std::vector<size_t> order;
+        for (auto &[key, members] : groups) {
+            const size_t begin = slot;
+            for (size_t i : members) {
+                const BatchModel &b = *h->models[h->model_of[i]];
+                const deac::Model &m = b.m;
+                deac::BatchDesc &d = descs[slot++];
+                d = deac::BatchDesc{};
+                d.ktile = b.ktile;
+                d.krem = b.krem;
+                d.obs = b.obs;
+                d.sig = b.sig;
+                d.walk_rows = b.walk_rows;
+                d.sub_tab = b.sub_tab;
+                d.pop = h->pop + h->pop_off[i];
+                d.fitness = h->fitness + i * deac::kBatchMaxP;
+                d.cr_old = h->cr_old + i * deac::kBatchMaxP * 2;
+                d.f_old = h->f_old + i * deac::kBatchMaxP * 2;
+                d.masks = h->masks + h->mask_off[i];
+                d.rng = h->rng + 4 * i;
+                d.stats = stats[i];
+                d.status = h->status + 2 * i;
+                d.stats_len = static_cast<long>(options[i].number_of_generations);
+                d.gen0 = next_gen[i];
+                d.gen_end = std::min<long>(next_gen[i] + slice, static_cast<long>(options[i].number_of_generations) - 1);
+                d.stop = options[i].stop_minimum_fitness;
+                d.tau_cr = m.tau_cr;
+                d.tau_f = m.tau_f;
+                d.mod_magic = ~UINT64_C(0) / m.P;
+                d.init = first ? 1 : 0;
+                d.T = static_cast<int>(m.T);
+                d.G = static_cast<int>(m.G);
+                d.nsign = m.nsign;
+                d.L = static_cast<int>(m.Kc);
+                d.Lpad = b.Lpad;
+                d.Lw = b.Lw;
+                d.nchunk = b.nchunk;
+                d.ntt = b.ntt;
+                d.nrem = b.nrem;
+                d.P = static_cast<int>(m.P);
+                d.S = b.S;
+                d.Lsub = b.Lsub;
+                d.allow_neg = m.allow_negative ? 1 : 0;
+                order.push_back(i);
+            }
+            launches.push_back({key, {begin, slot - begin}});
+        }
+        BATCH_HIP(hipMemcpyAsync(h->descs, descs.data(), slot * sizeof(deac::BatchDesc), hipMemcpyHostToDevice, s));
+        BATCH_HIP(hipEventRecord(h->kev0, s));
+        for (auto &[key, range] : launches)
+            BATCH_HIP(deac::launch_batch(h->descs + range.first, static_cast<int>(range.second), key.first, key.second,
+                                         s));
+        BATCH_HIP(hipEventRecord(h->kev1, s));
+        std::vector<int> st(2 * n);
+        BATCH_HIP(hipMemcpyAsync(st.data(), h->status, st.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        BATCH_HIP(hipStreamSynchronize(s));
+        float ms = 0.0f;
+        BATCH_HIP(hipEventElapsedTime(&ms, h->kev0, h->kev1));
+        prof.evolve_ms += ms;
+        prof.evolve_launches += launches.size();
+        for (size_t k = 0; k < slot; k++) {
+            const size_t i = order[k];
+            const long reached = st[2 * i + 1];
+            prof.generations += static_cast<size_t>(reached - next_gen[i]);
+            next_gen[i] = reached;
+            if (st[2 * i]) {
+                stopped_at[i] = reached;
+                active[i] = 0;
+            } else if (reached >= static_cast<long>(options[i].number_of_generations) - 1) {
+                active[i] = 0;
+            }
+        }
+        first = false;
+    }
+    BATCH_HIP(hipEventRecord(h->ev1, s));
+    BATCH_HIP(hipEventSynchronize(h->ev1));
+    float loop = 0.0f;
+    BATCH_HIP(hipEventElapsedTime(&loop, h->ev0, h->ev1));
+    prof.loop_ms = loop;
+    h->last = prof;
+
+    // Epilogue per instance (deac.cpp:1576-1685), as deac_run.
+    std::vector<double> fit_all(n * deac::kBatchMaxP);
+    BATCH_HIP(hipMemcpy(fit_all.data(), h->fitness, fit_all.size() * sizeof(double), hipMemcpyDeviceToHost));
+    BATCH_HIP(hipMemcpy(rng_host.data(), h->rng, rng_host.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    std::vector<double> row;
+    for (size_t i = 0; i < n; i++) {
+        const BatchModel &b = *h->models[h->model_of[i]];
+        const deac::Model &m = b.m;
+        const size_t P = m.P, N = options[i].number_of_generations;
+        deac_result *r = &results[i];
+        const double *fit = &fit_all[i * deac::kBatchMaxP];
+        size_t best = 0;
+        double minimum = fit[0];
+        for (size_t k = 0; k < P; k++)
+            if (fit[k] < minimum) {
+                minimum = fit[k];
+                best = k;
+            }
+        row.resize(m.Kc);
+        BATCH_HIP(hipMemcpy(row.data(), h->pop + h->pop_off[i] + best * b.Lpad, m.Kc * sizeof(double),
+                            hipMemcpyDeviceToHost));
+        deac::output_transform(m, row.data(), r->dsf, r->frequency);
+        size_t generation = stopped_at[i] >= 0 ? static_cast<size_t>(stopped_at[i]) : N - 2;
+        if (stats[i]) {
+            std::vector<double> hs(3 * N, 0.0);
+            const size_t k = generation + 1;
+            BATCH_HIP(hipMemcpy(hs.data(), stats[i], 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+            if (r->stats_mean) std::memcpy(r->stats_mean, hs.data(), k * sizeof(double));
+            if (r->stats_minimum) std::memcpy(r->stats_minimum, hs.data() + N, k * sizeof(double));
+            if (r->stats_squared_mean) std::memcpy(r->stats_squared_mean, hs.data() + 2 * N, k * sizeof(double));
+        }
+        if (generation + 2 == N) {  // deac.cpp:1677-1685
+            generation += 1;
+            if (stats[i]) {
+                if (r->stats_mean) r->stats_mean[generation] = deac::mean_of(fit, P);
+                if (r->stats_minimum) r->stats_minimum[generation] = minimum;
+                if (r->stats_squared_mean) r->stats_squared_mean[generation] = deac::squared_mean_of(fit, P);
+            }
+        }
+        r->generation = generation;
+        r->minimum_fitness = minimum;
+        r->best_index = best;
+        if (r->final_fitness) std::memcpy(r->final_fitness, fit, P * sizeof(double));
+        std::memcpy(&rng_states[4 * i], &rng_host[4 * i], 4 * sizeof(uint64_t));
+    }
+    return DEAC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,677 @@
+// Batched DEAC on gfx950: one persistent workgroup per reconstruction.
+//
+// Each instance runs the reference generation loop (deac.cpp:1138-1573) with
+// the reference's random stream and FP64 evaluation order, so its results are
+// those of deac_run (and of the reference CPU build) bit for bit.  Instances
+// are independent: no grid synchronisation, and a workgroup whose instance
+// stops early frees its CU for the next queued instance.
+//
+// Wavefront roles inside a workgroup (896 threads):
+//   0-3   chain waves: model ISF E = K.S on the FP64 matrix cores.  Each
+//         v_mfma_f64_16x16x4f64 continues 256 sequential fused chains by four
+//         genome positions in ascending order, which is exactly the
+//         reference's FMA chain (matrix_multiply_LxM_by_MxN, deac.cpp:110-118).
+//   4-11  workers: stage the population chunk by chunk into LDS, rebuild the
+//         genomes accepted in the previous generation, form this
+//         generation's trial vectors (deac.cpp:1261-1311), and draw the next
+//         generation's crossover masks from per-lane sub-streams.
+//   12    walk wave: the sequential part of the random stream (jDE redraws,
+//         donor indices, mask-state skips) two generations ahead.
+//   13    remainder wave: kernel rows past the last full 16-row tile as plain
+//         fused chains on the vector ALU.
+// The genome is processed in chunks of C positions (C * padded members = 2048
+// doubles); the stages of one chunk run in consecutive steps separated by
+// workgroup barriers: load(c), rebuild(c-1), trial(c-2), chains(c-3).
+//
+// Population storage: [P][Lpad] member-major, one buffer.  A generation's
+// accepted trials are not written when they are selected; the next
+// generation recomputes them from the same inputs (stored genomes, masks, F,
+// donors) while its chunks pass through LDS, and writes them back then, so
+// HBM sees one read of the population and one write per accepted genome.
+#include <hip/hip_runtime.h>
+
+#include "batch_kernels.hpp"
+#include "device_common.hpp"
+
+namespace deac {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kWorker0 = 4;
+constexpr int kWalkWave = 12;
+constexpr int kCD = kBatchChunkDoubles;
+constexpr int kOffPop = 0;             // 3 buffers: genomes as stored, [Ppad][C]
+constexpr int kOffNew = 3 * kCD;       // 2 buffers: rebuilt (accepted) genomes, [Ppad][C]
+constexpr int kOffTrial = 5 * kCD;     // 2 buffers: trial genomes in MFMA B order
+constexpr int kRemRing = 1024;         // doubles per remainder-row kernel buffer (2 buffers)
+
+enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3 };
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const void global_cvoid;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Random choices of one member in one generation (deac.cpp:1180-1240).
+struct WalkRec {
+    uint64_t mask_state[4];   // state before the member's crossover draws
+    double cr_value[2];       // redrawn CR per sign (valid when flagged)
+    double f_value[2];        // redrawn F per sign
+    uint32_t mutant[3];
+    uint32_t flags;           // bit 2s: CR redrawn, bit 2s+1: F redrawn
+};
+
+struct BatchShared {
+    uint64_t rows[256 * 4];                  // T^L by rows
+    WalkRec rec[4][kBatchMaxP];              // ring over generations
+    uint64_t gen_start[4][4];                // state at the top of each generation
+    uint64_t walk_state[4];                  // walk position between passes
+    double fitness[kBatchMaxP];
+    double cr_old[kBatchMaxP][2], f_old[kBatchMaxP][2];
+    double cr_trial[2][kBatchMaxP][2], f_trial[2][kBatchMaxP][2];
+    double y0t[kBatchMaxP], y0r[kBatchMaxP];  // trial value at position 0 (zero-frequency match)
+    int accepted[3][kBatchMaxP];
+    int flag;                                 // 0 evolve, 1 stop test fired, 2 end of the launch
+};
+
+// LDS index of trial value (member m, chunk position sl) in the order the
+// MFMA B operand reads it: 4-position blocks of 16 members, lane
+// (sl & 3) * 16 + (m & 15), with the member XOR-swizzled by sl & 15 so that
+// both the writers (one member, 64 consecutive positions) and the readers
+// (one block) touch 16 distinct bank pairs per 16 lanes.
+template <int NM>
+__device__ __forceinline__ int trial_index(int m, int sl)
+{
+    return ((sl >> 2) * NM + (m >> 4)) * 64 + (sl & 3) * 16 + ((m & 15) ^ (sl & 15));
+}
+
+// Workgroup barrier of the chunk pipeline: LDS traffic retired, global loads
+// in flight are kept (each wave waits for what it must before arriving).
+__device__ __forceinline__ void step_barrier()
+{
+    DEAC_WAIT_LDS();
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// u < cr for u = (x >> 11) * 2^-53  <=>  (x >> 11) < ceil(cr * 2^53).
+__device__ __forceinline__ uint64_t cr_threshold(double cr)
+{
+    if (!(cr > 0.0)) return 0;
+    if (cr >= 1.0) return UINT64_C(1) << 53;
+    return static_cast<uint64_t>(ceil(cr * 0x1.0p53));
+}
+
+__device__ __forceinline__ double de_value(uint32_t bit, double F, double par, double x1, double x2, double x3,
+                                           int allow_neg)
+{
+    if (!bit) return par;
+    const double v = fma(F, x2 - x3, x1);  // deac.cpp:1275-1290
+    return allow_neg ? v : fabs(v);
+}
+
+// s <- M s with M given as a nibble table tab[n][v] = M (v << 4n).
+__device__ __forceinline__ void nibble_jump(const uint64_t *tab, uint64_t (&s)[4])
+{
+    uint64_t o[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+#pragma unroll 4
+        for (int n = 0; n < 16; n++) {
+            const int v = static_cast<int>((s[w] >> (4 * n)) & 15);
+            const uint64_t *e = tab + ((w * 16 + n) * 16 + v) * 4;
+            o[0] ^= e[0];
+            o[1] ^= e[1];
+            o[2] ^= e[2];
+            o[3] ^= e[3];
+        }
+    }
+    for (int w = 0; w < 4; w++) s[w] = o[w];
+}
+
+// One item of the sequential walk of generation `gen`: items 0..P-1 are the
+// jDE redraws of member i (CR+, F+, CR-, F-: deac.cpp:1180-1213), items
+// P..2P-1 the donor indices of member i (deac.cpp:200-216) followed by a skip
+// of its nsign*G crossover draws.  Every lane carries the same state.
+__device__ void walk_item(const BatchDesc &d, BatchShared &sh, uint64_t (&s)[4], long gen, int item, int lane)
+{
+    const int slot = static_cast<int>(gen & 3);
+    const int P = d.P;
+    if (item == 0 && lane == 0)
+        for (int w = 0; w < 4; w++) sh.gen_start[slot][w] = s[w];
+    if (item < P) {
+        uint32_t flags = 0;
+        double crv[2] = {0.0, 0.0}, fv[2] = {0.0, 0.0};
+        for (int sg = 0; sg < d.nsign; sg++) {
+            if (to_unit(xnext(s)) < d.tau_cr) {
+                flags |= 1u << (2 * sg);
+                crv[sg] = to_unit(xnext(s));
+            }
+            if (to_unit(xnext(s)) < d.tau_f) {
+                flags |= 2u << (2 * sg);
+                fv[sg] = 2.0 * to_unit(xnext(s));
+            }
+        }
+        if (lane == 0) {
+            WalkRec &r = sh.rec[slot][item];
+            r.flags = flags;
+            r.cr_value[0] = crv[0];
+            r.cr_value[1] = crv[1];
+            r.f_value[0] = fv[0];
+            r.f_value[1] = fv[1];
+        }
+    } else {
+        const uint32_t i0 = static_cast<uint32_t>(item - P), Pu = static_cast<uint32_t>(P);
+        uint32_t m0 = i0, m1 = i0, m2 = i0;
+        while (m0 == i0) m0 = mod_p(xnext(s), d.mod_magic, Pu);
+        while (m1 == i0 || m1 == m0) m1 = mod_p(xnext(s), d.mod_magic, Pu);
+        while (m2 == i0 || m2 == m0 || m2 == m1) m2 = mod_p(xnext(s), d.mod_magic, Pu);
+        if (lane == 0) {
+            WalkRec &r = sh.rec[slot][i0];
+            r.mutant[0] = m0;
+            r.mutant[1] = m1;
+            r.mutant[2] = m2;
+            for (int w = 0; w < 4; w++) r.mask_state[w] = s[w];
+        }
+        // s <- T^L s: lane l forms output bits l, 64+l, 128+l, 192+l as
+        // parities of (row & s); ballots assemble the words.
+        uint64_t out[4];
+        for (int w = 0; w < 4; w++) {
+            const uint64_t *r = sh.rows + (w * 64 + lane) * 4;
+            const uint64_t x = (r[0] & s[0]) ^ (r[1] & s[1]) ^ (r[2] & s[2]) ^ (r[3] & s[3]);
+            out[w] = __ballot(parity64(x));
+        }
+        for (int w = 0; w < 4; w++) s[w] = out[w];
+    }
+}
+
+// Crossover-mask generator of one worker lane: a sub-stream of Lsub draws of
+// one member's mask stream, reached by a nibble-table jump.  Each draw sets
+// two candidate bits: the member's CR if its trial is rejected in the
+// generation running now, and the trial's CR if it is accepted (a redrawn CR
+// makes both the same).  The next generation picks one by the selection.
+struct MaskGen {
+    uint64_t s[4];
+    uint64_t thr[2][2];   // [candidate][sign]
+    uint32_t wa, wb;
+    int pos, end;
+    uint32_t *out;
+};
+
+template <int NM>
+__device__ void mask_setup(MaskGen &mg, const BatchDesc &d, const BatchShared &sh, long mgen, bool evolving, long g,
+                           int wl)
+{
+    constexpr int Ppad = 16 * NM;
+    const int m = wl / d.S, k = wl % d.S;
+    mg.pos = mg.end = 0;
+    mg.wa = mg.wb = 0;
+    mg.out = d.masks;
+    if (m >= d.P) return;
+    const WalkRec &r = sh.rec[mgen & 3][m];
+    for (int sg = 0; sg < 2; sg++) {
+        double a = 0.0, b = 0.0;
+        if (sg < d.nsign) {
+            const bool redraw = (r.flags >> (2 * sg)) & 1;
+            a = redraw ? r.cr_value[sg] : sh.cr_old[m][sg];
+            b = redraw ? r.cr_value[sg] : (evolving ? sh.cr_trial[g & 1][m][sg] : a);
+        }
+        mg.thr[0][sg] = cr_threshold(a);
+        mg.thr[1][sg] = cr_threshold(b);
+    }
+    uint64_t s[4] = {r.mask_state[0], r.mask_state[1], r.mask_state[2], r.mask_state[3]};
+    if (k > 0) nibble_jump(d.sub_tab + static_cast<size_t>(k - 1) * (64 * 16 * 4), s);
+    for (int w = 0; w < 4; w++) mg.s[w] = s[w];
+    mg.pos = min(k * d.Lsub, d.L);
+    mg.end = min(mg.pos + d.Lsub, d.L);
+    mg.out = d.masks + (static_cast<size_t>(mgen % 3) * Ppad + m) * 2 * d.Lw;
+}
+
+__device__ __forceinline__ void mask_draws(MaskGen &mg, int q, const BatchDesc &d)
+{
+    for (int u = 0; u < q; u++) {
+        if (mg.pos >= mg.end) break;
+        const uint64_t r = xnext(mg.s) >> 11;
+        const bool neg = d.nsign == 2 && mg.pos >= d.G;
+        const uint32_t bit = 1u << (mg.pos & 31);
+        if (r < (neg ? mg.thr[0][1] : mg.thr[0][0])) mg.wa |= bit;
+        if (r < (neg ? mg.thr[1][1] : mg.thr[1][0])) mg.wb |= bit;
+        if ((mg.pos & 31) == 31 || mg.pos + 1 == mg.end) {
+            mg.out[mg.pos >> 5] = mg.wa;
+            mg.out[d.Lw + (mg.pos >> 5)] = mg.wb;
+            mg.wa = mg.wb = 0;
+        }
+        mg.pos++;
+    }
+}
+
+// One pass over the genome in chunks.  kind: kPassInit (model ISF of the
+// stored population), kPassEvolve (generation g), kPassApply (write back the
+// genomes accepted in generation g), kPassPre (masks only).  mgen: the
+// generation whose masks the workers draw (-1: none); wgen: the generation
+// the walk wave walks (-1: none).
+template <int NT, int NM>
+__device__ void run_pass(const BatchDesc &d, BatchShared &sh, double *lds, int kind, long g, long mgen, long wgen)
+{
+    constexpr int C = 128 / NM, Ppad = 16 * NM;
+    constexpr int KD = NT == 1 ? 8 : (NT == 2 ? 4 : 2);   // kernel-tile prefetch depth (pairs of blocks)
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int nchunk = d.nchunk;
+    const bool chains = kind == kPassInit || kind == kPassEvolve;
+    const int nsteps = kind == kPassApply ? nchunk + 1 : nchunk + 3;
+
+    if (wave < kWorker0) {
+        // ------------------------------------------------ chain waves
+        const double2 *kt = reinterpret_cast<const double2 *>(d.ktile);
+        const int nkb2 = d.Lpad / 8;
+        int tile[NT];
+        bool tv[NT];
+#pragma unroll
+        for (int i = 0; i < NT; i++) {
+            tile[i] = wave + 4 * i;
+            tv[i] = chains && tile[i] < d.ntt;
+        }
+        int boff[4];
+        {
+            const int q = lane >> 4, j = lane & 15;
+#pragma unroll
+            for (int u = 0; u < 4; u++) boff[u] = q * 16 + (j ^ ((4 * u + q) & 15));
+        }
+        d4 acc[NT][NM];
+#pragma unroll
+        for (int i = 0; i < NT; i++)
+#pragma unroll
+            for (int mt = 0; mt < NM; mt++) acc[i][mt] = d4{0.0, 0.0, 0.0, 0.0};
+        double2 kr[KD][NT];
+#pragma unroll
+        for (int p = 0; p < KD; p++)
+#pragma unroll
+            for (int i = 0; i < NT; i++)
+                if (tv[i] && p < nkb2) kr[p][i] = kt[(static_cast<size_t>(tile[i]) * nkb2 + p) * kWave + lane];
+        for (int c = 0; c < nsteps; c++) {
+            const int cm = c - 3;
+            if (chains && cm >= 0 && cm < nchunk) {
+                const double *tb = lds + kOffTrial + (cm & 1) * kCD;
+#pragma unroll
+                for (int jj = 0; jj < C / 8; jj++) {
+                    const int sl = jj % KD;
+                    const int g2 = cm * (C / 8) + jj;
+#pragma unroll
+                    for (int e = 0; e < 2; e++) {
+                        const int kbl = 2 * jj + e;
+                        double b[NM];
+#pragma unroll
+                        for (int mt = 0; mt < NM; mt++) b[mt] = tb[(kbl * NM + mt) * kWave + boff[kbl & 3]];
+#pragma unroll
+                        for (int i = 0; i < NT; i++)
+                            if (tv[i]) {
+                                const double a = e ? kr[sl][i].y : kr[sl][i].x;
+#pragma unroll
+                                for (int mt = 0; mt < NM; mt++)
+                                    acc[i][mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[mt], acc[i][mt], 0, 0, 0);
+                            }
+                    }
+#pragma unroll
+                    for (int i = 0; i < NT; i++)
+                        if (tv[i] && g2 + KD < nkb2)
+                            kr[sl][i] = kt[(static_cast<size_t>(tile[i]) * nkb2 + g2 + KD) * kWave + lane];
+                }
+            }
+            step_barrier();
+        }
+        __syncthreads();
+        // E[t][m] into LDS (the chunk buffers are free now): lane l holds
+        // rows 4v + l/16 of the tile, column l%16.
+        if (chains) {
+#pragma unroll
+            for (int i = 0; i < NT; i++)
+                if (tv[i])
+#pragma unroll
+                    for (int mt = 0; mt < NM; mt++) {
+                        const int m = mt * 16 + (lane & 15);
+                        if (m < d.P)
+#pragma unroll
+                            for (int v = 0; v < 4; v++) {
+                                const int t = tile[i] * 16 + 4 * v + (lane >> 4);
+                                lds[t * Ppad + m] = acc[i][mt][v];
+                            }
+                    }
+        }
+    } else if (wave < kWalkWave) {
+        // ------------------------------------------------ workers
+        const int wk = wave - kWorker0;
+        const int wl = wk * kWave + lane;
+        constexpr int kSegs = C / 64;
+        MaskGen mg;
+        const bool draws = mgen >= 0;
+        const int q = draws ? (d.Lsub + nsteps - 1) / nsteps : 0;
+        if (draws) mask_setup<NM>(mg, d, sh, mgen, kind == kPassEvolve, g, wl);
+        const bool rebuild = kind == kPassEvolve || kind == kPassApply;
+        const long pg = kind == kPassEvolve ? g - 1 : g;      // generation whose accepted trials are rebuilt
+        const int pslot = static_cast<int>((pg + 3) % 3);
+        const int pprev = static_cast<int>((pg + 2) % 3);     // generation pg-1
+        const int gslot = static_cast<int>(g % 3 + 3) % 3;
+        for (int c = 0; c < nsteps; c++) {
+            // load(c): the stored genomes of chunk c, straight into LDS
+            if (kind != kPassPre && c < nchunk) {
+                for (int j = wk; j < 16; j += 8) {
+                    const int od = j * 128 + lane * 2;
+                    const int m = od / C, x = od % C;
+                    double *dst = lds + kOffPop + (c % 3) * kCD + j * 128;
+                    if (m < d.P)
+                        __builtin_amdgcn_global_load_lds(
+                            (global_cvoid *)(d.pop + static_cast<size_t>(m) * d.Lpad + static_cast<size_t>(c) * C + x),
+                            (lds_void *)dst, 16, 0, 0);
+                }
+            }
+            // rebuild(c-1): genomes accepted in generation pg
+            const int c1 = c - 1;
+            if (rebuild && pg >= 0 && c1 >= 0 && c1 < nchunk) {
+                const double *popb = lds + kOffPop + (c1 % 3) * kCD;
+                double *newb = lds + kOffNew + (c1 & 1) * kCD;
+                for (int j = 0; j < 4; j++) {
+                    const int t = wk + 8 * j;
+                    const int m = t / kSegs, seg = t % kSegs;
+                    if (m >= d.P || !sh.accepted[pslot][m]) continue;
+                    const WalkRec &r = sh.rec[pg & 3][m];
+                    const int cand = sh.accepted[pprev][m];
+                    const uint32_t *mw = d.masks + (static_cast<size_t>(pslot) * Ppad + m) * 2 * d.Lw + cand * d.Lw;
+                    const int sl = seg * 64 + lane, s = c1 * C + sl;
+                    const double *pr = popb + m * C, *r1 = popb + r.mutant[0] * C, *r2 = popb + r.mutant[1] * C,
+                                 *r3 = popb + r.mutant[2] * C;
+                    const int sg = (d.nsign == 2 && s >= d.G) ? 1 : 0;
+                    double v = de_value((mw[s >> 5] >> (s & 31)) & 1, sh.f_trial[pg & 1][m][sg], pr[sl], r1[sl],
+                                        r2[sl], r3[sl], d.allow_neg);
+                    if (d.nsign == 2 && s == d.G)  // deac.cpp:1305-1308
+                        v = c1 == 0 ? de_value(mw[0] & 1, sh.f_trial[pg & 1][m][0], pr[0], r1[0], r2[0], r3[0],
+                                               d.allow_neg)
+                                    : sh.y0r[m];
+                    if (s == 0) sh.y0r[m] = v;
+                    if (s >= d.L) v = 0.0;
+                    newb[m * C + sl] = v;
+                    if (s < d.L) d.pop[static_cast<size_t>(m) * d.Lpad + s] = v;
+                }
+            }
+            // trial(c-2): this generation's trial vectors (or, for the
+            // initial evaluation, the stored genomes) in MFMA B order
+            const int c2 = c - 2;
+            if (chains && c2 >= 0 && c2 < nchunk) {
+                const double *popb = lds + kOffPop + (c2 % 3) * kCD;
+                const double *newb = lds + kOffNew + (c2 & 1) * kCD;
+                double *tb = lds + kOffTrial + (c2 & 1) * kCD;
+                const int aslot = static_cast<int>((g + 2) % 3);  // generation g-1
+                for (int j = 0; j < 4; j++) {
+                    const int t = wk + 8 * j;
+                    const int m = t / kSegs, seg = t % kSegs;
+                    if (m >= d.P) continue;
+                    const int sl = seg * 64 + lane, s = c2 * C + sl;
+                    double v;
+                    if (kind == kPassInit) {
+                        v = popb[m * C + sl];
+                    } else {
+                        const WalkRec &r = sh.rec[g & 3][m];
+                        const int cand = sh.accepted[aslot][m];
+                        const uint32_t *mw =
+                            d.masks + (static_cast<size_t>(gslot) * Ppad + m) * 2 * d.Lw + cand * d.Lw;
+                        const double *pr = (cand ? newb : popb) + m * C;
+                        const double *r1 = (sh.accepted[aslot][r.mutant[0]] ? newb : popb) + r.mutant[0] * C;
+                        const double *r2 = (sh.accepted[aslot][r.mutant[1]] ? newb : popb) + r.mutant[1] * C;
+                        const double *r3 = (sh.accepted[aslot][r.mutant[2]] ? newb : popb) + r.mutant[2] * C;
+                        const int sg = (d.nsign == 2 && s >= d.G) ? 1 : 0;
+                        v = de_value((mw[s >> 5] >> (s & 31)) & 1, sh.f_trial[g & 1][m][sg], pr[sl], r1[sl], r2[sl],
+                                     r3[sl], d.allow_neg);
+                        if (d.nsign == 2 && s == d.G)
+                            v = c2 == 0 ? de_value(mw[0] & 1, sh.f_trial[g & 1][m][0], pr[0], r1[0], r2[0], r3[0],
+                                                   d.allow_neg)
+                                        : sh.y0t[m];
+                        if (s == 0) sh.y0t[m] = v;
+                    }
+                    if (s >= d.L) v = 0.0;
+                    tb[trial_index<NM>(m, sl)] = v;
+                }
+            }
+            if (draws) mask_draws(mg, q, d);
+            DEAC_WAIT_VM(0);
+            step_barrier();
+        }
+        __syncthreads();
+    } else if (wave == kWalkWave) {
+        // ------------------------------------------------ walk wave
+        const int items = 2 * d.P;
+        const int per = wgen >= 0 ? (items + nsteps - 1) / nsteps : 0;
+        uint64_t ws[4] = {sh.walk_state[0], sh.walk_state[1], sh.walk_state[2], sh.walk_state[3]};
+        int next = 0;
+        for (int c = 0; c < nsteps; c++) {
+            const int stop = min(items, (c + 1) * per);
+            for (; next < stop; next++) walk_item(d, sh, ws, wgen, next, lane);
+            step_barrier();
+        }
+        if (lane == 0)
+            for (int w = 0; w < 4; w++) sh.walk_state[w] = ws[w];
+        __syncthreads();
+    } else {
+        // ------------------------------------------------ remainder wave
+        const int nch = d.nrem * Ppad;
+        const int nu = (nch + kWave - 1) / kWave;
+        double racc[kBatchRemChains];
+        int rr[kBatchRemChains], rm[kBatchRemChains];
+#pragma unroll
+        for (int u = 0; u < kBatchRemChains; u++) {
+            const int k = lane + kWave * u;
+            racc[u] = 0.0;
+            rr[u] = k < nch ? k / Ppad : 0;
+            rm[u] = k < nch ? k % Ppad : 0;
+        }
+        double *kring = lds + 7 * kCD;
+        const int ncopy = (d.nrem * C + 127) / 128;
+        for (int c = 0; c < nsteps; c++) {
+            const int cm = c - 3;
+            // kernel rows of chunk cm+1 into the ring (used next step)
+            const int cn = cm + 1;
+            if (chains && nch > 0 && cn >= 0 && cn < nchunk) {
+                for (int j = 0; j < ncopy; j++) {
+                    const int od = j * 128 + lane * 2;
+                    const int r = od / C, x = od % C;
+                    if (r < d.nrem)
+                        __builtin_amdgcn_global_load_lds(
+                            (global_cvoid *)(d.krem + static_cast<size_t>(r) * d.Lpad + static_cast<size_t>(cn) * C + x),
+                            (lds_void *)(kring + (cn & 1) * kRemRing + j * 128), 16, 0, 0);
+                }
+            }
+            if (chains && nch > 0 && cm >= 0 && cm < nchunk) {
+                const double *tb = lds + kOffTrial + (cm & 1) * kCD;
+                const double *kb = kring + (cm & 1) * kRemRing;
+#pragma unroll 8
+                for (int sl = 0; sl < C; sl++) {
+#pragma unroll
+                    for (int u = 0; u < kBatchRemChains; u++)
+                        if (u < nu && lane + kWave * u < nch)
+                            racc[u] = fma(kb[rr[u] * C + sl], tb[trial_index<NM>(rm[u], sl)], racc[u]);
+                }
+            }
+            DEAC_WAIT_VM(0);
+            step_barrier();
+        }
+        __syncthreads();
+        if (chains)
+#pragma unroll
+            for (int u = 0; u < kBatchRemChains; u++)
+                if (u < nu && lane + kWave * u < nch && rm[u] < d.P) lds[(d.ntt * 16 + rr[u]) * Ppad + rm[u]] = racc[u];
+    }
+    __syncthreads();
+}
+
+}  // namespace
+
+template <int NT, int NM>
+__global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *__restrict__ descs)
+{
+    constexpr int Ppad = 16 * NM;
+    extern __shared__ double lds[];
+    __shared__ BatchShared sh;
+    const BatchDesc &d = descs[blockIdx.x];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int P = d.P;
+
+    for (int k = threadIdx.x; k < 256 * 4; k += kBatchThreads) sh.rows[k] = d.walk_rows[k];
+    for (int k = threadIdx.x; k < 2 * kCD; k += kBatchThreads) lds[kOffTrial + k] = 0.0;  // padded members stay 0
+    if (threadIdx.x < kBatchMaxP) {
+        const int m = threadIdx.x;
+        sh.fitness[m] = m < P ? d.fitness[m] : 0.0;
+        for (int sg = 0; sg < 2; sg++) {
+            sh.cr_old[m][sg] = m < P ? d.cr_old[m * 2 + sg] : 0.0;
+            sh.f_old[m][sg] = m < P ? d.f_old[m * 2 + sg] : 0.0;
+        }
+        for (int k = 0; k < 3; k++) sh.accepted[k][m] = 0;
+    }
+    __syncthreads();
+
+    // Prologue: the walk of the first generation whose masks are drawn.
+    const long g0 = d.gen0;
+    if (wave == kWalkWave) {
+        uint64_t ws[4] = {d.rng[0], d.rng[1], d.rng[2], d.rng[3]};
+        for (int item = 0; item < 2 * P; item++) walk_item(d, sh, ws, g0, item, lane);
+        if (lane == 0)
+            for (int w = 0; w < 4; w++) sh.walk_state[w] = ws[w];
+    }
+    __syncthreads();
+
+    if (d.init) {
+        // Initial fitness (deac.cpp:857-976) while the masks of generation 0
+        // are drawn and generation 1 is walked.
+        run_pass<NT, NM>(d, sh, lds, kPassInit, -1, g0, g0 + 1);
+        if (wave == kWorker0 && lane < P) {
+            double chi = 0.0;
+            const int tlast = (d.T & 1) ? d.T - 1 : -1;
+            for (int t = 0; t < d.T; t++) {
+                const double dv = (d.obs[t] - lds[t * Ppad + lane]) / d.sig[t];
+                chi = t == tlast ? fma(dv, dv, chi) : chi + dv * dv;
+            }
+            sh.fitness[lane] = chi / static_cast<double>(d.T);
+        }
+        __syncthreads();
+    } else {
+        run_pass<NT, NM>(d, sh, lds, kPassPre, g0 - 1, g0, g0 + 1);
+    }
+
+    long g = g0;
+    long last = -1;   // last generation evolved in this launch
+    for (;; g++) {
+        // Top of the generation: best fitness, statistics, stop test
+        // (deac.cpp:1145-1166); this generation's CR and F (deac.cpp:1180-1213).
+        if (wave == kWorker0) {
+            if (lane == 0) {
+                int flag = 0;
+                if (g >= d.gen_end) {
+                    flag = 2;
+                } else {
+                    double mn = sh.fitness[0];
+                    for (int i = 0; i < P; i++)
+                        if (sh.fitness[i] < mn) mn = sh.fitness[i];
+                    if (d.stats) {
+                        double s = 0.0, q = 0.0;
+                        const int p4 = P & ~3;
+                        for (int i = 0; i < P; i++) {
+                            const double v = sh.fitness[i];
+                            s += v;
+                            q = i < p4 ? q + v * v : fma(v, v, q);
+                        }
+                        d.stats[g] = s / static_cast<double>(P);
+                        d.stats[d.stats_len + g] = mn;
+                        d.stats[2 * d.stats_len + g] = q / static_cast<double>(P);
+                    }
+                    if (mn <= d.stop) flag = 1;
+                }
+                sh.flag = flag;
+            }
+            if (lane < P) {
+                const WalkRec &r = sh.rec[g & 3][lane];
+                for (int sg = 0; sg < 2; sg++) {
+                    sh.cr_trial[g & 1][lane][sg] = (r.flags >> (2 * sg)) & 1 ? r.cr_value[sg] : sh.cr_old[lane][sg];
+                    sh.f_trial[g & 1][lane][sg] = (r.flags >> (2 * sg + 1)) & 1 ? r.f_value[sg] : sh.f_old[lane][sg];
+                }
+            }
+        }
+        __syncthreads();
+        if (sh.flag) break;
+
+        run_pass<NT, NM>(d, sh, lds, kPassEvolve, g, g + 1, g + 2);
+
+        // Fitness and greedy selection (deac.cpp:1539-1572).
+        if (wave == kWorker0 && lane < Ppad) {
+            int acc = 0;
+            if (lane < P) {
+                double chi = 0.0;
+                const int tlast = (d.T & 1) ? d.T - 1 : -1;
+                for (int t = 0; t < d.T; t++) {
+                    const double dv = (d.obs[t] - lds[t * Ppad + lane]) / d.sig[t];
+                    chi = t == tlast ? fma(dv, dv, chi) : chi + dv * dv;
+                }
+                const double fit = chi / static_cast<double>(d.T);
+                if (fit <= sh.fitness[lane]) {
+                    acc = 1;
+                    sh.fitness[lane] = fit;
+                    for (int sg = 0; sg < 2; sg++) {
+                        sh.cr_old[lane][sg] = sh.cr_trial[g & 1][lane][sg];
+                        sh.f_old[lane][sg] = sh.f_trial[g & 1][lane][sg];
+                    }
+                }
+            }
+            sh.accepted[g % 3][lane] = acc;
+        }
+        __syncthreads();
+        last = g;
+    }
+    // Write back the genomes accepted in the last generation evolved.
+    if (last >= 0) run_pass<NT, NM>(d, sh, lds, kPassApply, last, -1, -1);
+
+    if (threadIdx.x < P) {
+        const int m = threadIdx.x;
+        d.fitness[m] = sh.fitness[m];
+        for (int sg = 0; sg < 2; sg++) {
+            d.cr_old[m * 2 + sg] = sh.cr_old[m][sg];
+            d.f_old[m * 2 + sg] = sh.f_old[m][sg];
+        }
+    }
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < 4; w++) d.rng[w] = sh.gen_start[g & 3][w];
+        d.status[0] = sh.flag == 1 ? 1 : 0;
+        d.status[1] = static_cast<int>(g);
+    }
+}
+
+namespace {
+
+template <int NT, int NM>
+hipError_t launch_one(const BatchDesc *descs, int count, hipStream_t s)
+{
+    constexpr size_t lds_bytes = (static_cast<size_t>(kBatchRegionDoubles) + 2 * kRemRing) * sizeof(double);
+    static bool configured = false;
+    if (!configured) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<NT, NM>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes));
+        if (e != hipSuccess) return e;
+        configured = true;
+    }
+    hipLaunchKernelGGL((batch_kernel<NT, NM>), dim3(count), dim3(kBatchThreads), lds_bytes, s, descs);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, hipStream_t s)
+{
+    if (nm == 1) {
+        if (nt == 1) return launch_one<1, 1>(descs, count, s);
+        if (nt == 2) return launch_one<2, 1>(descs, count, s);
+        if (nt == 4) return launch_one<4, 1>(descs, count, s);
+    } else if (nm == 2) {
+        if (nt == 1) return launch_one<1, 2>(descs, count, s);
+        if (nt == 2) return launch_one<2, 2>(descs, count, s);
+        if (nt == 4) return launch_one<4, 2>(descs, count, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace deac

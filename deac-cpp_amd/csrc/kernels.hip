@@ -19,6 +19,7 @@
 // written as fma().
 #include <hip/hip_runtime.h>
 
+#include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace deac {
@@ -28,37 +29,6 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kEvoThreads = 256;
 constexpr int kEvoWaves = kEvoThreads / kWave;
-
-__device__ __forceinline__ uint64_t xnext(uint64_t (&s)[4])  // rng.hpp:25-39
-{
-    const uint64_t out = s[0] + s[3];
-    const uint64_t t = s[1] << 17;
-    s[2] ^= s[0];
-    s[3] ^= s[1];
-    s[1] ^= s[2];
-    s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = (s[3] << 45) | (s[3] >> 19);
-    return out;
-}
-
-// (x >> 11) * 2^-53, exact (deac.cpp:437)
-__device__ __forceinline__ double to_unit(uint64_t x) { return static_cast<double>(x >> 11) * 0x1.0p-53; }
-
-// x % P for P < 2^32 with magic = floor((2^64-1)/P): the quotient estimate is
-// low by at most one, fixed by one conditional subtraction.
-__device__ __forceinline__ uint32_t mod_p(uint64_t x, uint64_t magic, uint32_t P)
-{
-    const uint64_t q = __umul64hi(x, magic);
-    uint64_t r = x - q * P;
-    if (r >= P) r -= P;
-    return static_cast<uint32_t>(r);
-}
-
-__device__ __forceinline__ int parity64(uint64_t x)
-{
-    return __popc(static_cast<uint32_t>(x) ^ static_cast<uint32_t>(x >> 32)) & 1;
-}
 
 // Grid-wide barrier of the cooperative evolution kernel.  The wait is
 // bounded: after kBarrierSpins polls the workgroup records a failure and every
@@ -284,10 +254,6 @@ __device__ __forceinline__ void stage_chunk_async(const EvoArgs &a, double *lds,
     }
 }
 
-// s_waitcnt with vmcnt <= n (n < 64) and the other counters unconstrained.
-#define DEAC_WAIT_VM(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8))
-// s_waitcnt lgkmcnt(0) (LDS reads retired), other counters unconstrained.
-#define DEAC_WAIT_LDS() __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4))
 
 // Workgroup barrier for the chunk pipeline.  Unlike __syncthreads it does not
 // drain outstanding global->LDS copies; every wave retires what the others

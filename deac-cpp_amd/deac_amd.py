@@ -20,7 +20,8 @@ STATUS = {0: "success", 1: "invalid argument", 2: "no usable gfx950 device", 3: 
           4: "device error", 5: "unsupported"}
 
 EXPORTED = ["deac_create", "deac_run", "deac_destroy", "deac_output_length", "deac_seed_state",
-            "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_run_profile"]
+            "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_run_profile",
+            "deac_batch_create", "deac_batch_run", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -106,6 +107,15 @@ def lib():
         L.deac_status_string.restype = C.c_char_p
         L.deac_last_run_profile.argtypes = [C.c_void_p, C.POINTER(RunProfile)]
         L.deac_last_run_profile.restype = C.c_int
+        L.deac_batch_create.argtypes = [C.POINTER(Problem), C.c_size_t, C.POINTER(C.c_void_p), C.POINTER(C.c_char_p)]
+        L.deac_batch_create.restype = C.c_int
+        L.deac_batch_run.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOptions), C.POINTER(Result)]
+        L.deac_batch_run.restype = C.c_int
+        L.deac_batch_destroy.argtypes = [C.c_void_p]
+        L.deac_batch_size.argtypes = [C.c_void_p]
+        L.deac_batch_size.restype = C.c_size_t
+        L.deac_batch_last_profile.argtypes = [C.c_void_p, C.POINTER(RunProfile)]
+        L.deac_batch_last_profile.restype = C.c_int
         _lib = L
     return _lib
 
@@ -204,6 +214,85 @@ class Engine:
                    rng=[int(v) for v in s], loop_ms=prof.loop_ms, loop_generations=prof.generations,
                    profile={k: getattr(prof, k) for k, _ in RunProfile._fields_})
         return out
+
+
+def _per_instance(v, n):
+    return list(v) if isinstance(v, (list, tuple, np.ndarray)) else [v] * n
+
+
+class Batch:
+    """One deac_batch: independent reconstructions run by one persistent launch.
+
+    Instance i computes what Engine(problems[i]).run(seeds[i], ...) computes:
+    the multi-seed workflow of the reference (one deac.e per seed) in one call.
+    """
+
+    def __init__(self, problems):
+        self.problems = list(problems)
+        n = len(self.problems)
+        arr = (Problem * n)(*self.problems)
+        self._arr = arr
+        h = C.c_void_p()
+        msg = C.c_char_p()
+        st = lib().deac_batch_create(arr, n, C.byref(h), C.byref(msg))
+        if st:
+            raise DeacError(st, msg.value.decode() if msg.value else None)
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().deac_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.problems)
+
+    def run(self, seeds, number_of_generations, stop_minimum_fitness=1.0, track_stats=True):
+        """seeds: list of --seed ints or 4-word states; N, stop, track_stats: scalar or per instance."""
+        n = len(self.problems)
+        Ns = _per_instance(number_of_generations, n)
+        stops = _per_instance(stop_minimum_fitness, n)
+        tracks = _per_instance(track_stats, n)
+        states = (C.c_uint64 * (4 * n))()
+        for i, sd in enumerate(seeds):
+            w = seed_state(sd) if isinstance(sd, (int, np.integer)) else sd
+            for k in range(4):
+                states[4 * i + k] = int(w[k])
+        opts = (RunOptions * n)(*[RunOptions(Ns[i], stops[i], int(tracks[i])) for i in range(n)])
+        outs, res = [], (Result * n)()
+        for i, p in enumerate(self.problems):
+            nout = lib().deac_output_length(C.byref(p))
+            o = {k: np.zeros(m) for k, m in (("dsf", nout), ("frequency", nout), ("mean", Ns[i]), ("minimum", Ns[i]),
+                                              ("squared_mean", Ns[i]), ("fitness", p.population_size))}
+            res[i] = Result(0, 0.0, 0, _ptr(o["dsf"]), _ptr(o["frequency"]), _ptr(o["mean"]), _ptr(o["minimum"]),
+                            _ptr(o["squared_mean"]), _ptr(o["fitness"]))
+            outs.append(o)
+        st = lib().deac_batch_run(self.handle, states, opts, res)
+        if st:
+            raise DeacError(st)
+        prof = RunProfile()
+        lib().deac_batch_last_profile(self.handle, C.byref(prof))
+        self.profile = {k: getattr(prof, k) for k, _ in RunProfile._fields_}
+        for i, o in enumerate(outs):
+            r = res[i]
+            k = r.generation + 1
+            for key in ("mean", "minimum", "squared_mean"):
+                o[key] = o[key][:k]
+            o.update(generation=r.generation, minimum_fitness=r.minimum_fitness, best_index=r.best_index,
+                     rng=[int(states[4 * i + w]) for w in range(4)])
+        return outs
 
 
 def run(problem, seed, number_of_generations, stop_minimum_fitness=1.0, track_stats=True):

@@ -148,6 +148,34 @@ typedef struct deac_run_profile {
 } deac_run_profile;
 deac_status deac_last_run_profile(const deac_handle *handle, deac_run_profile *profile);
 
+/* ---------------------------------------------------------------- batches
+ * Many independent reconstructions in one persistent launch: the reference's
+ * multi-seed workflow (tools/generate_commands.sh:18-32, one deac.e process
+ * per seed, averaged by tools/plot_simple_average_deac_results.py:8-28) as
+ * one call.  Each instance i computes exactly what deac_run would compute for
+ * problems[i] from rng_states[4i..4i+3] (plain FP64, the reference's order and
+ * random stream), with its own generation count and stop test; an instance
+ * that stops leaves the launch.  Identical problems share one device copy of
+ * the kernel matrix.  Population size at most 32; moment constraints and
+ * per-generation renormalization run only through deac_run
+ * (DEAC_ERR_UNSUPPORTED here). */
+typedef struct deac_batch deac_batch;
+
+deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_batch **out, const char **message);
+
+/* rng_states: 4*count words, in/out as deac_run's rng_state; options and
+ * results: count entries each, as deac_run's. */
+deac_status deac_batch_run(deac_batch *batch, uint64_t *rng_states, const deac_run_options *options,
+                           deac_result *results);
+
+void deac_batch_destroy(deac_batch *batch);
+
+size_t deac_batch_size(const deac_batch *batch);
+
+/* Device timing of the last deac_batch_run: loop_ms covers every launch
+ * (initial evaluation included), generations is summed over instances. */
+deac_status deac_batch_last_profile(const deac_batch *batch, deac_run_profile *profile);
+
 #ifdef __cplusplus
 }
 #endif
