@@ -157,7 +157,7 @@ deac_status plan_model(BatchModel &b, std::string &why)
         why = "batched runs take at most 271 timeslices";
         return DEAC_ERR_UNSUPPORTED;
     }
-    b.S = deac::kBatchWorkerLanes / b.Ppad;
+    b.S = deac::kBatchMaskLanes / b.Ppad;
     b.Lsub = (L + 32 * b.S - 1) / (32 * b.S) * 32;
     return DEAC_OK;
 }

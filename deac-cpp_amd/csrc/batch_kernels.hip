@@ -6,19 +6,20 @@
 // are independent: no grid synchronisation, and a workgroup whose instance
 // stops early frees its CU for the next queued instance.
 //
-// Wavefront roles inside a workgroup (896 threads):
+// Wavefront roles inside a workgroup (1024 threads, four per SIMD):
 //   0-3   chain waves: model ISF E = K.S on the FP64 matrix cores.  Each
 //         v_mfma_f64_16x16x4f64 continues 256 sequential fused chains by four
 //         genome positions in ascending order, which is exactly the
 //         reference's FMA chain (matrix_multiply_LxM_by_MxN, deac.cpp:110-118).
 //   4-11  workers: stage the population chunk by chunk into LDS, rebuild the
-//         genomes accepted in the previous generation, form this
-//         generation's trial vectors (deac.cpp:1261-1311), and draw the next
-//         generation's crossover masks from per-lane sub-streams.
+//         genomes accepted in the previous generation and form this
+//         generation's trial vectors (deac.cpp:1261-1311).
 //   12    walk wave: the sequential part of the random stream (jDE redraws,
 //         donor indices, mask-state skips) two generations ahead.
 //   13    remainder wave: kernel rows past the last full 16-row tile as plain
 //         fused chains on the vector ALU.
+//   14-15 mask waves: the next generation's crossover draws from per-lane
+//         sub-streams of each member's mask stream.
 // The genome is processed in chunks of C positions (C * padded members = 2048
 // doubles); the stages of one chunk run in consecutive steps separated by
 // workgroup barriers: load(c), rebuild(c-1), trial(c-2), chains(c-3).
@@ -40,17 +41,26 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kWorker0 = 4;
 constexpr int kWalkWave = 12;
+constexpr int kRngWave0 = 14;          // 14, 15: mask waves
 constexpr int kCD = kBatchChunkDoubles;
 constexpr int kOffPop = 0;             // 3 buffers: genomes as stored, [Ppad][C]
 constexpr int kOffNew = 3 * kCD;       // 2 buffers: rebuilt (accepted) genomes, [Ppad][C]
 constexpr int kOffTrial = 5 * kCD;     // 2 buffers: trial genomes in MFMA B order
 constexpr int kRemRing = 1024;         // doubles per remainder-row kernel buffer (2 buffers)
+constexpr int kMaskWords = 256;        // mask words per chunk: 2 generations x Ppad x 2 candidates x C/32
 
 enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3 };
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const void global_cvoid;
 typedef __attribute__((address_space(3))) void lds_void;
+// Global-memory views of the descriptor's pointers: loads and stores through
+// them are global_* instructions, counted by vmcnt only, so LDS waits never
+// wait for them.
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) uint32_t guint;
+typedef __attribute__((address_space(1))) const uint64_t gcu64;
 
 // Random choices of one member in one generation (deac.cpp:1180-1240).
 struct WalkRec {
@@ -72,6 +82,9 @@ struct BatchShared {
     double y0t[kBatchMaxP], y0r[kBatchMaxP];  // trial value at position 0 (zero-frequency match)
     int accepted[3][kBatchMaxP];
     int flag;                                 // 0 evolve, 1 stop test fired, 2 end of the launch
+    // Crossover words of the chunks in flight, [chunk % 3][generation: this,
+    // previous][member][candidate][C/32], staged from global memory.
+    uint32_t mring[3][kMaskWords];
 };
 
 // LDS index of trial value (member m, chunk position sl) in the order the
@@ -95,6 +108,15 @@ __device__ __forceinline__ void step_barrier()
     asm volatile("" ::: "memory");
 }
 
+// Two adjacent doubles (16-byte aligned) from global memory, one dwordx4 load.
+__device__ __forceinline__ double2 load2(const gcdouble *p)
+{
+    double2 v;
+    v.x = p[0];
+    v.y = p[1];
+    return v;
+}
+
 // u < cr for u = (x >> 11) * 2^-53  <=>  (x >> 11) < ceil(cr * 2^53).
 __device__ __forceinline__ uint64_t cr_threshold(double cr)
 {
@@ -112,7 +134,7 @@ __device__ __forceinline__ double de_value(uint32_t bit, double F, double par, d
 }
 
 // s <- M s with M given as a nibble table tab[n][v] = M (v << 4n).
-__device__ __forceinline__ void nibble_jump(const uint64_t *tab, uint64_t (&s)[4])
+__device__ __forceinline__ void nibble_jump(const gcu64 *tab, uint64_t (&s)[4])
 {
     uint64_t o[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -120,7 +142,7 @@ __device__ __forceinline__ void nibble_jump(const uint64_t *tab, uint64_t (&s)[4
 #pragma unroll 4
         for (int n = 0; n < 16; n++) {
             const int v = static_cast<int>((s[w] >> (4 * n)) & 15);
-            const uint64_t *e = tab + ((w * 16 + n) * 16 + v) * 4;
+            const gcu64 *e = tab + ((w * 16 + n) * 16 + v) * 4;
             o[0] ^= e[0];
             o[1] ^= e[1];
             o[2] ^= e[2];
@@ -193,10 +215,11 @@ __device__ void walk_item(const BatchDesc &d, BatchShared &sh, uint64_t (&s)[4],
 // makes both the same).  The next generation picks one by the selection.
 struct MaskGen {
     uint64_t s[4];
-    uint64_t thr[2][2];   // [candidate][sign]
-    uint32_t wa, wb;
-    int pos, end;
-    uint32_t *out;
+    uint64_t ta, tb;      // thresholds of the two candidates at the current position
+    uint64_t na, nb;      // thresholds after the sign change (negative frequencies)
+    uint32_t wa, wb, bit;
+    int pos, end, flip;   // flip: position of the sign change (or past the end)
+    guint *out;
 };
 
 template <int NM>
@@ -207,9 +230,13 @@ __device__ void mask_setup(MaskGen &mg, const BatchDesc &d, const BatchShared &s
     const int m = wl / d.S, k = wl % d.S;
     mg.pos = mg.end = 0;
     mg.wa = mg.wb = 0;
-    mg.out = d.masks;
+    mg.bit = 1;
+    mg.flip = 0;
+    mg.ta = mg.tb = mg.na = mg.nb = 0;
+    mg.out = (guint *)d.masks;
     if (m >= d.P) return;
     const WalkRec &r = sh.rec[mgen & 3][m];
+    uint64_t thr[2][2];
     for (int sg = 0; sg < 2; sg++) {
         double a = 0.0, b = 0.0;
         if (sg < d.nsign) {
@@ -217,31 +244,41 @@ __device__ void mask_setup(MaskGen &mg, const BatchDesc &d, const BatchShared &s
             a = redraw ? r.cr_value[sg] : sh.cr_old[m][sg];
             b = redraw ? r.cr_value[sg] : (evolving ? sh.cr_trial[g & 1][m][sg] : a);
         }
-        mg.thr[0][sg] = cr_threshold(a);
-        mg.thr[1][sg] = cr_threshold(b);
+        thr[0][sg] = cr_threshold(a);
+        thr[1][sg] = cr_threshold(b);
     }
     uint64_t s[4] = {r.mask_state[0], r.mask_state[1], r.mask_state[2], r.mask_state[3]};
-    if (k > 0) nibble_jump(d.sub_tab + static_cast<size_t>(k - 1) * (64 * 16 * 4), s);
+    if (k > 0) nibble_jump((const gcu64 *)d.sub_tab + static_cast<size_t>(k - 1) * (64 * 16 * 4), s);
     for (int w = 0; w < 4; w++) mg.s[w] = s[w];
     mg.pos = min(k * d.Lsub, d.L);
     mg.end = min(mg.pos + d.Lsub, d.L);
-    mg.out = d.masks + (static_cast<size_t>(mgen % 3) * Ppad + m) * 2 * d.Lw;
+    mg.bit = 1u << (mg.pos & 31);
+    mg.flip = d.nsign == 2 ? d.G : d.L;
+    const bool neg = mg.pos >= mg.flip;
+    mg.ta = neg ? thr[0][1] : thr[0][0];
+    mg.tb = neg ? thr[1][1] : thr[1][0];
+    mg.na = thr[0][1];
+    mg.nb = thr[1][1];
+    mg.out = (guint *)d.masks + (static_cast<size_t>(mgen % 3) * Ppad + m) * 2 * d.Lw;
 }
 
-__device__ __forceinline__ void mask_draws(MaskGen &mg, int q, const BatchDesc &d)
+__device__ __forceinline__ void mask_draws(MaskGen &mg, int q, int Lw)
 {
-    for (int u = 0; u < q; u++) {
-        if (mg.pos >= mg.end) break;
+    const int n = min(q, mg.end - mg.pos);
+    for (int u = 0; u < n; u++) {
+        if (mg.pos == mg.flip) {
+            mg.ta = mg.na;
+            mg.tb = mg.nb;
+        }
         const uint64_t r = xnext(mg.s) >> 11;
-        const bool neg = d.nsign == 2 && mg.pos >= d.G;
-        const uint32_t bit = 1u << (mg.pos & 31);
-        if (r < (neg ? mg.thr[0][1] : mg.thr[0][0])) mg.wa |= bit;
-        if (r < (neg ? mg.thr[1][1] : mg.thr[1][0])) mg.wb |= bit;
-        if ((mg.pos & 31) == 31 || mg.pos + 1 == mg.end) {
+        if (r < mg.ta) mg.wa |= mg.bit;
+        if (r < mg.tb) mg.wb |= mg.bit;
+        if (mg.bit == 0x80000000u || mg.pos + 1 == mg.end) {
             mg.out[mg.pos >> 5] = mg.wa;
-            mg.out[d.Lw + (mg.pos >> 5)] = mg.wb;
+            mg.out[Lw + (mg.pos >> 5)] = mg.wb;
             mg.wa = mg.wb = 0;
         }
+        mg.bit = mg.bit == 0x80000000u ? 1u : mg.bit << 1;
         mg.pos++;
     }
 }
@@ -252,25 +289,30 @@ __device__ __forceinline__ void mask_draws(MaskGen &mg, int q, const BatchDesc &
 // generation whose masks the workers draw (-1: none); wgen: the generation
 // the walk wave walks (-1: none).
 template <int NT, int NM>
-__device__ void run_pass(const BatchDesc &d, BatchShared &sh, double *lds, int kind, long g, long mgen, long wgen)
+__device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, double *lds, int kind, long g, long mgen,
+                                         long wgen)
 {
     constexpr int C = 128 / NM, Ppad = 16 * NM;
     constexpr int KD = NT == 1 ? 8 : (NT == 2 ? 4 : 2);   // kernel-tile prefetch depth (pairs of blocks)
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     const int nchunk = d.nchunk;
     const bool chains = kind == kPassInit || kind == kPassEvolve;
     const int nsteps = kind == kPassApply ? nchunk + 1 : nchunk + 3;
 
     if (wave < kWorker0) {
         // ------------------------------------------------ chain waves
-        const double2 *kt = reinterpret_cast<const double2 *>(d.ktile);
+        const gcdouble *kt = (const gcdouble *)d.ktile;
         const int nkb2 = d.Lpad / 8;
-        int tile[NT];
+        // Tiles past ntt are computed from a clamped (valid) tile and
+        // discarded: straight-line loads and MFMAs keep the prefetch waits
+        // exact.
+        int tile[NT], tload[NT];
         bool tv[NT];
 #pragma unroll
         for (int i = 0; i < NT; i++) {
             tile[i] = wave + 4 * i;
             tv[i] = chains && tile[i] < d.ntt;
+            tload[i] = min(tile[i], max(d.ntt - 1, 0));
         }
         int boff[4];
         {
@@ -288,34 +330,48 @@ __device__ void run_pass(const BatchDesc &d, BatchShared &sh, double *lds, int k
         for (int p = 0; p < KD; p++)
 #pragma unroll
             for (int i = 0; i < NT; i++)
-                if (tv[i] && p < nkb2) kr[p][i] = kt[(static_cast<size_t>(tile[i]) * nkb2 + p) * kWave + lane];
+                kr[p][i] = load2(kt + ((static_cast<size_t>(tload[i]) * nkb2 + min(p, nkb2 - 1)) * kWave + lane) * 2);
         for (int c = 0; c < nsteps; c++) {
             const int cm = c - 3;
             if (chains && cm >= 0 && cm < nchunk) {
                 const double *tb = lds + kOffTrial + (cm & 1) * kCD;
+                // B operands one block pair ahead of the MFMAs that use them
+                double bc[2][NM], bn[2][NM];
+#pragma unroll
+                for (int e = 0; e < 2; e++)
+#pragma unroll
+                    for (int mt = 0; mt < NM; mt++) bc[e][mt] = tb[(e * NM + mt) * kWave + boff[e]];
 #pragma unroll
                 for (int jj = 0; jj < C / 8; jj++) {
                     const int sl = jj % KD;
                     const int g2 = cm * (C / 8) + jj;
+                    if (jj + 1 < C / 8) {
 #pragma unroll
-                    for (int e = 0; e < 2; e++) {
-                        const int kbl = 2 * jj + e;
-                        double b[NM];
+                        for (int e = 0; e < 2; e++) {
+                            const int kbl = 2 * (jj + 1) + e;
 #pragma unroll
-                        for (int mt = 0; mt < NM; mt++) b[mt] = tb[(kbl * NM + mt) * kWave + boff[kbl & 3]];
-#pragma unroll
-                        for (int i = 0; i < NT; i++)
-                            if (tv[i]) {
-                                const double a = e ? kr[sl][i].y : kr[sl][i].x;
-#pragma unroll
-                                for (int mt = 0; mt < NM; mt++)
-                                    acc[i][mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[mt], acc[i][mt], 0, 0, 0);
-                            }
+                            for (int mt = 0; mt < NM; mt++) bn[e][mt] = tb[(kbl * NM + mt) * kWave + boff[kbl & 3]];
+                        }
                     }
 #pragma unroll
+                    for (int e = 0; e < 2; e++) {
+#pragma unroll
+                        for (int i = 0; i < NT; i++) {
+                            const double a = e ? kr[sl][i].y : kr[sl][i].x;
+#pragma unroll
+                            for (int mt = 0; mt < NM; mt++)
+                                acc[i][mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bc[e][mt], acc[i][mt], 0, 0, 0);
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 2; e++)
+#pragma unroll
+                        for (int mt = 0; mt < NM; mt++) bc[e][mt] = bn[e][mt];
+                    const int gn = min(g2 + KD, nkb2 - 1);
+#pragma unroll
                     for (int i = 0; i < NT; i++)
-                        if (tv[i] && g2 + KD < nkb2)
-                            kr[sl][i] = kt[(static_cast<size_t>(tile[i]) * nkb2 + g2 + KD) * kWave + lane];
+                        kr[sl][i] = load2(kt + ((static_cast<size_t>(tload[i]) * nkb2 + gn) * kWave + lane) * 2);
+                    __builtin_amdgcn_sched_barrier(0);  // issue the refill here, a full step ahead of its use
                 }
             }
             step_barrier();
@@ -341,19 +397,19 @@ __device__ void run_pass(const BatchDesc &d, BatchShared &sh, double *lds, int k
     } else if (wave < kWalkWave) {
         // ------------------------------------------------ workers
         const int wk = wave - kWorker0;
-        const int wl = wk * kWave + lane;
         constexpr int kSegs = C / 64;
-        MaskGen mg;
-        const bool draws = mgen >= 0;
-        const int q = draws ? (d.Lsub + nsteps - 1) / nsteps : 0;
-        if (draws) mask_setup<NM>(mg, d, sh, mgen, kind == kPassEvolve, g, wl);
+        constexpr int kWpc = C / 32;  // mask words per member and chunk
         const bool rebuild = kind == kPassEvolve || kind == kPassApply;
         const long pg = kind == kPassEvolve ? g - 1 : g;      // generation whose accepted trials are rebuilt
         const int pslot = static_cast<int>((pg + 3) % 3);
         const int pprev = static_cast<int>((pg + 2) % 3);     // generation pg-1
-        const int gslot = static_cast<int>(g % 3 + 3) % 3;
+        const int gslot = static_cast<int>((g + 3) % 3);
+        const int aslot = static_cast<int>((g + 2) % 3);      // generation g-1
+        const guint *gmask = (const guint *)d.masks;
+        gdouble *gpop = (gdouble *)d.pop;
         for (int c = 0; c < nsteps; c++) {
-            // load(c): the stored genomes of chunk c, straight into LDS
+            // load(c): the stored genomes of chunk c and its crossover words
+            // (this generation's and the rebuilt one's), straight into LDS
             if (kind != kPassPre && c < nchunk) {
                 for (int j = wk; j < 16; j += 8) {
                     const int od = j * 128 + lane * 2;
@@ -364,77 +420,157 @@ __device__ void run_pass(const BatchDesc &d, BatchShared &sh, double *lds, int k
                             (global_cvoid *)(d.pop + static_cast<size_t>(m) * d.Lpad + static_cast<size_t>(c) * C + x),
                             (lds_void *)dst, 16, 0, 0);
                 }
+                if (wk < kMaskWords / kWave) {
+                    const int e = wk * kWave + lane;
+                    const int w = e % kWpc, cand = (e / kWpc) & 1, m = (e / (2 * kWpc)) % Ppad;
+                    const int gs = e / (2 * kWpc * Ppad);
+                    const int slot = gs == 0 ? gslot : pslot;
+                    const guint *src = gmask + ((static_cast<size_t>(slot) * Ppad + m) * 2 + cand) * d.Lw +
+                                       static_cast<size_t>(c) * kWpc + w;
+                    __builtin_amdgcn_global_load_lds((global_cvoid *)src, (lds_void *)(sh.mring[c % 3] + wk * kWave), 4,
+                                                     0, 0);
+                }
             }
-            // rebuild(c-1): genomes accepted in generation pg
+            // rebuild(c-1): genomes accepted in generation pg.  Three phases
+            // over the wave's four tasks (metadata, operand loads, results) so
+            // each costs one LDS round trip for all tasks.  Operands are
+            // addressed by 32-bit offsets into the dynamic LDS array.
             const int c1 = c - 1;
             if (rebuild && pg >= 0 && c1 >= 0 && c1 < nchunk) {
-                const double *popb = lds + kOffPop + (c1 % 3) * kCD;
-                double *newb = lds + kOffNew + (c1 & 1) * kCD;
+                const int popo = kOffPop + (c1 % 3) * kCD;
+                const int newo = kOffNew + (c1 & 1) * kCD;
+                const uint32_t *mrb = sh.mring[c1 % 3] + Ppad * 2 * kWpc;
+                int o4[4][4], wo[4];
+                double Fj[4];
+#pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const int t = wk + 8 * j;
-                    const int m = t / kSegs, seg = t % kSegs;
-                    if (m >= d.P || !sh.accepted[pslot][m]) continue;
-                    const WalkRec &r = sh.rec[pg & 3][m];
-                    const int cand = sh.accepted[pprev][m];
-                    const uint32_t *mw = d.masks + (static_cast<size_t>(pslot) * Ppad + m) * 2 * d.Lw + cand * d.Lw;
+                    const int t = wk + 8 * j, m = t / kSegs, seg = t % kSegs;
+                    const int mv = m < d.P ? m : 0;
+                    const WalkRec &r = sh.rec[pg & 3][mv];
                     const int sl = seg * 64 + lane, s = c1 * C + sl;
-                    const double *pr = popb + m * C, *r1 = popb + r.mutant[0] * C, *r2 = popb + r.mutant[1] * C,
-                                 *r3 = popb + r.mutant[2] * C;
-                    const int sg = (d.nsign == 2 && s >= d.G) ? 1 : 0;
-                    double v = de_value((mw[s >> 5] >> (s & 31)) & 1, sh.f_trial[pg & 1][m][sg], pr[sl], r1[sl],
-                                        r2[sl], r3[sl], d.allow_neg);
-                    if (d.nsign == 2 && s == d.G)  // deac.cpp:1305-1308
-                        v = c1 == 0 ? de_value(mw[0] & 1, sh.f_trial[pg & 1][m][0], pr[0], r1[0], r2[0], r3[0],
-                                               d.allow_neg)
-                                    : sh.y0r[m];
+                    o4[j][0] = popo + mv * C + sl;
+                    o4[j][1] = popo + static_cast<int>(r.mutant[0]) * C + sl;
+                    o4[j][2] = popo + static_cast<int>(r.mutant[1]) * C + sl;
+                    o4[j][3] = popo + static_cast<int>(r.mutant[2]) * C + sl;
+                    wo[j] = (mv * 2 + sh.accepted[pprev][mv]) * kWpc + (sl >> 5);
+                    Fj[j] = sh.f_trial[pg & 1][mv][(d.nsign == 2 && s >= d.G) ? 1 : 0];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                double v4[4][4];
+                uint32_t w4[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) v4[j][k] = lds[o4[j][k]];
+                    w4[j] = mrb[wo[j]];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int t = wk + 8 * j, m = t / kSegs, seg = t % kSegs;
+                    if (m >= d.P || !sh.accepted[pslot][m]) continue;
+                    const int sl = seg * 64 + lane, s = c1 * C + sl;
+                    double v = de_value((w4[j] >> (s & 31)) & 1, Fj[j], v4[j][0], v4[j][1], v4[j][2], v4[j][3],
+                                        d.allow_neg);
+                    if (d.nsign == 2 && s == d.G) {  // deac.cpp:1305-1308
+                        if (c1 == 0) {
+                            const WalkRec &r = sh.rec[pg & 3][m];
+                            v = de_value(mrb[wo[j] - (sl >> 5)] & 1, sh.f_trial[pg & 1][m][0], lds[popo + m * C],
+                                         lds[popo + r.mutant[0] * C], lds[popo + r.mutant[1] * C],
+                                         lds[popo + r.mutant[2] * C], d.allow_neg);
+                        } else {
+                            v = sh.y0r[m];
+                        }
+                    }
                     if (s == 0) sh.y0r[m] = v;
                     if (s >= d.L) v = 0.0;
-                    newb[m * C + sl] = v;
-                    if (s < d.L) d.pop[static_cast<size_t>(m) * d.Lpad + s] = v;
+                    lds[newo + m * C + sl] = v;
+                    if (s < d.L) gpop[static_cast<size_t>(m) * d.Lpad + s] = v;
                 }
             }
             // trial(c-2): this generation's trial vectors (or, for the
             // initial evaluation, the stored genomes) in MFMA B order
             const int c2 = c - 2;
             if (chains && c2 >= 0 && c2 < nchunk) {
-                const double *popb = lds + kOffPop + (c2 % 3) * kCD;
-                const double *newb = lds + kOffNew + (c2 & 1) * kCD;
+                const int popo = kOffPop + (c2 % 3) * kCD;
+                const int newo = kOffNew + (c2 & 1) * kCD;
                 double *tb = lds + kOffTrial + (c2 & 1) * kCD;
-                const int aslot = static_cast<int>((g + 2) % 3);  // generation g-1
-                for (int j = 0; j < 4; j++) {
-                    const int t = wk + 8 * j;
-                    const int m = t / kSegs, seg = t % kSegs;
-                    if (m >= d.P) continue;
-                    const int sl = seg * 64 + lane, s = c2 * C + sl;
-                    double v;
-                    if (kind == kPassInit) {
-                        v = popb[m * C + sl];
-                    } else {
-                        const WalkRec &r = sh.rec[g & 3][m];
-                        const int cand = sh.accepted[aslot][m];
-                        const uint32_t *mw =
-                            d.masks + (static_cast<size_t>(gslot) * Ppad + m) * 2 * d.Lw + cand * d.Lw;
-                        const double *pr = (cand ? newb : popb) + m * C;
-                        const double *r1 = (sh.accepted[aslot][r.mutant[0]] ? newb : popb) + r.mutant[0] * C;
-                        const double *r2 = (sh.accepted[aslot][r.mutant[1]] ? newb : popb) + r.mutant[1] * C;
-                        const double *r3 = (sh.accepted[aslot][r.mutant[2]] ? newb : popb) + r.mutant[2] * C;
-                        const int sg = (d.nsign == 2 && s >= d.G) ? 1 : 0;
-                        v = de_value((mw[s >> 5] >> (s & 31)) & 1, sh.f_trial[g & 1][m][sg], pr[sl], r1[sl], r2[sl],
-                                     r3[sl], d.allow_neg);
-                        if (d.nsign == 2 && s == d.G)
-                            v = c2 == 0 ? de_value(mw[0] & 1, sh.f_trial[g & 1][m][0], pr[0], r1[0], r2[0], r3[0],
-                                                   d.allow_neg)
-                                        : sh.y0t[m];
-                        if (s == 0) sh.y0t[m] = v;
+                const uint32_t *mrt = sh.mring[c2 % 3];
+                if (kind == kPassInit) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int t = wk + 8 * j, m = t / kSegs, seg = t % kSegs;
+                        if (m >= d.P) continue;
+                        const int sl = seg * 64 + lane, s = c2 * C + sl;
+                        tb[trial_index<NM>(m, sl)] = s < d.L ? lds[popo + m * C + sl] : 0.0;
                     }
-                    if (s >= d.L) v = 0.0;
-                    tb[trial_index<NM>(m, sl)] = v;
+                } else {
+                    int o4[4][4], wo[4];
+                    double Fj[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int t = wk + 8 * j, m = t / kSegs, seg = t % kSegs;
+                        const int mv = m < d.P ? m : 0;
+                        const WalkRec &r = sh.rec[g & 3][mv];
+                        const int sl = seg * 64 + lane, s = c2 * C + sl;
+                        const int cand = sh.accepted[aslot][mv];
+                        o4[j][0] = (cand ? newo : popo) + mv * C + sl;
+#pragma unroll
+                        for (int k = 0; k < 3; k++) {
+                            const int x = static_cast<int>(r.mutant[k]);
+                            o4[j][k + 1] = (sh.accepted[aslot][x] ? newo : popo) + x * C + sl;
+                        }
+                        wo[j] = (mv * 2 + cand) * kWpc + (sl >> 5);
+                        Fj[j] = sh.f_trial[g & 1][mv][(d.nsign == 2 && s >= d.G) ? 1 : 0];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    double v4[4][4];
+                    uint32_t w4[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+#pragma unroll
+                        for (int k = 0; k < 4; k++) v4[j][k] = lds[o4[j][k]];
+                        w4[j] = mrt[wo[j]];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int t = wk + 8 * j, m = t / kSegs, seg = t % kSegs;
+                        if (m >= d.P) continue;
+                        const int sl = seg * 64 + lane, s = c2 * C + sl;
+                        double v = de_value((w4[j] >> (s & 31)) & 1, Fj[j], v4[j][0], v4[j][1], v4[j][2], v4[j][3],
+                                            d.allow_neg);
+                        if (d.nsign == 2 && s == d.G) {
+                            if (c2 == 0) {
+                                const int sl0 = o4[j][0] - sl;
+                                v = de_value(mrt[wo[j] - (sl >> 5)] & 1, sh.f_trial[g & 1][m][0], lds[sl0],
+                                             lds[o4[j][1] - sl], lds[o4[j][2] - sl], lds[o4[j][3] - sl], d.allow_neg);
+                            } else {
+                                v = sh.y0t[m];
+                            }
+                        }
+                        if (s == 0) sh.y0t[m] = v;
+                        if (s >= d.L) v = 0.0;
+                        tb[trial_index<NM>(m, sl)] = v;
+                    }
                 }
             }
-            if (draws) mask_draws(mg, q, d);
             DEAC_WAIT_VM(0);
             step_barrier();
         }
+        __syncthreads();
+    } else if (wave >= kRngWave0) {
+        // ------------------------------------------------ mask waves
+        // The next generation's crossover draws, Lsub per lane per pass.
+        MaskGen mg;
+        const bool draws = mgen >= 0;
+        const int q = draws ? (d.Lsub + nsteps - 1) / nsteps : 0;
+        if (draws) mask_setup<NM>(mg, d, sh, mgen, kind == kPassEvolve, g, (wave - kRngWave0) * kWave + lane);
+        for (int c = 0; c < nsteps; c++) {
+            if (draws) mask_draws(mg, q, d.Lw);
+            step_barrier();
+        }
+        DEAC_WAIT_VM(0);
         __syncthreads();
     } else if (wave == kWalkWave) {
         // ------------------------------------------------ walk wave
@@ -482,12 +618,32 @@ __device__ void run_pass(const BatchDesc &d, BatchShared &sh, double *lds, int k
             if (chains && nch > 0 && cm >= 0 && cm < nchunk) {
                 const double *tb = lds + kOffTrial + (cm & 1) * kCD;
                 const double *kb = kring + (cm & 1) * kRemRing;
-#pragma unroll 8
-                for (int sl = 0; sl < C; sl++) {
+                // Straight-line chains (lanes past nch run on row 0 / member
+                // 0 and are discarded) so the LDS reads pipeline.
 #pragma unroll
-                    for (int u = 0; u < kBatchRemChains; u++)
-                        if (u < nu && lane + kWave * u < nch)
-                            racc[u] = fma(kb[rr[u] * C + sl], tb[trial_index<NM>(rm[u], sl)], racc[u]);
+                for (int u = 0; u < kBatchRemChains; u++) {
+                    if (u < nu) {
+                        const double *krow = kb + rr[u] * C;
+                        double a = racc[u];
+                        // 8 positions per block; the lane offsets of a block
+                        // (trial swizzle) are formed per block.
+                        const int xm = rm[u] & 15, mb = (rm[u] >> 4) * kWave;
+#pragma unroll 1
+                        for (int s0 = 0; s0 < C; s0 += 8) {
+                            const double *tbb = tb + (s0 >> 2) * NM * kWave + mb;
+                            const int h = s0 & 8;
+                            double kv[8], sv[8];
+#pragma unroll
+                            for (int k = 0; k < 8; k++) {
+                                kv[k] = krow[s0 + k];
+                                sv[k] = tbb[(k >> 2) * NM * kWave + (k & 3) * 16 + (xm ^ (h + k))];
+                            }
+                            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int k = 0; k < 8; k++) a = fma(kv[k], sv[k], a);
+                        }
+                        racc[u] = a;
+                    }
                 }
             }
             DEAC_WAIT_VM(0);
@@ -511,17 +667,20 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
     extern __shared__ double lds[];
     __shared__ BatchShared sh;
     const BatchDesc &d = descs[blockIdx.x];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     const int P = d.P;
+    const gcdouble *obs = (const gcdouble *)d.obs, *sig = (const gcdouble *)d.sig;
 
-    for (int k = threadIdx.x; k < 256 * 4; k += kBatchThreads) sh.rows[k] = d.walk_rows[k];
+    for (int k = threadIdx.x; k < 256 * 4; k += kBatchThreads) sh.rows[k] = ((const gcu64 *)d.walk_rows)[k];
     for (int k = threadIdx.x; k < 2 * kCD; k += kBatchThreads) lds[kOffTrial + k] = 0.0;  // padded members stay 0
     if (threadIdx.x < kBatchMaxP) {
         const int m = threadIdx.x;
-        sh.fitness[m] = m < P ? d.fitness[m] : 0.0;
+        const gcdouble *fit = (const gcdouble *)d.fitness, *cr = (const gcdouble *)d.cr_old,
+                       *fw = (const gcdouble *)d.f_old;
+        sh.fitness[m] = m < P ? fit[m] : 0.0;
         for (int sg = 0; sg < 2; sg++) {
-            sh.cr_old[m][sg] = m < P ? d.cr_old[m * 2 + sg] : 0.0;
-            sh.f_old[m][sg] = m < P ? d.f_old[m * 2 + sg] : 0.0;
+            sh.cr_old[m][sg] = m < P ? cr[m * 2 + sg] : 0.0;
+            sh.f_old[m][sg] = m < P ? fw[m * 2 + sg] : 0.0;
         }
         for (int k = 0; k < 3; k++) sh.accepted[k][m] = 0;
     }
@@ -530,35 +689,60 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
     // Prologue: the walk of the first generation whose masks are drawn.
     const long g0 = d.gen0;
     if (wave == kWalkWave) {
-        uint64_t ws[4] = {d.rng[0], d.rng[1], d.rng[2], d.rng[3]};
+        const gcu64 *rng = (const gcu64 *)d.rng;
+        uint64_t ws[4] = {rng[0], rng[1], rng[2], rng[3]};
         for (int item = 0; item < 2 * P; item++) walk_item(d, sh, ws, g0, item, lane);
         if (lane == 0)
             for (int w = 0; w < 4; w++) sh.walk_state[w] = ws[w];
     }
     __syncthreads();
 
-    if (d.init) {
-        // Initial fitness (deac.cpp:857-976) while the masks of generation 0
-        // are drawn and generation 1 is walked.
-        run_pass<NT, NM>(d, sh, lds, kPassInit, -1, g0, g0 + 1);
-        if (wave == kWorker0 && lane < P) {
-            double chi = 0.0;
-            const int tlast = (d.T & 1) ? d.T - 1 : -1;
-            for (int t = 0; t < d.T; t++) {
-                const double dv = (d.obs[t] - lds[t * Ppad + lane]) / d.sig[t];
-                chi = t == tlast ? fma(dv, dv, chi) : chi + dv * dv;
-            }
-            sh.fitness[lane] = chi / static_cast<double>(d.T);
-        }
-        __syncthreads();
-    } else {
-        run_pass<NT, NM>(d, sh, lds, kPassPre, g0 - 1, g0, g0 + 1);
-    }
-
-    long g = g0;
+    // Pass sequence: [initial evaluation | masks of gen0], then one pass per
+    // generation, then the write-back of the last generation's acceptances.
+    int kind = d.init ? kPassInit : kPassPre;
+    long g = g0;      // generation at the top of the loop
     long last = -1;   // last generation evolved in this launch
-    for (;; g++) {
-        // Top of the generation: best fitness, statistics, stop test
+    const int tlast = (d.T & 1) ? d.T - 1 : -1;  // fused element of the chi^2 sum
+    for (;;) {
+        const long pg = kind == kPassEvolve ? g : kind == kPassApply ? last : g0 - 1;
+        const long mgen = kind == kPassEvolve ? g + 1 : kind == kPassApply ? -1 : g0;
+        const long wgen = kind == kPassEvolve ? g + 2 : kind == kPassApply ? -1 : g0 + 1;
+        run_pass<NT, NM>(d, sh, lds, kind, pg, mgen, wgen);
+        if (kind == kPassApply) break;
+
+        if (kind == kPassInit || kind == kPassEvolve) {
+            // Fitness (deac.cpp:1539-1555; initial fitness deac.cpp:957-976)
+            // and greedy selection (deac.cpp:1556-1572).
+            if (wave == kWorker0 && lane < Ppad) {
+                int acc = 0;
+                if (lane < P) {
+                    double chi = 0.0;
+                    for (int t = 0; t < d.T; t++) {
+                        const double dv = (obs[t] - lds[t * Ppad + lane]) / sig[t];
+                        chi = t == tlast ? fma(dv, dv, chi) : chi + dv * dv;
+                    }
+                    const double fit = chi / static_cast<double>(d.T);
+                    if (kind == kPassInit) {
+                        sh.fitness[lane] = fit;
+                    } else if (fit <= sh.fitness[lane]) {
+                        acc = 1;
+                        sh.fitness[lane] = fit;
+                        for (int sg = 0; sg < 2; sg++) {
+                            sh.cr_old[lane][sg] = sh.cr_trial[g & 1][lane][sg];
+                            sh.f_old[lane][sg] = sh.f_trial[g & 1][lane][sg];
+                        }
+                    }
+                }
+                if (kind == kPassEvolve) sh.accepted[g % 3][lane] = acc;
+            }
+            if (kind == kPassEvolve) {
+                last = g;
+                g++;
+            }
+            __syncthreads();
+        }
+
+        // Top of generation g: best fitness, statistics, stop test
         // (deac.cpp:1145-1166); this generation's CR and F (deac.cpp:1180-1213).
         if (wave == kWorker0) {
             if (lane == 0) {
@@ -570,16 +754,17 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
                     for (int i = 0; i < P; i++)
                         if (sh.fitness[i] < mn) mn = sh.fitness[i];
                     if (d.stats) {
-                        double s = 0.0, q = 0.0;
+                        gdouble *st = (gdouble *)d.stats;
+                        double sum = 0.0, sq = 0.0;
                         const int p4 = P & ~3;
                         for (int i = 0; i < P; i++) {
                             const double v = sh.fitness[i];
-                            s += v;
-                            q = i < p4 ? q + v * v : fma(v, v, q);
+                            sum += v;
+                            sq = i < p4 ? sq + v * v : fma(v, v, sq);
                         }
-                        d.stats[g] = s / static_cast<double>(P);
-                        d.stats[d.stats_len + g] = mn;
-                        d.stats[2 * d.stats_len + g] = q / static_cast<double>(P);
+                        st[g] = sum / static_cast<double>(P);
+                        st[d.stats_len + g] = mn;
+                        st[2 * d.stats_len + g] = sq / static_cast<double>(P);
                     }
                     if (mn <= d.stop) flag = 1;
                 }
@@ -594,50 +779,29 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
             }
         }
         __syncthreads();
-        if (sh.flag) break;
-
-        run_pass<NT, NM>(d, sh, lds, kPassEvolve, g, g + 1, g + 2);
-
-        // Fitness and greedy selection (deac.cpp:1539-1572).
-        if (wave == kWorker0 && lane < Ppad) {
-            int acc = 0;
-            if (lane < P) {
-                double chi = 0.0;
-                const int tlast = (d.T & 1) ? d.T - 1 : -1;
-                for (int t = 0; t < d.T; t++) {
-                    const double dv = (d.obs[t] - lds[t * Ppad + lane]) / d.sig[t];
-                    chi = t == tlast ? fma(dv, dv, chi) : chi + dv * dv;
-                }
-                const double fit = chi / static_cast<double>(d.T);
-                if (fit <= sh.fitness[lane]) {
-                    acc = 1;
-                    sh.fitness[lane] = fit;
-                    for (int sg = 0; sg < 2; sg++) {
-                        sh.cr_old[lane][sg] = sh.cr_trial[g & 1][lane][sg];
-                        sh.f_old[lane][sg] = sh.f_trial[g & 1][lane][sg];
-                    }
-                }
-            }
-            sh.accepted[g % 3][lane] = acc;
+        if (sh.flag) {
+            if (last < 0) break;
+            kind = kPassApply;  // write back the genomes accepted in the last generation
+        } else {
+            kind = kPassEvolve;
         }
-        __syncthreads();
-        last = g;
     }
-    // Write back the genomes accepted in the last generation evolved.
-    if (last >= 0) run_pass<NT, NM>(d, sh, lds, kPassApply, last, -1, -1);
 
     if (threadIdx.x < P) {
         const int m = threadIdx.x;
-        d.fitness[m] = sh.fitness[m];
+        gdouble *fit = (gdouble *)d.fitness, *cr = (gdouble *)d.cr_old, *fw = (gdouble *)d.f_old;
+        fit[m] = sh.fitness[m];
         for (int sg = 0; sg < 2; sg++) {
-            d.cr_old[m * 2 + sg] = sh.cr_old[m][sg];
-            d.f_old[m * 2 + sg] = sh.f_old[m][sg];
+            cr[m * 2 + sg] = sh.cr_old[m][sg];
+            fw[m * 2 + sg] = sh.f_old[m][sg];
         }
     }
     if (threadIdx.x == 0) {
-        for (int w = 0; w < 4; w++) d.rng[w] = sh.gen_start[g & 3][w];
-        d.status[0] = sh.flag == 1 ? 1 : 0;
-        d.status[1] = static_cast<int>(g);
+        __attribute__((address_space(1))) uint64_t *rng = (__attribute__((address_space(1))) uint64_t *)d.rng;
+        __attribute__((address_space(1))) int *status = (__attribute__((address_space(1))) int *)d.status;
+        for (int w = 0; w < 4; w++) rng[w] = sh.gen_start[g & 3][w];
+        status[0] = sh.flag == 1 ? 1 : 0;
+        status[1] = static_cast<int>(g);
     }
 }
 

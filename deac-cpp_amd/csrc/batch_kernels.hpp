@@ -10,9 +10,9 @@
 
 namespace deac {
 
-constexpr int kBatchWaves = 14;             // 4 chain, 8 worker, 1 walk, 1 remainder
+constexpr int kBatchWaves = 16;             // 4 chain, 8 worker, 1 walk, 1 remainder, 2 mask
 constexpr int kBatchThreads = kBatchWaves * 64;
-constexpr int kBatchWorkerLanes = 512;      // 8 worker wavefronts
+constexpr int kBatchMaskLanes = 128;        // 2 mask wavefronts
 constexpr int kBatchChunkDoubles = 2048;    // members (padded) x genome positions per chunk
 constexpr int kBatchMaxP = 32;
 constexpr int kBatchRegionDoubles = 7 * kBatchChunkDoubles;  // 3 load + 2 update + 2 trial buffers
