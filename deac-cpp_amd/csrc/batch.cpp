@@ -254,20 +254,8 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
             return v;
         }
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-        (void)hipGetLastError();
-        return DEAC_ERR_NO_DEVICE;
-    }
+    // Distinct problems, their models and kernel layouts (host only).
     std::unique_ptr<deac_batch> h(new deac_batch());
-    BATCH_HIP(hipGetDevice(&h->device));
-    hipDeviceProp_t prop;
-    BATCH_HIP(hipGetDeviceProperties(&prop, h->device));
-    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DEAC_ERR_NO_DEVICE;
-    BATCH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    for (hipEvent_t *e : {&h->ev0, &h->ev1, &h->kev0, &h->kev1}) BATCH_HIP(hipEventCreate(e));
-
-    // Distinct problems.
     std::map<std::string, int> seen;
     h->model_of.resize(count);
     for (size_t i = 0; i < count; i++) {
@@ -286,11 +274,25 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
             if (message) *message = g_batch_message.c_str();
             return st;
         }
-        const deac_status su = upload_model(h.get(), *b);
-        if (su != DEAC_OK) return su;
         seen.emplace(key, static_cast<int>(h->models.size()));
         h->model_of[i] = static_cast<int>(h->models.size());
         h->models.push_back(std::move(b));
+    }
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return DEAC_ERR_NO_DEVICE;
+    }
+    BATCH_HIP(hipGetDevice(&h->device));
+    hipDeviceProp_t prop;
+    BATCH_HIP(hipGetDeviceProperties(&prop, h->device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DEAC_ERR_NO_DEVICE;
+    BATCH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&h->ev0, &h->ev1, &h->kev0, &h->kev1}) BATCH_HIP(hipEventCreate(e));
+    for (auto &b : h->models) {
+        const deac_status su = upload_model(h.get(), *b);
+        if (su != DEAC_OK) return su;
     }
 
     // Per-instance state pools.

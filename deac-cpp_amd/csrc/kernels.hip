@@ -10,7 +10,7 @@
 //       mask_kernel  - one wavefront per member regenerates the mask draws:
 //                      lane l jumps to draw l*chunk with a nibble table and
 //                      steps chunk draws;
-//   * evolve_kernel - a cooperative persistent kernel that runs a batch of
+//   * evolve_kernel - a persistent kernel (all workgroups co-resident) that runs a batch of
 //     generations: best-fitness/stats/stop test, trial vectors (DE/rand/1,
 //     binomial crossover, |.|, zero-frequency match), the model ISF
 //     E = K.S as exact FP64 FMA chains, chi^2 in the reference's summation
@@ -30,7 +30,7 @@ constexpr int kWave = 64;
 constexpr int kEvoThreads = 256;
 constexpr int kEvoWaves = kEvoThreads / kWave;
 
-// Grid-wide barrier of the cooperative evolution kernel.  The wait is
+// Grid-wide barrier of the persistent evolution kernel.  The wait is
 // bounded: after kBarrierSpins polls the workgroup records a failure and every
 // caller leaves the kernel, so a broken launch cannot hang the device.
 constexpr unsigned kBarrierSpins = 1u << 24;
@@ -618,11 +618,17 @@ hipError_t launch_mask(const MaskArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
+// A regular launch: the device holds one cooperative queue, which one
+// process keeps once it has used it, so cooperative launches from a second
+// process fail.  The grid never exceeds the co-resident limit that
+// evolve_max_workgroups reports (one workgroup per CU, bounded by LDS), so on
+// an otherwise idle device every workgroup is resident; if one is not, the
+// bounded grid barrier reports a timeout instead of hanging.
 hipError_t launch_evolve(const EvoArgs &a, int nwg, hipStream_t s)
 {
     void *args[] = {const_cast<EvoArgs *>(&a)};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(evolve_kernel), dim3(nwg), dim3(kEvoThreads),
-                                      args, kChainLds * sizeof(double), s);
+    return hipLaunchKernel(reinterpret_cast<const void *>(evolve_kernel), dim3(nwg), dim3(kEvoThreads), args,
+                           kChainLds * sizeof(double), s);
 }
 
 hipError_t launch_zero_u32(unsigned *p, size_t n, hipStream_t s)

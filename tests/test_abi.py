@@ -111,3 +111,32 @@ def test_zero_temperature_skips_temperature_check_and_halves_output():
 def test_status_strings():
     for code, text in da.STATUS.items():
         assert da.lib().deac_status_string(code).decode() == text
+
+
+def _batch_problem(**kw):
+    tau = np.linspace(0.0, 0.37, 17)
+    isf = np.exp(-tau)
+    return da.make_problem(tau, isf, 1e-3 * isf, da.frequency_grid(64, 8.0), 1.0, **kw)
+
+
+@pytest.mark.parametrize("kw,status,text", [
+    (dict(population_size=40), 5, "at most 32 members"),
+    (dict(first_moment=1.0), 5, "moment constraints"),
+    (dict(normalize=True, renormalize=True), 5, "renormalize each generation"),
+    (dict(population_size=2), 1, "population_size"),
+])
+def test_batch_create_rejects_before_touching_a_device(kw, status, text):
+    # deac_batch_create validates and plans every problem on the host first
+    with pytest.raises(da.DeacError) as e:
+        da.Batch([_batch_problem(), _batch_problem(**kw)])
+    assert e.value.status == status
+    assert "problem 1" in str(e.value) and text in str(e.value)
+
+
+def test_batch_create_without_device_reports_no_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(da.DeacError) as e:
+        da.Batch([_batch_problem()] * 3)
+    assert e.value.status == 2

@@ -1,0 +1,142 @@
+"""Batched engine (deac_batch_*) on the GPU against the reference and the oracle.
+
+Bar: bit-exact per instance.  Every instance of a batch must reproduce what
+the reference CPU build (the golden files) or the oracle computes for the
+same problem and seed: per-generation statistics, DSF, frequency grid,
+generation count and the final RNG state.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import deac_amd as da
+import golden_cases as gc
+import oracle_binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def batch_compatible(case):
+    m = case["meta"]
+    a = m["args"]
+    return (a["P"] <= 32 and a.get("first_moment", -1.0) < 0 and a.get("third_moment", -1.0) < 0
+            and m["variant"] != "standard_sp")
+
+
+def _engine_problem(case):
+    a = case["meta"]["args"]
+    freq = gc.frequency_of(case, da.frequency_grid)
+    return da.make_problem(case["tau"], case["isf"], case["err"], freq, a["T"], **gc.problem_kwargs(case))
+
+
+def _compare(got, ref, keys=("minimum", "mean", "squared_mean", "dsf", "frequency")):
+    errors = []
+    for k in keys:
+        d = gc.bit_diff(got[k], ref[k])
+        if d:
+            errors.append(f"{k}: {d}")
+    return errors
+
+
+def test_batch_of_reference_goldens():
+    # every golden case the batch kernel runs, all in one heterogeneous batch
+    # (different T, G, P, spectra, models, N and stop thresholds)
+    names = [n for n in gc.names() if batch_compatible(gc.load(n))]
+    print(f"[gpu] batch of {len(names)} goldens: {names}", flush=True)
+    cases = [gc.load(n) for n in names]
+    with da.Batch([_engine_problem(c) for c in cases]) as b:
+        outs = b.run([c["meta"]["args"]["seed"] for c in cases], [c["meta"]["args"]["N"] for c in cases],
+                     [c["meta"]["args"]["stop"] for c in cases], True)
+    bad = {}
+    for n, c, o in zip(names, cases, outs):
+        errors = gc.compare_outputs(o, c)
+        if errors:
+            bad[n] = errors
+    assert not bad, bad
+
+
+SYN = {
+    "zeroT_T17_G37_P5": dict(T=17, G=37, wmax=8.0, temp=0.0, seeds=[9, 10],
+                             kw=dict(model="normalization", spectra_type="ffull", zero_temperature=True,
+                                     population_size=5, normalize=True)),
+    "hyper_negw_T30_G100_P12": dict(T=30, G=100, wmax=10.0, temp=1.0, seeds=[4],
+                                    kw=dict(model="hyperbolic", allow_negative=True, population_size=12)),
+    "T20_G300_P32": dict(T=20, G=300, wmax=20.0, temp=1.0, seeds=[11],
+                         kw=dict(population_size=32, normalize=True)),
+    "T47_G129_P17": dict(T=47, G=129, wmax=30.0, temp=0.8, seeds=[3],
+                         kw=dict(population_size=17, spectra_type="bfull")),
+}
+
+
+def _synthetic_case(spec):
+    T = spec["T"]
+    beta = 1.0 / spec["temp"] if spec["temp"] > 0 else 1.0
+    tau = np.linspace(0, beta / 2, T)
+    isf = np.exp(-1.5 * tau) + np.exp(-0.3 * tau)
+    return tau, isf, 1e-3 * isf
+
+
+@pytest.mark.parametrize("name", sorted(SYN))
+@pytest.mark.parametrize("slice_len", [None, "7"])
+def test_batch_matches_oracle(name, slice_len, monkeypatch):
+    # shapes the goldens do not cover: zero temperature, odd T (16k+r rows
+    # split between the matrix cores and the remainder chains), P not a
+    # multiple of 16; and launches cut every 7 generations
+    spec = SYN[name]
+    print(f"[gpu] batch vs oracle {name} slice={slice_len}", flush=True)
+    if slice_len:
+        monkeypatch.setenv("DEAC_BATCH_SLICE", slice_len)
+    tau, isf, err = _synthetic_case(spec)
+    freq = ob.frequency_grid(spec["G"], spec["wmax"])
+    N = 250
+    with da.Batch([da.make_problem(tau, isf, err, freq, spec["temp"], **spec["kw"])] * len(spec["seeds"])) as b:
+        outs = b.run(spec["seeds"], N, 0.0, True)
+    for sd, o in zip(spec["seeds"], outs):
+        r = ob.run(ob.make_problem(tau, isf, err, freq, spec["temp"], number_of_generations=N,
+                                   stop_minimum_fitness=0.0, **spec["kw"]), sd)
+        assert not _compare(o, r), (sd, _compare(o, r))
+        assert o["rng"] == r["rng"] and o["generation"] == r["generation"]
+
+
+def test_batch_matches_plain_engine_and_continues_rng():
+    # the batch and deac_run agree, and each instance's returned RNG state
+    # continues the reference stream (the state at the first generation not
+    # evolved)
+    print("[gpu] batch vs plain engine", flush=True)
+    case = gc.load("smoke_stop")
+    p = _engine_problem(case)
+    a = case["meta"]["args"]
+    with da.Engine(p) as e:
+        plain = [e.run(sd, a["N"], a["stop"]) for sd in (a["seed"], 5, 6)]
+    with da.Batch([p] * 3) as b:
+        outs = b.run([a["seed"], 5, 6], a["N"], a["stop"], True)
+    for pl, o in zip(plain, outs):
+        assert not _compare(o, pl)
+        assert o["rng"] == pl["rng"] and o["generation"] == pl["generation"]
+
+
+def test_seed_batch_full_size():
+    # 256 seeds of the G=4096, T=65, P=32 workload in one launch; seeds 1 and
+    # 256 checked bit for bit against the oracle, the rest by properties
+    print("[gpu] 256-seed batch", flush=True)
+    t, f, s = np.fromfile(os.path.join(gc.GOLDEN_DIR, "isf_c2_T65.bin")).reshape(3, -1)
+    freq = da.frequency_grid(4096, 512.0)
+    N = 25
+    prob = da.make_problem(t, f, s, freq, 1.35, population_size=32, normalize=True, spectra_type="bfull")
+    seeds = list(range(1, 257))
+    with da.Batch([prob] * 256) as b:
+        outs = b.run(seeds, N, 1.0, True)
+        assert b.profile["generations"] == 256 * (N - 1)
+    for sd in (1, 256):
+        r = ob.run(ob.make_problem(t, f, s, freq, 1.35, number_of_generations=N, population_size=32,
+                                   normalize=True, stop_minimum_fitness=1.0), sd)
+        assert not _compare(outs[sd - 1], r), _compare(outs[sd - 1], r)
+        assert outs[sd - 1]["rng"] == r["rng"]
+    finals = np.array([o["minimum_fitness"] for o in outs])
+    assert len(set(finals.tolist())) > 250
+    for o in outs:
+        assert np.all(np.diff(o["minimum"]) <= 0)
+        assert np.all(o["dsf"] >= 0)
+    avg = np.mean([o["dsf"] for o in outs], axis=0)
+    assert np.all(np.isfinite(avg)) and avg.shape == (2 * 4096 - 1,)
