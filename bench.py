@@ -77,6 +77,18 @@ def cpu_baseline(gens):
                       f"wall time incl. setup {dt:.2f} s"}
 
 
+def traffic_per_generation():
+    """HBM bytes per generation of the seed batch, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated against a plain copy)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "batch_kernel_seed256_hbm_counters.md")
+    try:
+        m = re.search(r"traffic_bytes_per_generation:\s*([0-9.]+)", open(path).read())
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,11 +97,25 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=5000, help="timed generations of the plain path")
     ap.add_argument("--cpu-gens", type=int, default=2000, help="generations of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: instances are independent and nothing here spans devices "
               "(see DESIGN.md for how the work shards); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_bytes:
+        # Reference kernel for the HBM byte counters: reads and writes exactly
+        # copy_bytes (three launches).
+        import torch
+        x = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes, "launches": 3}), flush=True)
+        return
 
     import deac_amd as da
     t, f, s = load_isf()
@@ -113,6 +139,10 @@ def main():
     kern_s = prof["evolve_ms"] / 1e3
     achieved = evals * flop_per_eval / kern_s / 1e12
     stopped = sum(1 for o in outs if o["generation"] < args.steps)
+    per_gen = traffic_per_generation()
+    gens_per_launch = min(args.steps, 4096)
+    traffic = per_gen * gens_per_launch if per_gen else None
+    alg_launch = nb * Pb * 2 * G * 8 * gens_per_launch
 
     # ---- plain FP64 single instance
     pp = da.make_problem(t, f, s, freq, TEMPERATURE, population_size=PLAIN["P"], normalize=True, spectra_type="bfull")
@@ -148,7 +178,10 @@ def main():
                                   "ms_per_generation": pprof["loop_ms"] / pprof["generations"]},
                    "batched_over_plain": value / plain_value},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "traffic_note": "HBM bytes per launch (one launch = the timed generations, up to the "
+                                     "4096-generation slice) from profiles/batch_kernel_seed256_hbm_counters.md; "
+                                     "algorithmic bytes per launch: " + str(alg_launch),
                      "kernel": "batch_kernel<1,2>", "flop_per_genome_eval": flop_per_eval,
                      "kernel_ms_per_generation": prof["evolve_ms"] / args.steps,
                      "hbm_algorithmic_GBps": evals * 2 * G * 8 / kern_s / 1e9},
