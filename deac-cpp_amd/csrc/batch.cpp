@@ -16,6 +16,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "batch_kernels.hpp"
@@ -144,10 +145,9 @@ deac_status plan_model(BatchModel &b, std::string &why)
     b.Lw = b.Lpad / 32;
     b.ntt = T / 16;
     b.nrem = T % 16;
-    // Rows past the last full tile run as vector-ALU chains when they fit the
-    // remainder wave (2 chains per lane, one LDS ring buffer per chunk);
-    // otherwise they become one more MFMA tile padded with zero rows.
-    if (b.nrem * b.Ppad > 64 * deac::kBatchRemChains || b.nrem * b.C > 1024) {
+    // Up to four rows past the last full tile run on 4x4x4 MFMAs next to the
+    // 16x16x4 tiles; more become one more 16-row tile padded with zero rows.
+    if (b.nrem > deac::kBatchMaxRem) {
         b.ntt += 1;
         b.nrem = 0;
     }
@@ -177,10 +177,17 @@ deac_status upload_model(deac_batch *h, BatchModel &b)
                     if (tau < T && slot < L) v = m.kernel[static_cast<size_t>(tau) * L + slot];
                     kt[((static_cast<size_t>(t) * nkb2 + kb2) * 64 + l) * 2 + e] = v;
                 }
-    std::vector<double> kr(static_cast<size_t>(std::max(b.nrem, 1)) * b.Lpad, -0.0);
-    for (int r = 0; r < b.nrem; r++)
-        std::memcpy(&kr[static_cast<size_t>(r) * b.Lpad], &m.kernel[static_cast<size_t>(b.ntt * 16 + r) * L],
-                    L * sizeof(double));
+    // Rows ntt*16 .. T-1 in v_mfma_f64_4x4x4_4b A order: lane l holds row
+    // l&3 at position 4*kb + (l>>4) (the same for the four blocks).
+    std::vector<double> kr(static_cast<size_t>(nkb2) * 64 * 2, -0.0);
+    for (int kb2 = 0; kb2 < nkb2; kb2++)
+        for (int l = 0; l < 64; l++)
+            for (int e = 0; e < 2; e++) {
+                const int r = l & 3, slot = (2 * kb2 + e) * 4 + (l >> 4);
+                double v = -0.0;
+                if (slot < L) v = r < b.nrem ? m.kernel[static_cast<size_t>(b.ntt * 16 + r) * L + slot] : 0.0;
+                kr[(static_cast<size_t>(kb2) * 64 + l) * 2 + e] = v;
+            }
     // Jump tables: T^L by rows for the walk, T^(k*Lsub) by nibbles for the
     // mask sub-streams k = 1..S-1.
     std::vector<uint64_t> rows(256 * 4), tabs(static_cast<size_t>(std::max(b.S - 1, 1)) * 64 * 16 * 4);
@@ -401,15 +408,15 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
     bool first = true;
     for (;;) {
         // Group the active instances by kernel instantiation.
-        std::map<std::pair<int, int>, std::vector<size_t>> groups;
+        std::map<std::tuple<int, int, int>, std::vector<size_t>> groups;
         for (size_t i = 0; i < n; i++)
             if (active[i]) {
                 const BatchModel &b = *h->models[h->model_of[i]];
-                groups[{b.NT, b.NM}].push_back(i);
+                groups[{b.NT, b.NM, b.nrem > 0 ? 1 : 0}].push_back(i);
             }
         if (groups.empty()) break;
         size_t slot = 0;
-        std::vector<std::pair<std::pair<int, int>, std::pair<size_t, size_t>>> launches;
+        std::vector<std::pair<std::tuple<int, int, int>, std::pair<size_t, size_t>>> launches;
         std::vector<size_t> order;
         for (auto &[key, members] : groups) {
             const size_t begin = slot;
@@ -460,8 +467,8 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
         BATCH_HIP(hipMemcpyAsync(h->descs, descs.data(), slot * sizeof(deac::BatchDesc), hipMemcpyHostToDevice, s));
         BATCH_HIP(hipEventRecord(h->kev0, s));
         for (auto &[key, range] : launches)
-            BATCH_HIP(deac::launch_batch(h->descs + range.first, static_cast<int>(range.second), key.first, key.second,
-                                         s));
+            BATCH_HIP(deac::launch_batch(h->descs + range.first, static_cast<int>(range.second), std::get<0>(key),
+                                         std::get<1>(key), std::get<2>(key), s));
         BATCH_HIP(hipEventRecord(h->kev1, s));
         std::vector<int> st(2 * n);
         BATCH_HIP(hipMemcpyAsync(st.data(), h->status, st.size() * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -473,6 +480,10 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
         for (size_t k = 0; k < slot; k++) {
             const size_t i = order[k];
             const long reached = st[2 * i + 1];
+            // the kernel ends an instance inside its slice: anything else is
+            // a device-side failure, not a result
+            if (reached < descs[k].gen0 || reached > descs[k].gen_end || (st[2 * i] != 0 && st[2 * i] != 1))
+                return DEAC_ERR_DEVICE;
             prof.generations += static_cast<size_t>(reached - next_gen[i]);
             next_gen[i] = reached;
             if (st[2 * i]) {

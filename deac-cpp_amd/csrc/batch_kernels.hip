@@ -10,19 +10,20 @@
 //   0-3   chain waves: model ISF E = K.S on the FP64 matrix cores.  Each
 //         v_mfma_f64_16x16x4f64 continues 256 sequential fused chains by four
 //         genome positions in ascending order, which is exactly the
-//         reference's FMA chain (matrix_multiply_LxM_by_MxN, deac.cpp:110-118).
+//         reference's FMA chain (matrix_multiply_LxM_by_MxN, deac.cpp:110-118);
+//         up to four rows past the last full 16-row tile run the same way on
+//         v_mfma_f64_4x4x4_4b_f64, whose B operand is the 16x16x4 B register.
 //   4-11  workers: stage the population chunk by chunk into LDS, rebuild the
 //         genomes accepted in the previous generation and form this
 //         generation's trial vectors (deac.cpp:1261-1311).
 //   12    walk wave: the sequential part of the random stream (jDE redraws,
 //         donor indices, mask-state skips) two generations ahead.
-//   13    remainder wave: kernel rows past the last full 16-row tile as plain
-//         fused chains on the vector ALU.
-//   14-15 mask waves: the next generation's crossover draws from per-lane
+//   13-15 mask waves: the next generation's crossover draws from per-lane
 //         sub-streams of each member's mask stream.
 // The genome is processed in chunks of C positions (C * padded members = 2048
 // doubles); the stages of one chunk run in consecutive steps separated by
-// workgroup barriers: load(c), rebuild(c-1), trial(c-2), chains(c-3).
+// workgroup barriers: load(c), rebuild(c-2), trial(c-3), chains(c-4); a
+// chunk's DMA has a full step to land before it is read.
 //
 // Population storage: [P][Lpad] member-major, one buffer.  A generation's
 // accepted trials are not written when they are selected; the next
@@ -40,13 +41,14 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kWorker0 = 4;
+constexpr int kWorkers = 8;            // 4-11: workers
+constexpr int kTasks = 32 / kWorkers;  // 64-position tasks per worker and step
 constexpr int kWalkWave = 12;
-constexpr int kRngWave0 = 14;          // 14, 15: mask waves
+constexpr int kRngWave0 = 13;          // 13-15: mask waves
 constexpr int kCD = kBatchChunkDoubles;
-constexpr int kOffPop = 0;             // 3 buffers: genomes as stored, [Ppad][C]
-constexpr int kOffNew = 3 * kCD;       // 2 buffers: rebuilt (accepted) genomes, [Ppad][C]
-constexpr int kOffTrial = 5 * kCD;     // 2 buffers: trial genomes in MFMA B order
-constexpr int kRemRing = 1024;         // doubles per remainder-row kernel buffer (2 buffers)
+constexpr int kOffPop = 0;             // 4 buffers: genomes as stored, [Ppad][C]
+constexpr int kOffNew = 4 * kCD;       // 2 buffers: rebuilt (accepted) genomes, [Ppad][C]
+constexpr int kOffTrial = 6 * kCD;     // 2 buffers: trial genomes in MFMA B order
 constexpr int kMaskWords = 256;        // mask words per chunk: 2 generations x Ppad x 2 candidates x C/32
 
 enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3 };
@@ -82,9 +84,9 @@ struct BatchShared {
     double y0t[kBatchMaxP], y0r[kBatchMaxP];  // trial value at position 0 (zero-frequency match)
     int accepted[3][kBatchMaxP];
     int flag;                                 // 0 evolve, 1 stop test fired, 2 end of the launch
-    // Crossover words of the chunks in flight, [chunk % 3][generation: this,
+    // Crossover words of the chunks in flight, [chunk % 4][generation: this,
     // previous][member][candidate][C/32], staged from global memory.
-    uint32_t mring[3][kMaskWords];
+    uint32_t mring[4][kMaskWords];
 };
 
 // LDS index of trial value (member m, chunk position sl) in the order the
@@ -217,7 +219,7 @@ struct MaskGen {
     uint64_t s[4];
     uint64_t ta, tb;      // thresholds of the two candidates at the current position
     uint64_t na, nb;      // thresholds after the sign change (negative frequencies)
-    uint32_t wa, wb, bit;
+    uint32_t wa, wb;
     int pos, end, flip;   // flip: position of the sign change (or past the end)
     guint *out;
 };
@@ -230,7 +232,6 @@ __device__ void mask_setup(MaskGen &mg, const BatchDesc &d, const BatchShared &s
     const int m = wl / d.S, k = wl % d.S;
     mg.pos = mg.end = 0;
     mg.wa = mg.wb = 0;
-    mg.bit = 1;
     mg.flip = 0;
     mg.ta = mg.tb = mg.na = mg.nb = 0;
     mg.out = (guint *)d.masks;
@@ -252,7 +253,6 @@ __device__ void mask_setup(MaskGen &mg, const BatchDesc &d, const BatchShared &s
     for (int w = 0; w < 4; w++) mg.s[w] = s[w];
     mg.pos = min(k * d.Lsub, d.L);
     mg.end = min(mg.pos + d.Lsub, d.L);
-    mg.bit = 1u << (mg.pos & 31);
     mg.flip = d.nsign == 2 ? d.G : d.L;
     const bool neg = mg.pos >= mg.flip;
     mg.ta = neg ? thr[0][1] : thr[0][0];
@@ -262,24 +262,48 @@ __device__ void mask_setup(MaskGen &mg, const BatchDesc &d, const BatchShared &s
     mg.out = (guint *)d.masks + (static_cast<size_t>(mgen % 3) * Ppad + m) * 2 * d.Lw;
 }
 
+// Up to q draws (q a multiple of 8).  Bits enter each candidate word
+// most-significant first (w = 2w + bit); a finished word is bit-reversed so
+// that bit j belongs to draw j of the word.  Sub-streams start on word
+// boundaries, so 8-draw groups never straddle a word except at the stream's
+// end; the sign change of the thresholds is handled per draw only in the
+// group that contains it.
 __device__ __forceinline__ void mask_draws(MaskGen &mg, int q, int Lw)
 {
     const int n = min(q, mg.end - mg.pos);
-    for (int u = 0; u < n; u++) {
-        if (mg.pos == mg.flip) {
-            mg.ta = mg.na;
-            mg.tb = mg.nb;
+    for (int u = 0; u < n; u += 8) {
+        const int cnt = min(8, n - u);
+        if (cnt == 8 && (mg.pos >= mg.flip || mg.pos + 8 <= mg.flip)) {
+            if (mg.pos >= mg.flip) {
+                mg.ta = mg.na;
+                mg.tb = mg.nb;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint64_t r = xnext(mg.s) >> 11;
+                mg.wa = mg.wa * 2u + (r < mg.ta ? 1u : 0u);
+                mg.wb = mg.wb * 2u + (r < mg.tb ? 1u : 0u);
+            }
+            mg.pos += 8;
+        } else {
+            for (int k = 0; k < cnt; k++) {
+                if (mg.pos == mg.flip) {
+                    mg.ta = mg.na;
+                    mg.tb = mg.nb;
+                }
+                const uint64_t r = xnext(mg.s) >> 11;
+                mg.wa = mg.wa * 2u + (r < mg.ta ? 1u : 0u);
+                mg.wb = mg.wb * 2u + (r < mg.tb ? 1u : 0u);
+                mg.pos++;
+            }
         }
-        const uint64_t r = xnext(mg.s) >> 11;
-        if (r < mg.ta) mg.wa |= mg.bit;
-        if (r < mg.tb) mg.wb |= mg.bit;
-        if (mg.bit == 0x80000000u || mg.pos + 1 == mg.end) {
-            mg.out[mg.pos >> 5] = mg.wa;
-            mg.out[Lw + (mg.pos >> 5)] = mg.wb;
+        if ((mg.pos & 31) == 0 || mg.pos == mg.end) {
+            const int nb = ((mg.pos - 1) & 31) + 1;  // draws in this word
+            const int w = (mg.pos - 1) >> 5;
+            mg.out[w] = __brev(mg.wa) >> (32 - nb);
+            mg.out[Lw + w] = __brev(mg.wb) >> (32 - nb);
             mg.wa = mg.wb = 0;
         }
-        mg.bit = mg.bit == 0x80000000u ? 1u : mg.bit << 1;
-        mg.pos++;
     }
 }
 
@@ -288,16 +312,17 @@ __device__ __forceinline__ void mask_draws(MaskGen &mg, int q, int Lw)
 // genomes accepted in generation g), kPassPre (masks only).  mgen: the
 // generation whose masks the workers draw (-1: none); wgen: the generation
 // the walk wave walks (-1: none).
-template <int NT, int NM>
+template <int NT, int NM, int REM>
 __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, double *lds, int kind, long g, long mgen,
                                          long wgen)
 {
     constexpr int C = 128 / NM, Ppad = 16 * NM;
-    constexpr int KD = NT == 1 ? 8 : (NT == 2 ? 4 : 2);   // kernel-tile prefetch depth (pairs of blocks)
+    // kernel-tile prefetch depth in pairs of 4-position blocks (divides C/8)
+    constexpr int KD = NT == 1 ? (REM ? 4 : 8) : (NT == 2 ? 4 : 2);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     const int nchunk = d.nchunk;
     const bool chains = kind == kPassInit || kind == kPassEvolve;
-    const int nsteps = kind == kPassApply ? nchunk + 1 : nchunk + 3;
+    const int nsteps = kind == kPassApply ? nchunk + 2 : nchunk + 4;
 
     if (wave < kWorker0) {
         // ------------------------------------------------ chain waves
@@ -326,14 +351,35 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
 #pragma unroll
             for (int mt = 0; mt < NM; mt++) acc[i][mt] = d4{0.0, 0.0, 0.0, 0.0};
         double2 kr[KD][NT];
+        // Loads address a wave-uniform block base (scalar registers) plus the
+        // lane's 16-byte offset.
+        const gcdouble *ktw[NT];
+#pragma unroll
+        for (int i = 0; i < NT; i++) ktw[i] = kt + static_cast<size_t>(tload[i]) * nkb2 * (2 * kWave);
+        const int lo = 2 * lane;
 #pragma unroll
         for (int p = 0; p < KD; p++)
 #pragma unroll
-            for (int i = 0; i < NT; i++)
-                kr[p][i] = load2(kt + ((static_cast<size_t>(tload[i]) * nkb2 + min(p, nkb2 - 1)) * kWave + lane) * 2);
-        for (int c = 0; c < nsteps; c++) {
-            const int cm = c - 3;
-            if (chains && cm >= 0 && cm < nchunk) {
+            for (int i = 0; i < NT; i++) kr[p][i] = load2(ktw[i] + static_cast<size_t>(min(p, nkb2 - 1)) * (2 * kWave) + lo);
+        // Rows past the last full tile (REM): one 4x4x4 MFMA per 4 positions
+        // for member tile rmt = wave % NM; waves >= NM repeat a tile's work
+        // (discarded) to keep the code straight-line.
+        const gcdouble *krm = (const gcdouble *)d.krem;
+        const int rmt = wave % NM;
+        const uint64_t rsel = rmt ? ~UINT64_C(0) : UINT64_C(0);
+        double accr = 0.0;
+        double2 krr[KD];
+        if (REM) {
+#pragma unroll
+            for (int p = 0; p < KD; p++) krr[p] = load2(krm + static_cast<size_t>(min(p, nkb2 - 1)) * (2 * kWave) + lo);
+        }
+        // Steps 0-3 fill the pipeline, steps 4 .. nchunk+3 run the chains of
+        // chunk c-4; the hot loop has no branch, so the waits for the K
+        // prefetch are exact across its iterations.
+        const int nwork = chains ? nchunk : 0;
+        for (int c = 0; c < min(4, nsteps); c++) step_barrier();
+        for (int cm = 0; cm < nwork; cm++) {
+            {
                 const double *tb = lds + kOffTrial + (cm & 1) * kCD;
                 // B operands one block pair ahead of the MFMAs that use them
                 double bc[2][NM], bn[2][NM];
@@ -362,6 +408,16 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                             for (int mt = 0; mt < NM; mt++)
                                 acc[i][mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bc[e][mt], acc[i][mt], 0, 0, 0);
                         }
+                        if (REM) {
+                            // B register of member tile rmt by a bit select:
+                            // no branch (exact prefetch waits) and no indexed
+                            // register array (which would go through scratch)
+                            const double ar = e ? krr[sl].y : krr[sl].x;
+                            const uint64_t b0 = __builtin_bit_cast(uint64_t, bc[e][0]);
+                            const uint64_t b1 = __builtin_bit_cast(uint64_t, bc[e][NM - 1]);
+                            const double br = __builtin_bit_cast(double, (b1 & rsel) | (b0 & ~rsel));
+                            accr = __builtin_amdgcn_mfma_f64_4x4x4f64(ar, br, accr, 0, 0, 0);
+                        }
                     }
 #pragma unroll
                     for (int e = 0; e < 2; e++)
@@ -369,13 +425,14 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                         for (int mt = 0; mt < NM; mt++) bc[e][mt] = bn[e][mt];
                     const int gn = min(g2 + KD, nkb2 - 1);
 #pragma unroll
-                    for (int i = 0; i < NT; i++)
-                        kr[sl][i] = load2(kt + ((static_cast<size_t>(tload[i]) * nkb2 + gn) * kWave + lane) * 2);
+                    for (int i = 0; i < NT; i++) kr[sl][i] = load2(ktw[i] + static_cast<size_t>(gn) * (2 * kWave) + lo);
+                    if (REM) krr[sl] = load2(krm + static_cast<size_t>(gn) * (2 * kWave) + lo);
                     __builtin_amdgcn_sched_barrier(0);  // issue the refill here, a full step ahead of its use
                 }
             }
             step_barrier();
         }
+        for (int c = min(4, nsteps) + nwork; c < nsteps; c++) step_barrier();
         __syncthreads();
         // E[t][m] into LDS (the chunk buffers are free now): lane l holds
         // rows 4v + l/16 of the tile, column l%16.
@@ -393,10 +450,24 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                                 lds[t * Ppad + m] = acc[i][mt][v];
                             }
                     }
+            // 4x4x4 output: lane l holds row l>>4 of block (l>>2)&3, column
+            // l&3, i.e. member rmt*16 + (l & 15).
+            if (REM && wave < NM) {
+                const int m = rmt * 16 + (lane & 15), r = lane >> 4;
+                if (m < d.P && r < d.nrem) lds[(d.ntt * 16 + r) * Ppad + m] = accr;
+            }
         }
     } else if (wave < kWalkWave) {
         // ------------------------------------------------ workers
         const int wk = wave - kWorker0;
+        // Descriptor fields held in scalar registers for the whole pass
+        // (readfirstlane keeps them from being re-loaded from memory, which
+        // would make every task wait on the scalar cache).
+        const int nP = __builtin_amdgcn_readfirstlane(d.P), nG = __builtin_amdgcn_readfirstlane(d.G);
+        const int nL = __builtin_amdgcn_readfirstlane(d.L), nLpad = __builtin_amdgcn_readfirstlane(d.Lpad);
+        const int n2 = __builtin_amdgcn_readfirstlane(d.nsign) == 2 ? 1 : 0;
+        const int aneg = __builtin_amdgcn_readfirstlane(d.allow_neg);
+        const int nLw = __builtin_amdgcn_readfirstlane(d.Lw);
         constexpr int kSegs = C / 64;
         constexpr int kWpc = C / 32;  // mask words per member and chunk
         const bool rebuild = kind == kPassEvolve || kind == kPassApply;
@@ -407,155 +478,176 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         const int aslot = static_cast<int>((g + 2) % 3);      // generation g-1
         const guint *gmask = (const guint *)d.masks;
         gdouble *gpop = (gdouble *)d.pop;
+        // Per-pass task descriptors: donors r1 | r2<<5 | r3<<10, mask
+        // candidate <<15, operand from the rebuilt buffer (parent, r1, r2, r3)
+        // <<16..19, active <<20.  Lane j holds trial task j, lane kTasks+j
+        // rebuild task j; readlane makes them wave-uniform where used.
+        uint32_t dtab = 0;
+        if (lane < 2 * kTasks) {
+            const int j = lane % kTasks;
+            const int t = wk + kWorkers * j, m = t / kSegs;
+            // The walk record is read only when the task is active: the
+            // record of the generation before a launch's first one was never
+            // written in that launch.
+            if (lane < kTasks) {
+                if (kind == kPassEvolve && m < nP) {
+                    const WalkRec &r = sh.rec[g & 3][m];
+                    const uint32_t cand = sh.accepted[aslot][m] & 1;
+                    uint32_t td = (r.mutant[0] & 31) | ((r.mutant[1] & 31) << 5) | ((r.mutant[2] & 31) << 10) |
+                                  (cand << 15) | (cand << 16) | (1u << 20);
+                    for (int k = 0; k < 3; k++)
+                        td |= static_cast<uint32_t>(sh.accepted[aslot][r.mutant[k] & 31] & 1) << (17 + k);
+                    dtab = td;
+                }
+            } else if (rebuild && pg >= 0 && m < nP && sh.accepted[pslot][m]) {
+                const WalkRec &r = sh.rec[pg & 3][m];
+                dtab = (r.mutant[0] & 31) | ((r.mutant[1] & 31) << 5) | ((r.mutant[2] & 31) << 10) |
+                       (static_cast<uint32_t>(sh.accepted[pprev][m] & 1) << 15) | (1u << 20);
+            }
+        }
         for (int c = 0; c < nsteps; c++) {
+            // rebuild(c-2): genomes accepted in generation pg, then trial(c-3):
+            // this generation's trial vectors (or, for the initial evaluation,
+            // the stored genomes) in MFMA B order.  Each phase loads the
+            // operands, F and mask words of all its tasks at once (one LDS
+            // round trip), then computes; operands are addressed by 32-bit
+            // offsets into the dynamic LDS array.
+            const int c1 = c - 2;
+            if (rebuild && pg >= 0 && c1 >= 0 && c1 < nchunk) {
+                const int popo = kOffPop + (c1 % 4) * kCD;
+                const int newo = kOffNew + (c1 & 1) * kCD;
+                const uint32_t *mrb = sh.mring[c1 % 4] + Ppad * 2 * kWpc;
+                double v4[kTasks][4], Fv[kTasks];
+                uint32_t w4[kTasks];
+#pragma unroll
+                for (int j = 0; j < kTasks; j++) {
+                    const uint32_t ds = __builtin_amdgcn_readlane(dtab, kTasks + j);
+                    if (!((ds >> 20) & 1)) continue;  // most members were not accepted
+                    const int t = wk + kWorkers * j, m = min(t / kSegs, nP - 1), seg = t % kSegs;
+                    const int sl = seg * 64 + lane, s = c1 * C + sl;
+                    v4[j][0] = lds[popo + m * C + sl];
+                    v4[j][1] = lds[popo + static_cast<int>(ds & 31) * C + sl];
+                    v4[j][2] = lds[popo + static_cast<int>((ds >> 5) & 31) * C + sl];
+                    v4[j][3] = lds[popo + static_cast<int>((ds >> 10) & 31) * C + sl];
+                    w4[j] = mrb[(m * 2 + static_cast<int>((ds >> 15) & 1)) * kWpc + (sl >> 5)];
+                    Fv[j] = sh.f_trial[pg & 1][m][(n2 && s >= nG) ? 1 : 0];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < kTasks; j++) {
+                    const uint32_t ds = __builtin_amdgcn_readlane(dtab, kTasks + j);
+                    if (!((ds >> 20) & 1)) continue;
+                    const int t = wk + kWorkers * j, m = t / kSegs, seg = t % kSegs;
+                    const int sl = seg * 64 + lane, s = c1 * C + sl;
+                    double v = de_value((w4[j] >> (s & 31)) & 1, Fv[j], v4[j][0], v4[j][1], v4[j][2], v4[j][3],
+                                        aneg);
+                    if (n2 && s == nG) {  // deac.cpp:1305-1308
+                        if (c1 == 0) {
+                            const int wb = (m * 2 + static_cast<int>((ds >> 15) & 1)) * kWpc;
+                            v = de_value(mrb[wb] & 1, sh.f_trial[pg & 1][m][0], lds[popo + m * C],
+                                         lds[popo + static_cast<int>(ds & 31) * C],
+                                         lds[popo + static_cast<int>((ds >> 5) & 31) * C],
+                                         lds[popo + static_cast<int>((ds >> 10) & 31) * C], aneg);
+                        } else {
+                            v = sh.y0r[m];
+                        }
+                    }
+                    if (s == 0) sh.y0r[m] = v;
+                    if (s >= nL) v = 0.0;
+                    lds[newo + m * C + sl] = v;
+                    if (s < nL) gpop[static_cast<size_t>(m) * nLpad + s] = v;
+                }
+            }
+            const int c2 = c - 3;
+            if (chains && c2 >= 0 && c2 < nchunk) {
+                const int popo = kOffPop + (c2 % 4) * kCD;
+                const int newo = kOffNew + (c2 & 1) * kCD;
+                double *tb = lds + kOffTrial + (c2 & 1) * kCD;
+                const uint32_t *mrt = sh.mring[c2 % 4];
+                if (kind == kPassInit) {
+#pragma unroll
+                    for (int j = 0; j < kTasks; j++) {
+                        const int t = wk + kWorkers * j, m = t / kSegs, seg = t % kSegs;
+                        if (m >= nP) continue;
+                        const int sl = seg * 64 + lane, s = c2 * C + sl;
+                        tb[trial_index<NM>(m, sl)] = s < nL ? lds[popo + m * C + sl] : 0.0;
+                    }
+                } else {
+                    double v4[kTasks][4], Fv[kTasks];
+                    uint32_t w4[kTasks];
+#pragma unroll
+                    for (int j = 0; j < kTasks; j++) {
+                        const uint32_t ds = __builtin_amdgcn_readlane(dtab, j);
+                        const int t = wk + kWorkers * j, m = min(t / kSegs, nP - 1), seg = t % kSegs;
+                        const int sl = seg * 64 + lane, s = c2 * C + sl;
+                        v4[j][0] = lds[(((ds >> 16) & 1) ? newo : popo) + m * C + sl];
+#pragma unroll
+                        for (int k = 0; k < 3; k++)
+                            v4[j][k + 1] = lds[(((ds >> (17 + k)) & 1) ? newo : popo) +
+                                               static_cast<int>((ds >> (5 * k)) & 31) * C + sl];
+                        w4[j] = mrt[(m * 2 + static_cast<int>((ds >> 15) & 1)) * kWpc + (sl >> 5)];
+                        Fv[j] = sh.f_trial[g & 1][m][(n2 && s >= nG) ? 1 : 0];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int j = 0; j < kTasks; j++) {
+                        const uint32_t ds = __builtin_amdgcn_readlane(dtab, j);
+                        if (!((ds >> 20) & 1)) continue;
+                        const int t = wk + kWorkers * j, m = t / kSegs, seg = t % kSegs;
+                        const int sl = seg * 64 + lane, s = c2 * C + sl;
+                        double v = de_value((w4[j] >> (s & 31)) & 1, Fv[j], v4[j][0], v4[j][1], v4[j][2], v4[j][3],
+                                            aneg);
+                        if (n2 && s == nG) {
+                            if (c2 == 0) {
+                                const int wb = (m * 2 + static_cast<int>((ds >> 15) & 1)) * kWpc;
+                                double x[4];
+                                x[0] = lds[(((ds >> 16) & 1) ? newo : popo) + m * C];
+                                for (int k = 0; k < 3; k++)
+                                    x[k + 1] = lds[(((ds >> (17 + k)) & 1) ? newo : popo) +
+                                                   static_cast<int>((ds >> (5 * k)) & 31) * C];
+                                v = de_value(mrt[wb] & 1, sh.f_trial[g & 1][m][0], x[0], x[1], x[2], x[3], aneg);
+                            } else {
+                                v = sh.y0t[m];
+                            }
+                        }
+                        if (s == 0) sh.y0t[m] = v;
+                        if (s >= nL) v = 0.0;
+                        tb[trial_index<NM>(m, sl)] = v;
+                    }
+                }
+            }
             // load(c): the stored genomes of chunk c and its crossover words
-            // (this generation's and the rebuilt one's), straight into LDS
+            // (this generation's and the rebuilt one's), straight into LDS;
+            // issued after this step's stores, waited for one step later
             if (kind != kPassPre && c < nchunk) {
-                for (int j = wk; j < 16; j += 8) {
+                for (int j = wk; j < 16; j += kWorkers) {
                     const int od = j * 128 + lane * 2;
-                    const int m = od / C, x = od % C;
-                    double *dst = lds + kOffPop + (c % 3) * kCD + j * 128;
-                    if (m < d.P)
-                        __builtin_amdgcn_global_load_lds(
-                            (global_cvoid *)(d.pop + static_cast<size_t>(m) * d.Lpad + static_cast<size_t>(c) * C + x),
-                            (lds_void *)dst, 16, 0, 0);
+                    const int m = min(od / C, nP - 1), x = od % C;  // padded members: a copy of the last row
+                    double *dst = lds + kOffPop + (c % 4) * kCD + j * 128;
+                    __builtin_amdgcn_global_load_lds(
+                        (global_cvoid *)(d.pop + static_cast<size_t>(m) * nLpad + static_cast<size_t>(c) * C + x),
+                        (lds_void *)dst, 16, 0, 0);
                 }
                 if (wk < kMaskWords / kWave) {
                     const int e = wk * kWave + lane;
                     const int w = e % kWpc, cand = (e / kWpc) & 1, m = (e / (2 * kWpc)) % Ppad;
                     const int gs = e / (2 * kWpc * Ppad);
                     const int slot = gs == 0 ? gslot : pslot;
-                    const guint *src = gmask + ((static_cast<size_t>(slot) * Ppad + m) * 2 + cand) * d.Lw +
+                    const guint *src = gmask + ((static_cast<size_t>(slot) * Ppad + m) * 2 + cand) * nLw +
                                        static_cast<size_t>(c) * kWpc + w;
-                    __builtin_amdgcn_global_load_lds((global_cvoid *)src, (lds_void *)(sh.mring[c % 3] + wk * kWave), 4,
+                    __builtin_amdgcn_global_load_lds((global_cvoid *)src, (lds_void *)(sh.mring[c % 4] + wk * kWave), 4,
                                                      0, 0);
                 }
             }
-            // rebuild(c-1): genomes accepted in generation pg.  Three phases
-            // over the wave's four tasks (metadata, operand loads, results) so
-            // each costs one LDS round trip for all tasks.  Operands are
-            // addressed by 32-bit offsets into the dynamic LDS array.
-            const int c1 = c - 1;
-            if (rebuild && pg >= 0 && c1 >= 0 && c1 < nchunk) {
-                const int popo = kOffPop + (c1 % 3) * kCD;
-                const int newo = kOffNew + (c1 & 1) * kCD;
-                const uint32_t *mrb = sh.mring[c1 % 3] + Ppad * 2 * kWpc;
-                int o4[4][4], wo[4];
-                double Fj[4];
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int t = wk + 8 * j, m = t / kSegs, seg = t % kSegs;
-                    const int mv = m < d.P ? m : 0;
-                    const WalkRec &r = sh.rec[pg & 3][mv];
-                    const int sl = seg * 64 + lane, s = c1 * C + sl;
-                    o4[j][0] = popo + mv * C + sl;
-                    o4[j][1] = popo + static_cast<int>(r.mutant[0]) * C + sl;
-                    o4[j][2] = popo + static_cast<int>(r.mutant[1]) * C + sl;
-                    o4[j][3] = popo + static_cast<int>(r.mutant[2]) * C + sl;
-                    wo[j] = (mv * 2 + sh.accepted[pprev][mv]) * kWpc + (sl >> 5);
-                    Fj[j] = sh.f_trial[pg & 1][mv][(d.nsign == 2 && s >= d.G) ? 1 : 0];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-                double v4[4][4];
-                uint32_t w4[4];
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-#pragma unroll
-                    for (int k = 0; k < 4; k++) v4[j][k] = lds[o4[j][k]];
-                    w4[j] = mrb[wo[j]];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int t = wk + 8 * j, m = t / kSegs, seg = t % kSegs;
-                    if (m >= d.P || !sh.accepted[pslot][m]) continue;
-                    const int sl = seg * 64 + lane, s = c1 * C + sl;
-                    double v = de_value((w4[j] >> (s & 31)) & 1, Fj[j], v4[j][0], v4[j][1], v4[j][2], v4[j][3],
-                                        d.allow_neg);
-                    if (d.nsign == 2 && s == d.G) {  // deac.cpp:1305-1308
-                        if (c1 == 0) {
-                            const WalkRec &r = sh.rec[pg & 3][m];
-                            v = de_value(mrb[wo[j] - (sl >> 5)] & 1, sh.f_trial[pg & 1][m][0], lds[popo + m * C],
-                                         lds[popo + r.mutant[0] * C], lds[popo + r.mutant[1] * C],
-                                         lds[popo + r.mutant[2] * C], d.allow_neg);
-                        } else {
-                            v = sh.y0r[m];
-                        }
-                    }
-                    if (s == 0) sh.y0r[m] = v;
-                    if (s >= d.L) v = 0.0;
-                    lds[newo + m * C + sl] = v;
-                    if (s < d.L) gpop[static_cast<size_t>(m) * d.Lpad + s] = v;
-                }
+            // load(c-1) must have landed (the rebuild reads it next step);
+            // load(c) may stay in flight: it is the last 16/kWorkers pop
+            // copies plus, for workers 0-3, one mask-word copy
+            if (kind != kPassPre && c < nchunk) {
+                if (wk < kMaskWords / kWave) DEAC_WAIT_VM(16 / kWorkers + 1);
+                else DEAC_WAIT_VM(16 / kWorkers);
+            } else {
+                DEAC_WAIT_VM(0);
             }
-            // trial(c-2): this generation's trial vectors (or, for the
-            // initial evaluation, the stored genomes) in MFMA B order
-            const int c2 = c - 2;
-            if (chains && c2 >= 0 && c2 < nchunk) {
-                const int popo = kOffPop + (c2 % 3) * kCD;
-                const int newo = kOffNew + (c2 & 1) * kCD;
-                double *tb = lds + kOffTrial + (c2 & 1) * kCD;
-                const uint32_t *mrt = sh.mring[c2 % 3];
-                if (kind == kPassInit) {
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int t = wk + 8 * j, m = t / kSegs, seg = t % kSegs;
-                        if (m >= d.P) continue;
-                        const int sl = seg * 64 + lane, s = c2 * C + sl;
-                        tb[trial_index<NM>(m, sl)] = s < d.L ? lds[popo + m * C + sl] : 0.0;
-                    }
-                } else {
-                    int o4[4][4], wo[4];
-                    double Fj[4];
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int t = wk + 8 * j, m = t / kSegs, seg = t % kSegs;
-                        const int mv = m < d.P ? m : 0;
-                        const WalkRec &r = sh.rec[g & 3][mv];
-                        const int sl = seg * 64 + lane, s = c2 * C + sl;
-                        const int cand = sh.accepted[aslot][mv];
-                        o4[j][0] = (cand ? newo : popo) + mv * C + sl;
-#pragma unroll
-                        for (int k = 0; k < 3; k++) {
-                            const int x = static_cast<int>(r.mutant[k]);
-                            o4[j][k + 1] = (sh.accepted[aslot][x] ? newo : popo) + x * C + sl;
-                        }
-                        wo[j] = (mv * 2 + cand) * kWpc + (sl >> 5);
-                        Fj[j] = sh.f_trial[g & 1][mv][(d.nsign == 2 && s >= d.G) ? 1 : 0];
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-                    double v4[4][4];
-                    uint32_t w4[4];
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-#pragma unroll
-                        for (int k = 0; k < 4; k++) v4[j][k] = lds[o4[j][k]];
-                        w4[j] = mrt[wo[j]];
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int t = wk + 8 * j, m = t / kSegs, seg = t % kSegs;
-                        if (m >= d.P) continue;
-                        const int sl = seg * 64 + lane, s = c2 * C + sl;
-                        double v = de_value((w4[j] >> (s & 31)) & 1, Fj[j], v4[j][0], v4[j][1], v4[j][2], v4[j][3],
-                                            d.allow_neg);
-                        if (d.nsign == 2 && s == d.G) {
-                            if (c2 == 0) {
-                                const int sl0 = o4[j][0] - sl;
-                                v = de_value(mrt[wo[j] - (sl >> 5)] & 1, sh.f_trial[g & 1][m][0], lds[sl0],
-                                             lds[o4[j][1] - sl], lds[o4[j][2] - sl], lds[o4[j][3] - sl], d.allow_neg);
-                            } else {
-                                v = sh.y0t[m];
-                            }
-                        }
-                        if (s == 0) sh.y0t[m] = v;
-                        if (s >= d.L) v = 0.0;
-                        tb[trial_index<NM>(m, sl)] = v;
-                    }
-                }
-            }
-            DEAC_WAIT_VM(0);
             step_barrier();
         }
         __syncthreads();
@@ -564,7 +656,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         // The next generation's crossover draws, Lsub per lane per pass.
         MaskGen mg;
         const bool draws = mgen >= 0;
-        const int q = draws ? (d.Lsub + nsteps - 1) / nsteps : 0;
+        const int q = draws ? ((d.Lsub + nsteps - 1) / nsteps + 7) / 8 * 8 : 0;
         if (draws) mask_setup<NM>(mg, d, sh, mgen, kind == kPassEvolve, g, (wave - kRngWave0) * kWave + lane);
         for (int c = 0; c < nsteps; c++) {
             if (draws) mask_draws(mg, q, d.Lw);
@@ -572,7 +664,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         }
         DEAC_WAIT_VM(0);
         __syncthreads();
-    } else if (wave == kWalkWave) {
+    } else {
         // ------------------------------------------------ walk wave
         const int items = 2 * d.P;
         const int per = wgen >= 0 ? (items + nsteps - 1) / nsteps : 0;
@@ -586,81 +678,13 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         if (lane == 0)
             for (int w = 0; w < 4; w++) sh.walk_state[w] = ws[w];
         __syncthreads();
-    } else {
-        // ------------------------------------------------ remainder wave
-        const int nch = d.nrem * Ppad;
-        const int nu = (nch + kWave - 1) / kWave;
-        double racc[kBatchRemChains];
-        int rr[kBatchRemChains], rm[kBatchRemChains];
-#pragma unroll
-        for (int u = 0; u < kBatchRemChains; u++) {
-            const int k = lane + kWave * u;
-            racc[u] = 0.0;
-            rr[u] = k < nch ? k / Ppad : 0;
-            rm[u] = k < nch ? k % Ppad : 0;
-        }
-        double *kring = lds + 7 * kCD;
-        const int ncopy = (d.nrem * C + 127) / 128;
-        for (int c = 0; c < nsteps; c++) {
-            const int cm = c - 3;
-            // kernel rows of chunk cm+1 into the ring (used next step)
-            const int cn = cm + 1;
-            if (chains && nch > 0 && cn >= 0 && cn < nchunk) {
-                for (int j = 0; j < ncopy; j++) {
-                    const int od = j * 128 + lane * 2;
-                    const int r = od / C, x = od % C;
-                    if (r < d.nrem)
-                        __builtin_amdgcn_global_load_lds(
-                            (global_cvoid *)(d.krem + static_cast<size_t>(r) * d.Lpad + static_cast<size_t>(cn) * C + x),
-                            (lds_void *)(kring + (cn & 1) * kRemRing + j * 128), 16, 0, 0);
-                }
-            }
-            if (chains && nch > 0 && cm >= 0 && cm < nchunk) {
-                const double *tb = lds + kOffTrial + (cm & 1) * kCD;
-                const double *kb = kring + (cm & 1) * kRemRing;
-                // Straight-line chains (lanes past nch run on row 0 / member
-                // 0 and are discarded) so the LDS reads pipeline.
-#pragma unroll
-                for (int u = 0; u < kBatchRemChains; u++) {
-                    if (u < nu) {
-                        const double *krow = kb + rr[u] * C;
-                        double a = racc[u];
-                        // 8 positions per block; the lane offsets of a block
-                        // (trial swizzle) are formed per block.
-                        const int xm = rm[u] & 15, mb = (rm[u] >> 4) * kWave;
-#pragma unroll 1
-                        for (int s0 = 0; s0 < C; s0 += 8) {
-                            const double *tbb = tb + (s0 >> 2) * NM * kWave + mb;
-                            const int h = s0 & 8;
-                            double kv[8], sv[8];
-#pragma unroll
-                            for (int k = 0; k < 8; k++) {
-                                kv[k] = krow[s0 + k];
-                                sv[k] = tbb[(k >> 2) * NM * kWave + (k & 3) * 16 + (xm ^ (h + k))];
-                            }
-                            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                            for (int k = 0; k < 8; k++) a = fma(kv[k], sv[k], a);
-                        }
-                        racc[u] = a;
-                    }
-                }
-            }
-            DEAC_WAIT_VM(0);
-            step_barrier();
-        }
-        __syncthreads();
-        if (chains)
-#pragma unroll
-            for (int u = 0; u < kBatchRemChains; u++)
-                if (u < nu && lane + kWave * u < nch && rm[u] < d.P) lds[(d.ntt * 16 + rr[u]) * Ppad + rm[u]] = racc[u];
     }
     __syncthreads();
 }
 
 }  // namespace
 
-template <int NT, int NM>
+template <int NT, int NM, int REM>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *__restrict__ descs)
 {
     constexpr int Ppad = 16 * NM;
@@ -707,7 +731,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         const long pg = kind == kPassEvolve ? g : kind == kPassApply ? last : g0 - 1;
         const long mgen = kind == kPassEvolve ? g + 1 : kind == kPassApply ? -1 : g0;
         const long wgen = kind == kPassEvolve ? g + 2 : kind == kPassApply ? -1 : g0 + 1;
-        run_pass<NT, NM>(d, sh, lds, kind, pg, mgen, wgen);
+        run_pass<NT, NM, REM>(d, sh, lds, kind, pg, mgen, wgen);
         if (kind == kPassApply) break;
 
         if (kind == kPassInit || kind == kPassEvolve) {
@@ -807,34 +831,36 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
 
 namespace {
 
-template <int NT, int NM>
+template <int NT, int NM, int REM>
 hipError_t launch_one(const BatchDesc *descs, int count, hipStream_t s)
 {
-    constexpr size_t lds_bytes = (static_cast<size_t>(kBatchRegionDoubles) + 2 * kRemRing) * sizeof(double);
+    constexpr size_t lds_bytes = static_cast<size_t>(kBatchRegionDoubles) * sizeof(double);
     static bool configured = false;
     if (!configured) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<NT, NM>),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<NT, NM, REM>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes));
         if (e != hipSuccess) return e;
         configured = true;
     }
-    hipLaunchKernelGGL((batch_kernel<NT, NM>), dim3(count), dim3(kBatchThreads), lds_bytes, s, descs);
+    hipLaunchKernelGGL((batch_kernel<NT, NM, REM>), dim3(count), dim3(kBatchThreads), lds_bytes, s, descs);
     return hipGetLastError();
+}
+
+template <int NM, int REM>
+hipError_t launch_nt(const BatchDesc *descs, int count, int nt, hipStream_t s)
+{
+    if (nt == 1) return launch_one<1, NM, REM>(descs, count, s);
+    if (nt == 2) return launch_one<2, NM, REM>(descs, count, s);
+    if (nt == 4) return launch_one<4, NM, REM>(descs, count, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
 
-hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, hipStream_t s)
+hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, int rem, hipStream_t s)
 {
-    if (nm == 1) {
-        if (nt == 1) return launch_one<1, 1>(descs, count, s);
-        if (nt == 2) return launch_one<2, 1>(descs, count, s);
-        if (nt == 4) return launch_one<4, 1>(descs, count, s);
-    } else if (nm == 2) {
-        if (nt == 1) return launch_one<1, 2>(descs, count, s);
-        if (nt == 2) return launch_one<2, 2>(descs, count, s);
-        if (nt == 4) return launch_one<4, 2>(descs, count, s);
-    }
+    if (nm == 1) return rem ? launch_nt<1, 1>(descs, count, nt, s) : launch_nt<1, 0>(descs, count, nt, s);
+    if (nm == 2) return rem ? launch_nt<2, 1>(descs, count, nt, s) : launch_nt<2, 0>(descs, count, nt, s);
     return hipErrorInvalidValue;
 }
 

@@ -10,20 +10,21 @@
 
 namespace deac {
 
-constexpr int kBatchWaves = 16;             // 4 chain, 8 worker, 1 walk, 1 remainder, 2 mask
+constexpr int kBatchWaves = 16;             // 4 chain, 8 worker, 1 walk, 3 mask
 constexpr int kBatchThreads = kBatchWaves * 64;
-constexpr int kBatchMaskLanes = 128;        // 2 mask wavefronts
+constexpr int kBatchMaskLanes = 192;        // 3 mask wavefronts
 constexpr int kBatchChunkDoubles = 2048;    // members (padded) x genome positions per chunk
 constexpr int kBatchMaxP = 32;
-constexpr int kBatchRegionDoubles = 7 * kBatchChunkDoubles;  // 3 load + 2 update + 2 trial buffers
-constexpr int kBatchRemChains = 2;          // remainder-row chains per lane
+constexpr int kBatchRegionDoubles = 8 * kBatchChunkDoubles;  // 4 load + 2 update + 2 trial buffers
+constexpr int kBatchMaxRem = 4;             // kernel rows past the last full tile run on 4x4x4 MFMAs
 
 // Everything one workgroup needs about its reconstruction.  Model arrays may
 // be shared between instances; state arrays are the instance's own.
 struct BatchDesc {
     // model, laid out for the kernel by the host
     const double *ktile;       // [ntt][Lpad/8][64 lanes][2]: K tiles in MFMA A order
-    const double *krem;        // [nrem][Lpad]: kernel rows past the last full 16-row tile
+    const double *krem;        // [Lpad/8][64 lanes][2]: rows past the last full 16-row tile, in
+                               // v_mfma_f64_4x4x4_4b A order (row l&3, position 4k+(l>>4))
     const double *obs, *sig;   // [T] ISF and its error
     const uint64_t *walk_rows; // [256][4] rows of T^L, T the xoshiro transition
     const uint64_t *sub_tab;   // [S-1][64][16][4] nibble tables of T^(k*Lsub), k = 1..S-1
@@ -43,7 +44,8 @@ struct BatchDesc {
     int T, G, nsign, L, Lpad, Lw, nchunk, ntt, nrem, P, S, Lsub, allow_neg;
 };
 
-// NT: 16-row kernel tiles per chain wavefront (1, 2 or 4); NM: 16-member tiles (1 or 2).
-hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, hipStream_t s);
+// NT: 16-row kernel tiles per chain wavefront (1, 2 or 4); NM: 16-member tiles (1 or 2);
+// rem: 1 when the kernel has 1..4 rows past its last full tile.
+hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, int rem, hipStream_t s);
 
 }  // namespace deac

@@ -11,14 +11,19 @@ namespace deac {
 
 __device__ __forceinline__ uint64_t xnext(uint64_t (&s)[4])  // rng.hpp:25-39
 {
-    const uint64_t out = s[0] + s[3];
-    const uint64_t t = s[1] << 17;
-    s[2] ^= s[0];
-    s[3] ^= s[1];
-    s[1] ^= s[2];
-    s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = (s[3] << 45) | (s[3] >> 19);
+    // The reference's five in-place xors, written as three-input xors of the
+    // old state (v_xor3_b32) and a rotate by two funnel shifts.
+    const uint64_t s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
+    const uint64_t out = s0 + s3;
+    const uint64_t t = s1 << 17;
+    const uint64_t n3 = s3 ^ s1;
+    s[1] = s1 ^ s2 ^ s0;
+    s[0] = s0 ^ n3;
+    s[2] = s2 ^ s0 ^ t;
+    const uint32_t lo = static_cast<uint32_t>(n3), hi = static_cast<uint32_t>(n3 >> 32);
+    // rotl(n3, 45) = rotr(n3, 19)
+    const uint32_t rlo = __builtin_amdgcn_alignbit(hi, lo, 19), rhi = __builtin_amdgcn_alignbit(lo, hi, 19);
+    s[3] = (static_cast<uint64_t>(rhi) << 32) | rlo;
     return out;
 }
 
