@@ -22,8 +22,9 @@
 //         sub-streams of each member's mask stream.
 // The genome is processed in chunks of C positions (C * padded members = 2048
 // doubles); the stages of one chunk run in consecutive steps separated by
-// workgroup barriers: load(c), rebuild(c-2), trial(c-3), chains(c-4); a
-// chunk's DMA has a full step to land before it is read.
+// workgroup barriers: load(c), rebuild(c-1), trial(c-2), chains(c-3).  The
+// workers issue a chunk's global loads at the top of its step and store the
+// data into LDS at the bottom, so the loads fly while the step computes.
 //
 // Population storage: [P][Lpad] member-major, one buffer.  A generation's
 // accepted trials are not written when they are selected; the next
@@ -54,8 +55,6 @@ constexpr int kMaskWords = 256;        // mask words per chunk: 2 generations x 
 enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3 };
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(1))) const void global_cvoid;
-typedef __attribute__((address_space(3))) void lds_void;
 // Global-memory views of the descriptor's pointers: loads and stores through
 // them are global_* instructions, counted by vmcnt only, so LDS waits never
 // wait for them.
@@ -63,6 +62,7 @@ typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) const double gcdouble;
 typedef __attribute__((address_space(1))) uint32_t guint;
 typedef __attribute__((address_space(1))) const uint64_t gcu64;
+typedef __attribute__((address_space(1))) const char gchar;
 
 // Random choices of one member in one generation (deac.cpp:1180-1240).
 struct WalkRec {
@@ -322,7 +322,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     const int nchunk = d.nchunk;
     const bool chains = kind == kPassInit || kind == kPassEvolve;
-    const int nsteps = kind == kPassApply ? nchunk + 2 : nchunk + 4;
+    const int nsteps = kind == kPassApply ? nchunk + 1 : nchunk + 3;
 
     if (wave < kWorker0) {
         // ------------------------------------------------ chain waves
@@ -373,11 +373,11 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
 #pragma unroll
             for (int p = 0; p < KD; p++) krr[p] = load2(krm + static_cast<size_t>(min(p, nkb2 - 1)) * (2 * kWave) + lo);
         }
-        // Steps 0-3 fill the pipeline, steps 4 .. nchunk+3 run the chains of
-        // chunk c-4; the hot loop has no branch, so the waits for the K
+        // Steps 0-2 fill the pipeline, steps 3 .. nchunk+2 run the chains of
+        // chunk c-3; the hot loop has no branch, so the waits for the K
         // prefetch are exact across its iterations.
         const int nwork = chains ? nchunk : 0;
-        for (int c = 0; c < min(4, nsteps); c++) step_barrier();
+        for (int c = 0; c < min(3, nsteps); c++) step_barrier();
         for (int cm = 0; cm < nwork; cm++) {
             {
                 const double *tb = lds + kOffTrial + (cm & 1) * kCD;
@@ -432,7 +432,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             }
             step_barrier();
         }
-        for (int c = min(4, nsteps) + nwork; c < nsteps; c++) step_barrier();
+        for (int c = min(3, nsteps) + nwork; c < nsteps; c++) step_barrier();
         __syncthreads();
         // E[t][m] into LDS (the chunk buffers are free now): lane l holds
         // rows 4v + l/16 of the tile, column l%16.
@@ -476,7 +476,6 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         const int pprev = static_cast<int>((pg + 2) % 3);     // generation pg-1
         const int gslot = static_cast<int>((g + 3) % 3);
         const int aslot = static_cast<int>((g + 2) % 3);      // generation g-1
-        const guint *gmask = (const guint *)d.masks;
         gdouble *gpop = (gdouble *)d.pop;
         // Per-pass task descriptors: donors r1 | r2<<5 | r3<<10, mask
         // candidate <<15, operand from the rebuilt buffer (parent, r1, r2, r3)
@@ -505,14 +504,52 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                        (static_cast<uint32_t>(sh.accepted[pprev][m] & 1) << 15) | (1u << 20);
             }
         }
+        // load(c): the stored genomes of chunk c ([Ppad][C], 16 rows of 128
+        // doubles, rows wk and wk+8 for this wave; padded members copy the
+        // last member's row) and, for workers 0-3, its crossover words (this
+        // generation's and the rebuilt one's).  Issued at the top of step c
+        // into registers, written to LDS at its end: plain loads, so no LDS
+        // read in between waits for them (an LDS DMA would make every LDS
+        // read wait for the copy in flight).
+        // Addresses: a wave-uniform chunk base plus per-lane 32-bit byte
+        // offsets (the population of one instance stays below 2 GiB).
+        constexpr int kRows = 16 / kWorkers;
+        uint32_t poff[kRows], moff = 0;
+#pragma unroll
+        for (int u = 0; u < kRows; u++) {
+            const int od = (wk + kWorkers * u) * 128 + lane * 2;
+            const int m = min(od / C, nP - 1), x = od % C;
+            poff[u] = static_cast<uint32_t>(m * nLpad + x) * 8u;
+        }
+        if (wk < kMaskWords / kWave) {
+            const int e = wk * kWave + lane;
+            const int w = e % kWpc, cand = (e / kWpc) & 1, m = (e / (2 * kWpc)) % Ppad;
+            const int slot = e / (2 * kWpc * Ppad) == 0 ? gslot : pslot;
+            moff = static_cast<uint32_t>(((slot * Ppad + m) * 2 + cand) * nLw + w) * 4u;
+        }
+        double2 pv[kRows];
+        uint32_t mw = 0;
         for (int c = 0; c < nsteps; c++) {
-            // rebuild(c-2): genomes accepted in generation pg, then trial(c-3):
+            // The lane index passes through an empty asm each step, so the
+            // per-task offsets derived from it are recomputed here rather
+            // than held in registers across the loop (which spills them).
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const bool ld = kind != kPassPre && c < nchunk;
+            if (ld) {
+                const gchar *pb = (const gchar *)d.pop + static_cast<size_t>(c) * C * 8;
+#pragma unroll
+                for (int u = 0; u < kRows; u++) pv[u] = load2((const gcdouble *)(pb + poff[u]));
+                if (wk < kMaskWords / kWave)
+                    mw = *(const guint *)((const gchar *)d.masks + static_cast<size_t>(c) * kWpc * 4 + moff);
+            }
+            // rebuild(c-1): genomes accepted in generation pg, then trial(c-2):
             // this generation's trial vectors (or, for the initial evaluation,
             // the stored genomes) in MFMA B order.  Each phase loads the
             // operands, F and mask words of all its tasks at once (one LDS
             // round trip), then computes; operands are addressed by 32-bit
             // offsets into the dynamic LDS array.
-            const int c1 = c - 2;
+            const int c1 = c - 1;
             if (rebuild && pg >= 0 && c1 >= 0 && c1 < nchunk) {
                 const int popo = kOffPop + (c1 % 4) * kCD;
                 const int newo = kOffNew + (c1 & 1) * kCD;
@@ -524,7 +561,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     const uint32_t ds = __builtin_amdgcn_readlane(dtab, kTasks + j);
                     if (!((ds >> 20) & 1)) continue;  // most members were not accepted
                     const int t = wk + kWorkers * j, m = min(t / kSegs, nP - 1), seg = t % kSegs;
-                    const int sl = seg * 64 + lane, s = c1 * C + sl;
+                    const int sl = seg * 64 + ln, s = c1 * C + sl;
                     v4[j][0] = lds[popo + m * C + sl];
                     v4[j][1] = lds[popo + static_cast<int>(ds & 31) * C + sl];
                     v4[j][2] = lds[popo + static_cast<int>((ds >> 5) & 31) * C + sl];
@@ -538,7 +575,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     const uint32_t ds = __builtin_amdgcn_readlane(dtab, kTasks + j);
                     if (!((ds >> 20) & 1)) continue;
                     const int t = wk + kWorkers * j, m = t / kSegs, seg = t % kSegs;
-                    const int sl = seg * 64 + lane, s = c1 * C + sl;
+                    const int sl = seg * 64 + ln, s = c1 * C + sl;
                     double v = de_value((w4[j] >> (s & 31)) & 1, Fv[j], v4[j][0], v4[j][1], v4[j][2], v4[j][3],
                                         aneg);
                     if (n2 && s == nG) {  // deac.cpp:1305-1308
@@ -555,10 +592,10 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     if (s == 0) sh.y0r[m] = v;
                     if (s >= nL) v = 0.0;
                     lds[newo + m * C + sl] = v;
-                    if (s < nL) gpop[static_cast<size_t>(m) * nLpad + s] = v;
+                    if (s < nL) (gpop + (static_cast<size_t>(m) * nLpad + c1 * C + seg * 64))[ln] = v;
                 }
             }
-            const int c2 = c - 3;
+            const int c2 = c - 2;
             if (chains && c2 >= 0 && c2 < nchunk) {
                 const int popo = kOffPop + (c2 % 4) * kCD;
                 const int newo = kOffNew + (c2 & 1) * kCD;
@@ -569,7 +606,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     for (int j = 0; j < kTasks; j++) {
                         const int t = wk + kWorkers * j, m = t / kSegs, seg = t % kSegs;
                         if (m >= nP) continue;
-                        const int sl = seg * 64 + lane, s = c2 * C + sl;
+                        const int sl = seg * 64 + ln, s = c2 * C + sl;
                         tb[trial_index<NM>(m, sl)] = s < nL ? lds[popo + m * C + sl] : 0.0;
                     }
                 } else {
@@ -579,7 +616,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     for (int j = 0; j < kTasks; j++) {
                         const uint32_t ds = __builtin_amdgcn_readlane(dtab, j);
                         const int t = wk + kWorkers * j, m = min(t / kSegs, nP - 1), seg = t % kSegs;
-                        const int sl = seg * 64 + lane, s = c2 * C + sl;
+                        const int sl = seg * 64 + ln, s = c2 * C + sl;
                         v4[j][0] = lds[(((ds >> 16) & 1) ? newo : popo) + m * C + sl];
 #pragma unroll
                         for (int k = 0; k < 3; k++)
@@ -594,7 +631,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                         const uint32_t ds = __builtin_amdgcn_readlane(dtab, j);
                         if (!((ds >> 20) & 1)) continue;
                         const int t = wk + kWorkers * j, m = t / kSegs, seg = t % kSegs;
-                        const int sl = seg * 64 + lane, s = c2 * C + sl;
+                        const int sl = seg * 64 + ln, s = c2 * C + sl;
                         double v = de_value((w4[j] >> (s & 31)) & 1, Fv[j], v4[j][0], v4[j][1], v4[j][2], v4[j][3],
                                             aneg);
                         if (n2 && s == nG) {
@@ -616,37 +653,14 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     }
                 }
             }
-            // load(c): the stored genomes of chunk c and its crossover words
-            // (this generation's and the rebuilt one's), straight into LDS;
-            // issued after this step's stores, waited for one step later
-            if (kind != kPassPre && c < nchunk) {
-                for (int j = wk; j < 16; j += kWorkers) {
-                    const int od = j * 128 + lane * 2;
-                    const int m = min(od / C, nP - 1), x = od % C;  // padded members: a copy of the last row
-                    double *dst = lds + kOffPop + (c % 4) * kCD + j * 128;
-                    __builtin_amdgcn_global_load_lds(
-                        (global_cvoid *)(d.pop + static_cast<size_t>(m) * nLpad + static_cast<size_t>(c) * C + x),
-                        (lds_void *)dst, 16, 0, 0);
-                }
-                if (wk < kMaskWords / kWave) {
-                    const int e = wk * kWave + lane;
-                    const int w = e % kWpc, cand = (e / kWpc) & 1, m = (e / (2 * kWpc)) % Ppad;
-                    const int gs = e / (2 * kWpc * Ppad);
-                    const int slot = gs == 0 ? gslot : pslot;
-                    const guint *src = gmask + ((static_cast<size_t>(slot) * Ppad + m) * 2 + cand) * nLw +
-                                       static_cast<size_t>(c) * kWpc + w;
-                    __builtin_amdgcn_global_load_lds((global_cvoid *)src, (lds_void *)(sh.mring[c % 4] + wk * kWave), 4,
-                                                     0, 0);
-                }
-            }
-            // load(c-1) must have landed (the rebuild reads it next step);
-            // load(c) may stay in flight: it is the last 16/kWorkers pop
-            // copies plus, for workers 0-3, one mask-word copy
-            if (kind != kPassPre && c < nchunk) {
-                if (wk < kMaskWords / kWave) DEAC_WAIT_VM(16 / kWorkers + 1);
-                else DEAC_WAIT_VM(16 / kWorkers);
-            } else {
-                DEAC_WAIT_VM(0);
+            // load(c) lands: the chunk's genomes and crossover words go from
+            // registers into LDS for the next step's rebuild
+            if (ld) {
+#pragma unroll
+                for (int u = 0; u < kRows; u++)
+                    *reinterpret_cast<double2 *>(lds + kOffPop + (c % 4) * kCD + (wk + kWorkers * u) * 128 + 2 * ln) =
+                        pv[u];
+                if (wk < kMaskWords / kWave) sh.mring[c % 4][wk * kWave + ln] = mw;
             }
             step_barrier();
         }
@@ -688,7 +702,7 @@ template <int NT, int NM, int REM>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *__restrict__ descs)
 {
     constexpr int Ppad = 16 * NM;
-    extern __shared__ double lds[];
+    extern __shared__ __attribute__((aligned(16))) double lds[];  // 16-byte aligned: ds_*_b128 at full rate
     __shared__ BatchShared sh;
     const BatchDesc &d = descs[blockIdx.x];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
