@@ -22,15 +22,17 @@
 //         sub-streams of each member's mask stream.
 // The genome is processed in chunks of C positions (C * padded members = 2048
 // doubles); the stages of one chunk run in consecutive steps separated by
-// workgroup barriers: load(c), rebuild(c-1), trial(c-2), chains(c-3).  The
-// workers issue a chunk's global loads at the top of its step and store the
-// data into LDS at the bottom, so the loads fly while the step computes.
+// workgroup barriers: load(c), rebuild(c-1), write-back(c-2), trial(c-3),
+// chains(c-4).  The workers issue a chunk's global loads at the top of its
+// step and store the data into LDS at the bottom, so the loads fly while the
+// step computes.
 //
 // Population storage: [P][Lpad] member-major, one buffer.  A generation's
 // accepted trials are not written when they are selected; the next
 // generation recomputes them from the same inputs (stored genomes, masks, F,
-// donors) while its chunks pass through LDS, and writes them back then, so
-// HBM sees one read of the population and one write per accepted genome.
+// donors) while its chunks pass through LDS and writes them over the chunk in
+// place (LDS and HBM), so HBM sees one read of the population and one write
+// per accepted genome, and the trial vectors read a single chunk buffer.
 #include <hip/hip_runtime.h>
 
 #include "batch_kernels.hpp"
@@ -43,13 +45,12 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kWorker0 = 4;
 constexpr int kWorkers = 8;            // 4-11: workers
-constexpr int kTasks = 32 / kWorkers;  // 64-position tasks per worker and step
+constexpr int kT = 16 / kWorkers;      // chunk rows (of 128 doubles) per worker
 constexpr int kWalkWave = 12;
 constexpr int kRngWave0 = 13;          // 13-15: mask waves
 constexpr int kCD = kBatchChunkDoubles;
-constexpr int kOffPop = 0;             // 4 buffers: genomes as stored, [Ppad][C]
-constexpr int kOffNew = 4 * kCD;       // 2 buffers: rebuilt (accepted) genomes, [Ppad][C]
-constexpr int kOffTrial = 6 * kCD;     // 2 buffers: trial genomes in MFMA B order
+constexpr int kOffPop = 0;             // 4 buffers: genomes of chunks in flight, [Ppad][C]
+constexpr int kOffTrial = 4 * kCD;     // 2 buffers: trial genomes in MFMA B order
 constexpr int kMaskWords = 256;        // mask words per chunk: 2 generations x Ppad x 2 candidates x C/32
 
 enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3 };
@@ -127,12 +128,12 @@ __device__ __forceinline__ uint64_t cr_threshold(double cr)
     return static_cast<uint64_t>(ceil(cr * 0x1.0p53));
 }
 
-__device__ __forceinline__ double de_value(uint32_t bit, double F, double par, double x1, double x2, double x3,
-                                           int allow_neg)
+// A DE value F (x2 - x3) + x1 (deac.cpp:1275-1290) made non-negative by
+// clearing the sign bit, or kept (absmask all ones) for allow_negative.
+__device__ __forceinline__ double de_abs(double v, uint32_t absmask)
 {
-    if (!bit) return par;
-    const double v = fma(F, x2 - x3, x1);  // deac.cpp:1275-1290
-    return allow_neg ? v : fabs(v);
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    return __builtin_bit_cast(double, b & ((static_cast<uint64_t>(absmask) << 32) | 0xffffffffu));
 }
 
 // s <- M s with M given as a nibble table tab[n][v] = M (v << 4n).
@@ -322,7 +323,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     const int nchunk = d.nchunk;
     const bool chains = kind == kPassInit || kind == kPassEvolve;
-    const int nsteps = kind == kPassApply ? nchunk + 1 : nchunk + 3;
+    const int nsteps = kind == kPassApply ? nchunk + 2 : nchunk + 4;
 
     if (wave < kWorker0) {
         // ------------------------------------------------ chain waves
@@ -373,11 +374,11 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
 #pragma unroll
             for (int p = 0; p < KD; p++) krr[p] = load2(krm + static_cast<size_t>(min(p, nkb2 - 1)) * (2 * kWave) + lo);
         }
-        // Steps 0-2 fill the pipeline, steps 3 .. nchunk+2 run the chains of
-        // chunk c-3; the hot loop has no branch, so the waits for the K
+        // Steps 0-3 fill the pipeline, steps 4 .. nchunk+3 run the chains of
+        // chunk c-4; the hot loop has no branch, so the waits for the K
         // prefetch are exact across its iterations.
         const int nwork = chains ? nchunk : 0;
-        for (int c = 0; c < min(3, nsteps); c++) step_barrier();
+        for (int c = 0; c < min(4, nsteps); c++) step_barrier();
         for (int cm = 0; cm < nwork; cm++) {
             {
                 const double *tb = lds + kOffTrial + (cm & 1) * kCD;
@@ -432,7 +433,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             }
             step_barrier();
         }
-        for (int c = min(3, nsteps) + nwork; c < nsteps; c++) step_barrier();
+        for (int c = min(4, nsteps) + nwork; c < nsteps; c++) step_barrier();
         __syncthreads();
         // E[t][m] into LDS (the chunk buffers are free now): lane l holds
         // rows 4v + l/16 of the tile, column l%16.
@@ -459,16 +460,19 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         }
     } else if (wave < kWalkWave) {
         // ------------------------------------------------ workers
+        // Worker wk owns rows wk and wk + 8 of the chunk buffers ([Ppad][C]
+        // doubles, 16 rows of 128).  Lane l of row R holds positions p, p + 1
+        // (p = 2l mod C) of member m = (128 R + 2l) / C in every stage: the
+        // global load of the chunk, the rebuild of the genomes accepted in
+        // the previous generation, and this generation's trial vectors.
         const int wk = wave - kWorker0;
-        // Descriptor fields held in scalar registers for the whole pass
-        // (readfirstlane keeps them from being re-loaded from memory, which
-        // would make every task wait on the scalar cache).
+        // Descriptor fields held in scalar registers for the whole pass.
         const int nP = __builtin_amdgcn_readfirstlane(d.P), nG = __builtin_amdgcn_readfirstlane(d.G);
         const int nL = __builtin_amdgcn_readfirstlane(d.L), nLpad = __builtin_amdgcn_readfirstlane(d.Lpad);
         const int n2 = __builtin_amdgcn_readfirstlane(d.nsign) == 2 ? 1 : 0;
-        const int aneg = __builtin_amdgcn_readfirstlane(d.allow_neg);
         const int nLw = __builtin_amdgcn_readfirstlane(d.Lw);
-        constexpr int kSegs = C / 64;
+        // fabs of a DE value, or not (allow_negative): an and of the high word
+        const uint32_t absmask = __builtin_amdgcn_readfirstlane(d.allow_neg) ? 0xffffffffu : 0x7fffffffu;
         constexpr int kWpc = C / 32;  // mask words per member and chunk
         const bool rebuild = kind == kPassEvolve || kind == kPassApply;
         const long pg = kind == kPassEvolve ? g - 1 : g;      // generation whose accepted trials are rebuilt
@@ -476,191 +480,176 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         const int pprev = static_cast<int>((pg + 2) % 3);     // generation pg-1
         const int gslot = static_cast<int>((g + 3) % 3);
         const int aslot = static_cast<int>((g + 2) % 3);      // generation g-1
-        gdouble *gpop = (gdouble *)d.pop;
-        // Per-pass task descriptors: donors r1 | r2<<5 | r3<<10, mask
-        // candidate <<15, operand from the rebuilt buffer (parent, r1, r2, r3)
-        // <<16..19, active <<20.  Lane j holds trial task j, lane kTasks+j
-        // rebuild task j; readlane makes them wave-uniform where used.
-        uint32_t dtab = 0;
-        if (lane < 2 * kTasks) {
-            const int j = lane % kTasks;
-            const int t = wk + kWorkers * j, m = t / kSegs;
-            // The walk record is read only when the task is active: the
-            // record of the generation before a launch's first one was never
-            // written in that launch.
-            if (lane < kTasks) {
-                if (kind == kPassEvolve && m < nP) {
-                    const WalkRec &r = sh.rec[g & 3][m];
-                    const uint32_t cand = sh.accepted[aslot][m] & 1;
-                    uint32_t td = (r.mutant[0] & 31) | ((r.mutant[1] & 31) << 5) | ((r.mutant[2] & 31) << 10) |
-                                  (cand << 15) | (cand << 16) | (1u << 20);
-                    for (int k = 0; k < 3; k++)
-                        td |= static_cast<uint32_t>(sh.accepted[aslot][r.mutant[k] & 31] & 1) << (17 + k);
-                    dtab = td;
-                }
-            } else if (rebuild && pg >= 0 && m < nP && sh.accepted[pslot][m]) {
-                const WalkRec &r = sh.rec[pg & 3][m];
-                dtab = (r.mutant[0] & 31) | ((r.mutant[1] & 31) << 5) | ((r.mutant[2] & 31) << 10) |
-                       (static_cast<uint32_t>(sh.accepted[pprev][m] & 1) << 15) | (1u << 20);
-            }
-        }
-        // load(c): the stored genomes of chunk c ([Ppad][C], 16 rows of 128
-        // doubles, rows wk and wk+8 for this wave; padded members copy the
-        // last member's row) and, for workers 0-3, its crossover words (this
-        // generation's and the rebuilt one's).  Issued at the top of step c
-        // into registers, written to LDS at its end: plain loads, so no LDS
-        // read in between waits for them (an LDS DMA would make every LDS
-        // read wait for the copy in flight).
-        // Addresses: a wave-uniform chunk base plus per-lane 32-bit byte
-        // offsets (the population of one instance stays below 2 GiB).
-        constexpr int kRows = 16 / kWorkers;
-        uint32_t poff[kRows], moff = 0;
+        const int p = (2 * lane) % C;
+        // Per-row constants of the pass.  Padded members (m >= P) compute on
+        // the last member's data: their trial columns only feed their own
+        // (discarded) model values.
+        int mrow[kT], od[kT][3], odb[kT][3], mo[kT], mob[kT], td[kT][2];
+        uint32_t poff[kT];
+        uint64_t bm[kT];  // lanes whose member was accepted in generation pg
 #pragma unroll
-        for (int u = 0; u < kRows; u++) {
-            const int od = (wk + kWorkers * u) * 128 + lane * 2;
-            const int m = min(od / C, nP - 1), x = od % C;
-            poff[u] = static_cast<uint32_t>(m * nLpad + x) * 8u;
+        for (int u = 0; u < kT; u++) {
+            const int m = ((wk + kWorkers * u) * 128 + 2 * lane) / C, cm = min(m, nP - 1);
+            mrow[u] = m;
+            poff[u] = static_cast<uint32_t>(cm * nLpad + p) * 8u;
+            td[u][0] = trial_index<NM>(m, p);
+            td[u][1] = trial_index<NM>(m, p + 1);
+            od[u][0] = od[u][1] = od[u][2] = m * C + p;
+            odb[u][0] = odb[u][1] = odb[u][2] = m * C + p;
+            mo[u] = mob[u] = 0;
+            if (kind == kPassEvolve) {
+                const WalkRec &r = sh.rec[g & 3][cm];
+#pragma unroll
+                for (int k = 0; k < 3; k++) od[u][k] = static_cast<int>(r.mutant[k] & 31) * C + p;
+                mo[u] = (cm * 2 + (sh.accepted[aslot][cm] & 1)) * kWpc + (p >> 5);
+            }
+            // The walk record of pg is read only for accepted members: the
+            // record of the generation before a launch's first one was never
+            // written in that launch (and nothing is accepted there).
+            const bool acc = rebuild && pg >= 0 && m < nP && sh.accepted[pslot][cm];
+            if (acc) {
+                const WalkRec &r = sh.rec[pg & 3][cm];
+#pragma unroll
+                for (int k = 0; k < 3; k++) odb[u][k] = static_cast<int>(r.mutant[k] & 31) * C + p;
+                mob[u] = Ppad * 2 * kWpc + (cm * 2 + (sh.accepted[pprev][cm] & 1)) * kWpc + (p >> 5);
+            }
+            bm[u] = __ballot(acc);
         }
+        // Crossover words of chunk c, loaded by workers 0-3: this
+        // generation's (both candidates) and generation pg's.
+        uint32_t moff = 0;
         if (wk < kMaskWords / kWave) {
             const int e = wk * kWave + lane;
             const int w = e % kWpc, cand = (e / kWpc) & 1, m = (e / (2 * kWpc)) % Ppad;
             const int slot = e / (2 * kWpc * Ppad) == 0 ? gslot : pslot;
             moff = static_cast<uint32_t>(((slot * Ppad + m) * 2 + cand) * nLw + w) * 4u;
         }
-        double2 pv[kRows];
+
+        // DE values at positions p, p+1 of chunk cc for row u: operands
+        // (parent, r1, r2, r3) from chunk buffer buf, crossover word mw0,
+        // jDE weights fw (deac.cpp:1261-1311).  Chunks holding position 0,
+        // position G (the zero frequency of the negative half, which repeats
+        // position 0: deac.cpp:1305-1308) or padding take the per-position
+        // path; elsewhere the sign of the frequency is uniform.
+        auto de_pair = [&](const double *buf, const uint32_t *mr, int mw_idx, const int (&o)[3], int m, int cc,
+                           const double (&fw)[2], double *y0) -> double2 {
+            const double2 x0 = *reinterpret_cast<const double2 *>(buf + m * C + p);
+            const double2 x1 = *reinterpret_cast<const double2 *>(buf + o[0]);
+            const double2 x2 = *reinterpret_cast<const double2 *>(buf + o[1]);
+            const double2 x3 = *reinterpret_cast<const double2 *>(buf + o[2]);
+            const uint32_t w = mr[mw_idx] >> (p & 31);
+            const bool slow = cc == 0 || (n2 && cc == nG / C) || (cc + 1) * C > nL;
+            double2 v;
+            if (!slow) {
+                const double F = (n2 && cc * C >= nG) ? fw[1] : fw[0];
+                v.x = (w & 1) ? de_abs(fma(F, x2.x - x3.x, x1.x), absmask) : x0.x;
+                v.y = (w & 2) ? de_abs(fma(F, x2.y - x3.y, x1.y), absmask) : x0.y;
+            } else {
+                double r[2];
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int sp = cc * C + p + e;
+                    const double F = (n2 && sp >= nG) ? fw[1] : fw[0];
+                    const double a0 = e ? x0.y : x0.x, a1 = e ? x1.y : x1.x, a2 = e ? x2.y : x2.x,
+                                 a3 = e ? x3.y : x3.x;
+                    double t = ((w >> e) & 1) ? de_abs(fma(F, a2 - a3, a1), absmask) : a0;
+                    if (n2 && sp == nG) {
+                        if (cc == 0) {
+                            // position 0 of the same chunk, positive frequencies
+                            const uint32_t w0 = mr[mw_idx - (p >> 5)];
+                            t = (w0 & 1) ? de_abs(fma(fw[0], buf[o[1] - p] - buf[o[2] - p], buf[o[0] - p]), absmask)
+                                         : buf[m * C];
+                        } else {
+                            t = y0[m];
+                        }
+                    }
+                    if (sp == 0) y0[m] = t;
+                    if (sp >= nL) t = 0.0;
+                    r[e] = t;
+                }
+                v.x = r[0];
+                v.y = r[1];
+            }
+            return v;
+        };
+
+        // load(c) goes to registers at the top of step c and into LDS at its
+        // bottom: plain loads, so no LDS access in between waits for them (an
+        // LDS DMA would make every LDS read wait for the copy in flight).
+        // rebuild(c-1) computes the genomes accepted in pg from the chunk as
+        // loaded; they replace it in place (LDS and HBM) one step later, once
+        // every rebuild has read the old values; trial(c-3) then reads this
+        // generation's genomes from one buffer.
+        double2 pv[kT], rb[kT];
         uint32_t mw = 0;
         for (int c = 0; c < nsteps; c++) {
-            // The lane index passes through an empty asm each step, so the
-            // per-task offsets derived from it are recomputed here rather
-            // than held in registers across the loop (which spills them).
-            int ln = lane;
-            asm volatile("" : "+v"(ln));
             const bool ld = kind != kPassPre && c < nchunk;
             if (ld) {
                 const gchar *pb = (const gchar *)d.pop + static_cast<size_t>(c) * C * 8;
 #pragma unroll
-                for (int u = 0; u < kRows; u++) pv[u] = load2((const gcdouble *)(pb + poff[u]));
+                for (int u = 0; u < kT; u++) pv[u] = load2((const gcdouble *)(pb + poff[u]));
                 if (wk < kMaskWords / kWave)
                     mw = *(const guint *)((const gchar *)d.masks + static_cast<size_t>(c) * kWpc * 4 + moff);
             }
-            // rebuild(c-1): genomes accepted in generation pg, then trial(c-2):
-            // this generation's trial vectors (or, for the initial evaluation,
-            // the stored genomes) in MFMA B order.  Each phase loads the
-            // operands, F and mask words of all its tasks at once (one LDS
-            // round trip), then computes; operands are addressed by 32-bit
-            // offsets into the dynamic LDS array.
-            const int c1 = c - 1;
-            if (rebuild && pg >= 0 && c1 >= 0 && c1 < nchunk) {
-                const int popo = kOffPop + (c1 % 4) * kCD;
-                const int newo = kOffNew + (c1 & 1) * kCD;
-                const uint32_t *mrb = sh.mring[c1 % 4] + Ppad * 2 * kWpc;
-                double v4[kTasks][4], Fv[kTasks];
-                uint32_t w4[kTasks];
+            // rebuild write-back of chunk c-2
+            const int cw = c - 2;
+            if (rebuild && cw >= 0 && cw < nchunk) {
+                double *buf = lds + kOffPop + (cw % 4) * kCD;
+                gchar *gb = (gchar *)d.pop + static_cast<size_t>(cw) * C * 8;
 #pragma unroll
-                for (int j = 0; j < kTasks; j++) {
-                    const uint32_t ds = __builtin_amdgcn_readlane(dtab, kTasks + j);
-                    if (!((ds >> 20) & 1)) continue;  // most members were not accepted
-                    const int t = wk + kWorkers * j, m = min(t / kSegs, nP - 1), seg = t % kSegs;
-                    const int sl = seg * 64 + ln, s = c1 * C + sl;
-                    v4[j][0] = lds[popo + m * C + sl];
-                    v4[j][1] = lds[popo + static_cast<int>(ds & 31) * C + sl];
-                    v4[j][2] = lds[popo + static_cast<int>((ds >> 5) & 31) * C + sl];
-                    v4[j][3] = lds[popo + static_cast<int>((ds >> 10) & 31) * C + sl];
-                    w4[j] = mrb[(m * 2 + static_cast<int>((ds >> 15) & 1)) * kWpc + (sl >> 5)];
-                    Fv[j] = sh.f_trial[pg & 1][m][(n2 && s >= nG) ? 1 : 0];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int j = 0; j < kTasks; j++) {
-                    const uint32_t ds = __builtin_amdgcn_readlane(dtab, kTasks + j);
-                    if (!((ds >> 20) & 1)) continue;
-                    const int t = wk + kWorkers * j, m = t / kSegs, seg = t % kSegs;
-                    const int sl = seg * 64 + ln, s = c1 * C + sl;
-                    double v = de_value((w4[j] >> (s & 31)) & 1, Fv[j], v4[j][0], v4[j][1], v4[j][2], v4[j][3],
-                                        aneg);
-                    if (n2 && s == nG) {  // deac.cpp:1305-1308
-                        if (c1 == 0) {
-                            const int wb = (m * 2 + static_cast<int>((ds >> 15) & 1)) * kWpc;
-                            v = de_value(mrb[wb] & 1, sh.f_trial[pg & 1][m][0], lds[popo + m * C],
-                                         lds[popo + static_cast<int>(ds & 31) * C],
-                                         lds[popo + static_cast<int>((ds >> 5) & 31) * C],
-                                         lds[popo + static_cast<int>((ds >> 10) & 31) * C], aneg);
-                        } else {
-                            v = sh.y0r[m];
-                        }
+                for (int u = 0; u < kT; u++)
+                    if (bm[u] && ((bm[u] >> lane) & 1)) {
+                        *reinterpret_cast<double2 *>(buf + mrow[u] * C + p) = rb[u];
+                        gdouble *q = (gdouble *)(gb + poff[u]);
+                        q[0] = rb[u].x;
+                        q[1] = rb[u].y;
                     }
-                    if (s == 0) sh.y0r[m] = v;
-                    if (s >= nL) v = 0.0;
-                    lds[newo + m * C + sl] = v;
-                    if (s < nL) (gpop + (static_cast<size_t>(m) * nLpad + c1 * C + seg * 64))[ln] = v;
-                }
             }
-            const int c2 = c - 2;
-            if (chains && c2 >= 0 && c2 < nchunk) {
-                const int popo = kOffPop + (c2 % 4) * kCD;
-                const int newo = kOffNew + (c2 & 1) * kCD;
-                double *tb = lds + kOffTrial + (c2 & 1) * kCD;
-                const uint32_t *mrt = sh.mring[c2 % 4];
+            // rebuild of chunk c-1 (generation pg)
+            const int cb = c - 1;
+            if (rebuild && cb >= 0 && cb < nchunk) {
+                const double *buf = lds + kOffPop + (cb % 4) * kCD;
+                const uint32_t *mr = sh.mring[cb % 4];
+#pragma unroll
+                for (int u = 0; u < kT; u++)
+                    if (bm[u]) {
+                        const int cm = min(mrow[u], nP - 1);
+                        const double fw[2] = {sh.f_trial[pg & 1][cm][0], sh.f_trial[pg & 1][cm][1]};
+                        rb[u] = de_pair(buf, mr, mob[u], odb[u], mrow[u], cb, fw, sh.y0r);
+                    }
+            }
+            // trial of chunk c-3 (or, for the initial evaluation, the stored
+            // genomes) into the trial buffer in MFMA B order
+            const int ct = c - 3;
+            if (chains && ct >= 0 && ct < nchunk) {
+                const double *buf = lds + kOffPop + (ct % 4) * kCD;
+                double *tb = lds + kOffTrial + (ct & 1) * kCD;
                 if (kind == kPassInit) {
 #pragma unroll
-                    for (int j = 0; j < kTasks; j++) {
-                        const int t = wk + kWorkers * j, m = t / kSegs, seg = t % kSegs;
-                        if (m >= nP) continue;
-                        const int sl = seg * 64 + ln, s = c2 * C + sl;
-                        tb[trial_index<NM>(m, sl)] = s < nL ? lds[popo + m * C + sl] : 0.0;
+                    for (int u = 0; u < kT; u++) {
+                        const double2 x = *reinterpret_cast<const double2 *>(buf + mrow[u] * C + p);
+                        const int sp = ct * C + p;
+                        tb[td[u][0]] = sp < nL ? x.x : 0.0;
+                        tb[td[u][1]] = sp + 1 < nL ? x.y : 0.0;
                     }
                 } else {
-                    double v4[kTasks][4], Fv[kTasks];
-                    uint32_t w4[kTasks];
+                    const uint32_t *mr = sh.mring[ct % 4];
 #pragma unroll
-                    for (int j = 0; j < kTasks; j++) {
-                        const uint32_t ds = __builtin_amdgcn_readlane(dtab, j);
-                        const int t = wk + kWorkers * j, m = min(t / kSegs, nP - 1), seg = t % kSegs;
-                        const int sl = seg * 64 + ln, s = c2 * C + sl;
-                        v4[j][0] = lds[(((ds >> 16) & 1) ? newo : popo) + m * C + sl];
-#pragma unroll
-                        for (int k = 0; k < 3; k++)
-                            v4[j][k + 1] = lds[(((ds >> (17 + k)) & 1) ? newo : popo) +
-                                               static_cast<int>((ds >> (5 * k)) & 31) * C + sl];
-                        w4[j] = mrt[(m * 2 + static_cast<int>((ds >> 15) & 1)) * kWpc + (sl >> 5)];
-                        Fv[j] = sh.f_trial[g & 1][m][(n2 && s >= nG) ? 1 : 0];
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int j = 0; j < kTasks; j++) {
-                        const uint32_t ds = __builtin_amdgcn_readlane(dtab, j);
-                        if (!((ds >> 20) & 1)) continue;
-                        const int t = wk + kWorkers * j, m = t / kSegs, seg = t % kSegs;
-                        const int sl = seg * 64 + ln, s = c2 * C + sl;
-                        double v = de_value((w4[j] >> (s & 31)) & 1, Fv[j], v4[j][0], v4[j][1], v4[j][2], v4[j][3],
-                                            aneg);
-                        if (n2 && s == nG) {
-                            if (c2 == 0) {
-                                const int wb = (m * 2 + static_cast<int>((ds >> 15) & 1)) * kWpc;
-                                double x[4];
-                                x[0] = lds[(((ds >> 16) & 1) ? newo : popo) + m * C];
-                                for (int k = 0; k < 3; k++)
-                                    x[k + 1] = lds[(((ds >> (17 + k)) & 1) ? newo : popo) +
-                                                   static_cast<int>((ds >> (5 * k)) & 31) * C];
-                                v = de_value(mrt[wb] & 1, sh.f_trial[g & 1][m][0], x[0], x[1], x[2], x[3], aneg);
-                            } else {
-                                v = sh.y0t[m];
-                            }
-                        }
-                        if (s == 0) sh.y0t[m] = v;
-                        if (s >= nL) v = 0.0;
-                        tb[trial_index<NM>(m, sl)] = v;
+                    for (int u = 0; u < kT; u++) {
+                        const int cm = min(mrow[u], nP - 1);
+                        const double fw[2] = {sh.f_trial[g & 1][cm][0], sh.f_trial[g & 1][cm][1]};
+                        const double2 v = de_pair(buf, mr, mo[u], od[u], mrow[u], ct, fw, sh.y0t);
+                        tb[td[u][0]] = v.x;
+                        tb[td[u][1]] = v.y;
                     }
                 }
             }
-            // load(c) lands: the chunk's genomes and crossover words go from
-            // registers into LDS for the next step's rebuild
+            // load(c) lands
             if (ld) {
 #pragma unroll
-                for (int u = 0; u < kRows; u++)
-                    *reinterpret_cast<double2 *>(lds + kOffPop + (c % 4) * kCD + (wk + kWorkers * u) * 128 + 2 * ln) =
-                        pv[u];
-                if (wk < kMaskWords / kWave) sh.mring[c % 4][wk * kWave + ln] = mw;
+                for (int u = 0; u < kT; u++)
+                    *reinterpret_cast<double2 *>(lds + kOffPop + (c % 4) * kCD + mrow[u] * C + p) = pv[u];
+                if (wk < kMaskWords / kWave) sh.mring[c % 4][wk * kWave + lane] = mw;
             }
             step_barrier();
         }

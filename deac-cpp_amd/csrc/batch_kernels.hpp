@@ -15,7 +15,7 @@ constexpr int kBatchThreads = kBatchWaves * 64;
 constexpr int kBatchMaskLanes = 192;        // 3 mask wavefronts
 constexpr int kBatchChunkDoubles = 2048;    // members (padded) x genome positions per chunk
 constexpr int kBatchMaxP = 32;
-constexpr int kBatchRegionDoubles = 8 * kBatchChunkDoubles;  // 4 load + 2 update + 2 trial buffers
+constexpr int kBatchRegionDoubles = 6 * kBatchChunkDoubles;  // 4 population + 2 trial buffers
 constexpr int kBatchMaxRem = 4;             // kernel rows past the last full tile run on 4x4x4 MFMAs
 
 // Everything one workgroup needs about its reconstruction.  Model arrays may
