@@ -81,7 +81,8 @@ struct BatchShared {
     uint64_t walk_state[4];                  // walk position between passes
     double fitness[kBatchMaxP];
     double cr_old[kBatchMaxP][2], f_old[kBatchMaxP][2];
-    double cr_trial[2][kBatchMaxP][2], f_trial[2][kBatchMaxP][2];
+    double cr_trial[2][kBatchMaxP][2];
+    alignas(16) double f_trial[2][kBatchMaxP][2];   // read as pairs
     double y0t[kBatchMaxP], y0r[kBatchMaxP];  // trial value at position 0 (zero-frequency match)
     int accepted[3][kBatchMaxP];
     int flag;                                 // 0 evolve, 1 stop test fired, 2 end of the launch
@@ -109,6 +110,17 @@ __device__ __forceinline__ void step_barrier()
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
+}
+
+// A pointer held in scalar registers: its two halves made wave-uniform, so
+// the compiler keeps it rather than re-reading the descriptor in loops.
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+    return reinterpret_cast<T *>((static_cast<uint64_t>(hi) << 32) | lo);
 }
 
 // Two adjacent doubles (16-byte aligned) from global memory, one dwordx4 load.
@@ -471,6 +483,8 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         const int nL = __builtin_amdgcn_readfirstlane(d.L), nLpad = __builtin_amdgcn_readfirstlane(d.Lpad);
         const int n2 = __builtin_amdgcn_readfirstlane(d.nsign) == 2 ? 1 : 0;
         const int nLw = __builtin_amdgcn_readfirstlane(d.Lw);
+        double *const popp = uniform_ptr(d.pop);
+        uint32_t *const maskp = uniform_ptr(d.masks);
         // fabs of a DE value, or not (allow_negative): an and of the high word
         const uint32_t absmask = __builtin_amdgcn_readfirstlane(d.allow_neg) ? 0xffffffffu : 0x7fffffffu;
         constexpr int kWpc = C / 32;  // mask words per member and chunk
@@ -484,16 +498,20 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         // Per-row constants of the pass.  Padded members (m >= P) compute on
         // the last member's data: their trial columns only feed their own
         // (discarded) model values.
-        int mrow[kT], od[kT][3], odb[kT][3], mo[kT], mob[kT], td[kT][2];
+        // Offset (doubles) of the lane's pair in a chunk buffer, row 0; row u
+        // adds 1024.  Member of row u, and the member whose data it uses.
+        const int own = wk * 128 + 2 * lane;
+        auto mrow = [&](int u) { return (own + u * 1024) / C; };
+        auto fm = [&](int u) { return min(mrow(u), nP - 1); };
+        // trial-buffer index of position p of row 0 (the others follow from it)
+        const int td0 = trial_index<NM>(mrow(0), p);
+        int od[kT][3], odb[kT][3], mo[kT], mob[kT];
         uint32_t poff[kT];
         uint64_t bm[kT];  // lanes whose member was accepted in generation pg
 #pragma unroll
         for (int u = 0; u < kT; u++) {
-            const int m = ((wk + kWorkers * u) * 128 + 2 * lane) / C, cm = min(m, nP - 1);
-            mrow[u] = m;
+            const int m = mrow(u), cm = fm(u);
             poff[u] = static_cast<uint32_t>(cm * nLpad + p) * 8u;
-            td[u][0] = trial_index<NM>(m, p);
-            td[u][1] = trial_index<NM>(m, p + 1);
             od[u][0] = od[u][1] = od[u][2] = m * C + p;
             odb[u][0] = odb[u][1] = odb[u][2] = m * C + p;
             mo[u] = mob[u] = 0;
@@ -525,97 +543,120 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             moff = static_cast<uint32_t>(((slot * Ppad + m) * 2 + cand) * nLw + w) * 4u;
         }
 
-        // DE values at positions p, p+1 of chunk cc for row u: operands
-        // (parent, r1, r2, r3) from chunk buffer buf, crossover word mw0,
-        // jDE weights fw (deac.cpp:1261-1311).  Chunks holding position 0,
-        // position G (the zero frequency of the negative half, which repeats
-        // position 0: deac.cpp:1305-1308) or padding take the per-position
-        // path; elsewhere the sign of the frequency is uniform.
-        auto de_pair = [&](const double *buf, const uint32_t *mr, int mw_idx, const int (&o)[3], int m, int cc,
-                           const double (&fw)[2], double *y0) -> double2 {
-            const double2 x0 = *reinterpret_cast<const double2 *>(buf + m * C + p);
-            const double2 x1 = *reinterpret_cast<const double2 *>(buf + o[0]);
-            const double2 x2 = *reinterpret_cast<const double2 *>(buf + o[1]);
-            const double2 x3 = *reinterpret_cast<const double2 *>(buf + o[2]);
-            const uint32_t w = mr[mw_idx] >> (p & 31);
-            const bool slow = cc == 0 || (n2 && cc == nG / C) || (cc + 1) * C > nL;
+        // DE values at positions p, p+1 of chunk cc for one row: operands
+        // (parent, r1, r2, r3) from chunk buffer buf, crossover word mr[wi],
+        // jDE weight F by frequency sign (deac.cpp:1261-1311).  Chunks
+        // holding position 0, position G (the zero frequency of the negative
+        // half, which repeats position 0: deac.cpp:1305-1308) or padding take
+        // the per-position path; elsewhere the sign is uniform and one F
+        // serves the chunk.  Operands of both rows are read before either is
+        // computed (one LDS round trip per stage).
+        struct Ops {
+            double2 x0, x1, x2, x3;
+            double f0, f1;  // F (fast path); F+, F- (per-position path)
+            uint32_t w;
+        };
+        auto load_ops = [&](const double *buf, const uint32_t *mr, int wi, const int (&o)[3], int u) -> Ops {
+            Ops a;
+            a.x0 = *reinterpret_cast<const double2 *>(buf + own + u * 1024);
+            a.x1 = *reinterpret_cast<const double2 *>(buf + o[0]);
+            a.x2 = *reinterpret_cast<const double2 *>(buf + o[1]);
+            a.x3 = *reinterpret_cast<const double2 *>(buf + o[2]);
+            a.w = mr[wi] >> (p & 31);
+            return a;
+        };
+        auto de_fast = [&](const Ops &a) -> double2 {
             double2 v;
-            if (!slow) {
-                const double F = (n2 && cc * C >= nG) ? fw[1] : fw[0];
-                v.x = (w & 1) ? de_abs(fma(F, x2.x - x3.x, x1.x), absmask) : x0.x;
-                v.y = (w & 2) ? de_abs(fma(F, x2.y - x3.y, x1.y), absmask) : x0.y;
-            } else {
-                double r[2];
-#pragma unroll
-                for (int e = 0; e < 2; e++) {
-                    const int sp = cc * C + p + e;
-                    const double F = (n2 && sp >= nG) ? fw[1] : fw[0];
-                    const double a0 = e ? x0.y : x0.x, a1 = e ? x1.y : x1.x, a2 = e ? x2.y : x2.x,
-                                 a3 = e ? x3.y : x3.x;
-                    double t = ((w >> e) & 1) ? de_abs(fma(F, a2 - a3, a1), absmask) : a0;
-                    if (n2 && sp == nG) {
-                        if (cc == 0) {
-                            // position 0 of the same chunk, positive frequencies
-                            const uint32_t w0 = mr[mw_idx - (p >> 5)];
-                            t = (w0 & 1) ? de_abs(fma(fw[0], buf[o[1] - p] - buf[o[2] - p], buf[o[0] - p]), absmask)
-                                         : buf[m * C];
-                        } else {
-                            t = y0[m];
-                        }
-                    }
-                    if (sp == 0) y0[m] = t;
-                    if (sp >= nL) t = 0.0;
-                    r[e] = t;
-                }
-                v.x = r[0];
-                v.y = r[1];
-            }
+            v.x = (a.w & 1) ? de_abs(fma(a.f0, a.x2.x - a.x3.x, a.x1.x), absmask) : a.x0.x;
+            v.y = (a.w & 2) ? de_abs(fma(a.f0, a.x2.y - a.x3.y, a.x1.y), absmask) : a.x0.y;
             return v;
+        };
+        auto de_slow = [&](const Ops &a, const double *buf, const uint32_t *mr, int wi, const int (&o)[3], int u,
+                           int cc, double *y0) -> double2 {
+            const int m = mrow(u);
+            double r[2];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int sp = cc * C + p + e;
+                const double F = (n2 && sp >= nG) ? a.f1 : a.f0;
+                const double a0 = e ? a.x0.y : a.x0.x, a1 = e ? a.x1.y : a.x1.x, a2 = e ? a.x2.y : a.x2.x,
+                             a3 = e ? a.x3.y : a.x3.x;
+                double t = ((a.w >> e) & 1) ? de_abs(fma(F, a2 - a3, a1), absmask) : a0;
+                if (n2 && sp == nG) {
+                    if (cc == 0) {
+                        // position 0 of the same chunk, positive frequencies
+                        const uint32_t w0 = mr[wi - (p >> 5)];
+                        t = (w0 & 1) ? de_abs(fma(a.f0, buf[o[1] - p] - buf[o[2] - p], buf[o[0] - p]), absmask)
+                                     : buf[m * C];
+                    } else {
+                        t = y0[m];
+                    }
+                }
+                if (sp == 0) y0[m] = t;
+                if (sp >= nL) t = 0.0;
+                r[e] = t;
+            }
+            double2 v;
+            v.x = r[0];
+            v.y = r[1];
+            return v;
+        };
+        // One DE stage over both rows: chunk cc of buffer buf, crossover words
+        // mr (+ wi), donors o, weights of generation gen, position-0 values y0.
+        auto de_stage = [&](const double *buf, const uint32_t *mr, const int (&wi)[kT], const int (&o)[kT][3], long gen,
+                            int cc, double *y0, double2 (&v)[kT]) {
+            Ops a[kT];
+            if (!(cc == 0 || (n2 && cc == nG / C) || (cc + 1) * C > nL)) {
+                const int neg = (n2 && cc * C >= nG) ? 1 : 0;
+#pragma unroll
+                for (int u = 0; u < kT; u++) {
+                    a[u] = load_ops(buf, mr, wi[u], o[u], u);
+                    a[u].f0 = sh.f_trial[gen & 1][fm(u)][neg];
+                }
+#pragma unroll
+                for (int u = 0; u < kT; u++) v[u] = de_fast(a[u]);
+            } else {
+#pragma unroll
+                for (int u = 0; u < kT; u++) {
+                    a[u] = load_ops(buf, mr, wi[u], o[u], u);
+                    a[u].f0 = sh.f_trial[gen & 1][fm(u)][0];
+                    a[u].f1 = sh.f_trial[gen & 1][fm(u)][1];
+                }
+#pragma unroll
+                for (int u = 0; u < kT; u++) v[u] = de_slow(a[u], buf, mr, wi[u], o[u], u, cc, y0);
+            }
         };
 
         // load(c) goes to registers at the top of step c and into LDS at its
         // bottom: plain loads, so no LDS access in between waits for them (an
         // LDS DMA would make every LDS read wait for the copy in flight).
         // rebuild(c-1) computes the genomes accepted in pg from the chunk as
-        // loaded; they replace it in place (LDS and HBM) one step later, once
-        // every rebuild has read the old values; trial(c-3) then reads this
-        // generation's genomes from one buffer.
+        // loaded; they replace it in place (LDS and HBM) at the top of the
+        // next step, once every rebuild has read the old values; trial(c-3)
+        // then reads this generation's genomes from one buffer.
         double2 pv[kT], rb[kT];
         uint32_t mw = 0;
         for (int c = 0; c < nsteps; c++) {
             const bool ld = kind != kPassPre && c < nchunk;
             if (ld) {
-                const gchar *pb = (const gchar *)d.pop + static_cast<size_t>(c) * C * 8;
+                const gchar *pb = (const gchar *)popp + static_cast<size_t>(c) * C * 8;
 #pragma unroll
                 for (int u = 0; u < kT; u++) pv[u] = load2((const gcdouble *)(pb + poff[u]));
                 if (wk < kMaskWords / kWave)
-                    mw = *(const guint *)((const gchar *)d.masks + static_cast<size_t>(c) * kWpc * 4 + moff);
+                    mw = *(const guint *)((const gchar *)maskp + static_cast<size_t>(c) * kWpc * 4 + moff);
             }
             // rebuild write-back of chunk c-2
             const int cw = c - 2;
             if (rebuild && cw >= 0 && cw < nchunk) {
                 double *buf = lds + kOffPop + (cw % 4) * kCD;
-                gchar *gb = (gchar *)d.pop + static_cast<size_t>(cw) * C * 8;
+                gchar *gb = (gchar *)popp + static_cast<size_t>(cw) * C * 8;
 #pragma unroll
                 for (int u = 0; u < kT; u++)
                     if (bm[u] && ((bm[u] >> lane) & 1)) {
-                        *reinterpret_cast<double2 *>(buf + mrow[u] * C + p) = rb[u];
+                        *reinterpret_cast<double2 *>(buf + own + u * 1024) = rb[u];
                         gdouble *q = (gdouble *)(gb + poff[u]);
                         q[0] = rb[u].x;
                         q[1] = rb[u].y;
-                    }
-            }
-            // rebuild of chunk c-1 (generation pg)
-            const int cb = c - 1;
-            if (rebuild && cb >= 0 && cb < nchunk) {
-                const double *buf = lds + kOffPop + (cb % 4) * kCD;
-                const uint32_t *mr = sh.mring[cb % 4];
-#pragma unroll
-                for (int u = 0; u < kT; u++)
-                    if (bm[u]) {
-                        const int cm = min(mrow[u], nP - 1);
-                        const double fw[2] = {sh.f_trial[pg & 1][cm][0], sh.f_trial[pg & 1][cm][1]};
-                        rb[u] = de_pair(buf, mr, mob[u], odb[u], mrow[u], cb, fw, sh.y0r);
                     }
             }
             // trial of chunk c-3 (or, for the initial evaluation, the stored
@@ -624,31 +665,35 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             if (chains && ct >= 0 && ct < nchunk) {
                 const double *buf = lds + kOffPop + (ct % 4) * kCD;
                 double *tb = lds + kOffTrial + (ct & 1) * kCD;
+                double2 v[kT];
                 if (kind == kPassInit) {
 #pragma unroll
                     for (int u = 0; u < kT; u++) {
-                        const double2 x = *reinterpret_cast<const double2 *>(buf + mrow[u] * C + p);
+                        const double2 x = *reinterpret_cast<const double2 *>(buf + own + u * 1024);
                         const int sp = ct * C + p;
-                        tb[td[u][0]] = sp < nL ? x.x : 0.0;
-                        tb[td[u][1]] = sp + 1 < nL ? x.y : 0.0;
+                        v[u].x = sp < nL ? x.x : 0.0;
+                        v[u].y = sp + 1 < nL ? x.y : 0.0;
                     }
                 } else {
-                    const uint32_t *mr = sh.mring[ct % 4];
+                    de_stage(buf, sh.mring[ct % 4], mo, od, g, ct, sh.y0t, v);
+                }
 #pragma unroll
-                    for (int u = 0; u < kT; u++) {
-                        const int cm = min(mrow[u], nP - 1);
-                        const double fw[2] = {sh.f_trial[g & 1][cm][0], sh.f_trial[g & 1][cm][1]};
-                        const double2 v = de_pair(buf, mr, mo[u], od[u], mrow[u], ct, fw, sh.y0t);
-                        tb[td[u][0]] = v.x;
-                        tb[td[u][1]] = v.y;
-                    }
+                for (int u = 0; u < kT; u++) {
+                    const int t0 = NM == 2 ? td0 + 64 * u : td0 ^ (8 * u);  // position p
+                    tb[t0] = v[u].x;
+                    tb[(t0 ^ 1) + 16] = v[u].y;                               // position p + 1
                 }
             }
+            // rebuild of chunk c-1 (generation pg), both rows when either
+            // holds an accepted member
+            const int cb = c - 1;
+            if (rebuild && cb >= 0 && cb < nchunk && (bm[0] | bm[kT - 1]))
+                de_stage(lds + kOffPop + (cb % 4) * kCD, sh.mring[cb % 4], mob, odb, pg, cb, sh.y0r, rb);
             // load(c) lands
             if (ld) {
 #pragma unroll
                 for (int u = 0; u < kT; u++)
-                    *reinterpret_cast<double2 *>(lds + kOffPop + (c % 4) * kCD + mrow[u] * C + p) = pv[u];
+                    *reinterpret_cast<double2 *>(lds + kOffPop + (c % 4) * kCD + own + u * 1024) = pv[u];
                 if (wk < kMaskWords / kWave) sh.mring[c % 4][wk * kWave + lane] = mw;
             }
             step_barrier();
