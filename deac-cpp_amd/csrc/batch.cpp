@@ -177,16 +177,18 @@ deac_status upload_model(deac_batch *h, BatchModel &b)
                     if (tau < T && slot < L) v = m.kernel[static_cast<size_t>(tau) * L + slot];
                     kt[((static_cast<size_t>(t) * nkb2 + kb2) * 64 + l) * 2 + e] = v;
                 }
-    // Rows ntt*16 .. T-1 in v_mfma_f64_4x4x4_4b A order: lane l holds row
-    // l&3 at position 4*kb + (l>>4) (the same for the four blocks).
-    std::vector<double> kr(static_cast<size_t>(nkb2) * 64 * 2, -0.0);
+    // Rows ntt*16 .. T-1 for v_mfma_f64_4x4x4_4b: per 8 positions, 16 pairs
+    // (row r + 4 * position k of a 4-position block; the pair holds the two
+    // blocks).  Lane l reads entry (l & 3) + 4 (l >> 4): the A operand, the
+    // same for the four 4x4 blocks.
+    std::vector<double> kr(static_cast<size_t>(nkb2) * 16 * 2, -0.0);
     for (int kb2 = 0; kb2 < nkb2; kb2++)
-        for (int l = 0; l < 64; l++)
+        for (int idx = 0; idx < 16; idx++)
             for (int e = 0; e < 2; e++) {
-                const int r = l & 3, slot = (2 * kb2 + e) * 4 + (l >> 4);
+                const int r = idx & 3, slot = (2 * kb2 + e) * 4 + (idx >> 2);
                 double v = -0.0;
                 if (slot < L) v = r < b.nrem ? m.kernel[static_cast<size_t>(b.ntt * 16 + r) * L + slot] : 0.0;
-                kr[(static_cast<size_t>(kb2) * 64 + l) * 2 + e] = v;
+                kr[(static_cast<size_t>(kb2) * 16 + idx) * 2 + e] = v;
             }
     // Jump tables: T^L by rows for the walk, T^(k*Lsub) by nibbles for the
     // mask sub-streams k = 1..S-1.

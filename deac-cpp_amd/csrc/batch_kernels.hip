@@ -35,6 +35,8 @@
 // per accepted genome, and the trial vectors read a single chunk buffer.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "batch_kernels.hpp"
 #include "device_common.hpp"
 
@@ -93,13 +95,14 @@ struct BatchShared {
 
 // LDS index of trial value (member m, chunk position sl) in the order the
 // MFMA B operand reads it: 4-position blocks of 16 members, lane
-// (sl & 3) * 16 + (m & 15), with the member XOR-swizzled by sl & 15 so that
-// both the writers (one member, 64 consecutive positions) and the readers
-// (one block) touch 16 distinct bank pairs per 16 lanes.
+// (sl & 3) * 16 + (m & 15), with the member XOR-swizzled by (sl >> 1) & 15
+// so that both the writers (16 lanes: one member, positions sl, sl + 2, ..,
+// sl + 30) and the readers (16 lanes: one position of a block) touch 16
+// distinct bank pairs.  Position sl + 1 of an even sl lies 16 further on.
 template <int NM>
 __device__ __forceinline__ int trial_index(int m, int sl)
 {
-    return ((sl >> 2) * NM + (m >> 4)) * 64 + (sl & 3) * 16 + ((m & 15) ^ (sl & 15));
+    return ((sl >> 2) * NM + (m >> 4)) * 64 + (sl & 3) * 16 + ((m & 15) ^ ((sl >> 1) & 15));
 }
 
 // Workgroup barrier of the chunk pipeline: LDS traffic retired, global loads
@@ -228,14 +231,66 @@ __device__ void walk_item(const BatchDesc &d, BatchShared &sh, uint64_t (&s)[4],
 // two candidate bits: the member's CR if its trial is rejected in the
 // generation running now, and the trial's CR if it is accepted (a redrawn CR
 // makes both the same).  The next generation picks one by the selection.
+//
+// A draw x passes the candidate threshold T = ceil(CR 2^53) when
+// (x >> 11) < T, i.e. x <= U = T 2^11 - 1 (mod 2^64) for 1 <= T <= 2^53; a
+// zero threshold never passes, which the group mask k (low 8 bits clear)
+// enforces after each group of 8 draws.
 struct MaskGen {
-    uint64_t s[4];
-    uint64_t ta, tb;      // thresholds of the two candidates at the current position
-    uint64_t na, nb;      // thresholds after the sign change (negative frequencies)
+    uint64_t s[4];        // xoshiro256+ state
+    uint64_t ua, ub;      // U of the two candidates at the current position
+    uint64_t una, unb;    // U after the sign change (negative frequencies)
+    uint32_t ka, kb, kna, knb;
     uint32_t wa, wb;
     int pos, end, flip;   // flip: position of the sign change (or past the end)
     guint *out;
 };
+
+__device__ __forceinline__ uint64_t u_of_threshold(uint64_t t) { return t * 2048u - 1u; }
+__device__ __forceinline__ uint32_t k_of_threshold(uint64_t t) { return t ? 0xffffffffu : 0xffffff00u; }
+
+// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96).  Written as asm so the
+// compiler keeps the three-input form instead of re-associating the xors of
+// consecutive generator steps into two-input ones.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+__device__ __forceinline__ uint64_t xor3_64(uint64_t a, uint64_t b, uint64_t c)
+{
+    const uint32_t lo = xor3(static_cast<uint32_t>(a), static_cast<uint32_t>(b), static_cast<uint32_t>(c));
+    const uint32_t hi = xor3(static_cast<uint32_t>(a >> 32), static_cast<uint32_t>(b >> 32), static_cast<uint32_t>(c >> 32));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// One xoshiro256+ step (rng.hpp:25-39) written over the old state:
+// s2' = s2^s0^t, s1' = s1^s2^s0, s0' = s0^s3^s1, s3' = rotl(s3^s1, 45).
+// Returns s0 + s3 of the old state.
+__device__ __forceinline__ uint64_t xnext_mask(uint64_t (&s)[4])
+{
+    const uint64_t s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
+    const uint64_t out = s0 + s3;
+    const uint64_t e = s3 ^ s1;
+    s[2] = xor3_64(s2, s0, s1 << 17);
+    s[1] = xor3_64(s1, s2, s0);
+    s[0] = s0 ^ e;
+    const uint32_t lo = static_cast<uint32_t>(e), hi = static_cast<uint32_t>(e >> 32);
+    s[3] = (static_cast<uint64_t>(__builtin_amdgcn_alignbit(lo, hi, 19)) << 32) | __builtin_amdgcn_alignbit(hi, lo, 19);
+    return out;
+}
+
+// w = 2 w + [lane bit of mask]: one v_addc_co_u32 with the compare's lane mask
+// as carry-in.
+__device__ __forceinline__ uint32_t shift_in(uint32_t w, uint64_t lanemask)
+{
+    uint32_t r;
+    uint64_t carry;
+    asm("v_addc_co_u32 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry) : "v"(w), "s"(lanemask));
+    return r;
+}
 
 template <int NM>
 __device__ void mask_setup(MaskGen &mg, const BatchDesc &d, const BatchShared &sh, long mgen, bool evolving, long g,
@@ -246,7 +301,9 @@ __device__ void mask_setup(MaskGen &mg, const BatchDesc &d, const BatchShared &s
     mg.pos = mg.end = 0;
     mg.wa = mg.wb = 0;
     mg.flip = 0;
-    mg.ta = mg.tb = mg.na = mg.nb = 0;
+    mg.ua = mg.ub = mg.una = mg.unb = 0;
+    mg.ka = mg.kb = mg.kna = mg.knb = 0;
+    for (int w = 0; w < 4; w++) mg.s[w] = 0;
     mg.out = (guint *)d.masks;
     if (m >= d.P) return;
     const WalkRec &r = sh.rec[mgen & 3][m];
@@ -267,11 +324,15 @@ __device__ void mask_setup(MaskGen &mg, const BatchDesc &d, const BatchShared &s
     mg.pos = min(k * d.Lsub, d.L);
     mg.end = min(mg.pos + d.Lsub, d.L);
     mg.flip = d.nsign == 2 ? d.G : d.L;
-    const bool neg = mg.pos >= mg.flip;
-    mg.ta = neg ? thr[0][1] : thr[0][0];
-    mg.tb = neg ? thr[1][1] : thr[1][0];
-    mg.na = thr[0][1];
-    mg.nb = thr[1][1];
+    const int sg = mg.pos >= mg.flip ? 1 : 0;
+    mg.ua = u_of_threshold(thr[0][sg]);
+    mg.ub = u_of_threshold(thr[1][sg]);
+    mg.ka = k_of_threshold(thr[0][sg]);
+    mg.kb = k_of_threshold(thr[1][sg]);
+    mg.una = u_of_threshold(thr[0][1]);
+    mg.unb = u_of_threshold(thr[1][1]);
+    mg.kna = k_of_threshold(thr[0][1]);
+    mg.knb = k_of_threshold(thr[1][1]);
     mg.out = (guint *)d.masks + (static_cast<size_t>(mgen % 3) * Ppad + m) * 2 * d.Lw;
 }
 
@@ -288,25 +349,31 @@ __device__ __forceinline__ void mask_draws(MaskGen &mg, int q, int Lw)
         const int cnt = min(8, n - u);
         if (cnt == 8 && (mg.pos >= mg.flip || mg.pos + 8 <= mg.flip)) {
             if (mg.pos >= mg.flip) {
-                mg.ta = mg.na;
-                mg.tb = mg.nb;
+                mg.ua = mg.una;
+                mg.ub = mg.unb;
+                mg.ka = mg.kna;
+                mg.kb = mg.knb;
             }
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const uint64_t r = xnext(mg.s) >> 11;
-                mg.wa = mg.wa * 2u + (r < mg.ta ? 1u : 0u);
-                mg.wb = mg.wb * 2u + (r < mg.tb ? 1u : 0u);
+                const uint64_t x = xnext_mask(mg.s);
+                mg.wa = shift_in(mg.wa, __builtin_amdgcn_ballot_w64(x <= mg.ua));
+                mg.wb = shift_in(mg.wb, __builtin_amdgcn_ballot_w64(x <= mg.ub));
             }
+            mg.wa &= mg.ka;
+            mg.wb &= mg.kb;
             mg.pos += 8;
         } else {
             for (int k = 0; k < cnt; k++) {
                 if (mg.pos == mg.flip) {
-                    mg.ta = mg.na;
-                    mg.tb = mg.nb;
+                    mg.ua = mg.una;
+                    mg.ub = mg.unb;
+                    mg.ka = mg.kna;
+                    mg.kb = mg.knb;
                 }
-                const uint64_t r = xnext(mg.s) >> 11;
-                mg.wa = mg.wa * 2u + (r < mg.ta ? 1u : 0u);
-                mg.wb = mg.wb * 2u + (r < mg.tb ? 1u : 0u);
+                const uint64_t x = xnext_mask(mg.s);
+                mg.wa = mg.wa * 2u + (x <= mg.ua ? mg.ka & 1u : 0u);
+                mg.wb = mg.wb * 2u + (x <= mg.ub ? mg.kb & 1u : 0u);
                 mg.pos++;
             }
         }
@@ -352,11 +419,11 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             tv[i] = chains && tile[i] < d.ntt;
             tload[i] = min(tile[i], max(d.ntt - 1, 0));
         }
-        int boff[4];
+        int boff[8];
         {
             const int q = lane >> 4, j = lane & 15;
 #pragma unroll
-            for (int u = 0; u < 4; u++) boff[u] = q * 16 + (j ^ ((4 * u + q) & 15));
+            for (int u = 0; u < 8; u++) boff[u] = q * 16 + (j ^ ((2 * u + (q >> 1)) & 15));
         }
         d4 acc[NT][NM];
 #pragma unroll
@@ -376,23 +443,28 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             for (int i = 0; i < NT; i++) kr[p][i] = load2(ktw[i] + static_cast<size_t>(min(p, nkb2 - 1)) * (2 * kWave) + lo);
         // Rows past the last full tile (REM): one 4x4x4 MFMA per 4 positions
         // for member tile rmt = wave % NM; waves >= NM repeat a tile's work
-        // (discarded) to keep the code straight-line.
+        // (discarded) to keep the code straight-line.  The remainder rows
+        // are stored 16 values (4 rows x 4 positions) per 4-position block,
+        // lane l reading row l & 3 at position l >> 4.
         const gcdouble *krm = (const gcdouble *)d.krem;
         const int rmt = wave % NM;
-        const uint64_t rsel = rmt ? ~UINT64_C(0) : UINT64_C(0);
+        const int lro = ((lane & 3) + 4 * (lane >> 4)) * 2;
         double accr = 0.0;
         double2 krr[KD];
         if (REM) {
 #pragma unroll
-            for (int p = 0; p < KD; p++) krr[p] = load2(krm + static_cast<size_t>(min(p, nkb2 - 1)) * (2 * kWave) + lo);
+            for (int p = 0; p < KD; p++) krr[p] = load2(krm + static_cast<size_t>(min(p, nkb2 - 1)) * 32 + lro);
         }
         // Steps 0-3 fill the pipeline, steps 4 .. nchunk+3 run the chains of
         // chunk c-4; the hot loop has no branch, so the waits for the K
         // prefetch are exact across its iterations.
         const int nwork = chains ? nchunk : 0;
         for (int c = 0; c < min(4, nsteps); c++) step_barrier();
-        for (int cm = 0; cm < nwork; cm++) {
-            {
+        // One copy of the loop per remainder member tile RT, so the 4x4x4
+        // MFMA takes its B register directly.
+        auto chain_loop = [&](auto rt) {
+            constexpr int RT = decltype(rt)::value;
+            for (int cm = 0; cm < nwork; cm++) {
                 const double *tb = lds + kOffTrial + (cm & 1) * kCD;
                 // B operands one block pair ahead of the MFMAs that use them
                 double bc[2][NM], bn[2][NM];
@@ -409,7 +481,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                         for (int e = 0; e < 2; e++) {
                             const int kbl = 2 * (jj + 1) + e;
 #pragma unroll
-                            for (int mt = 0; mt < NM; mt++) bn[e][mt] = tb[(kbl * NM + mt) * kWave + boff[kbl & 3]];
+                            for (int mt = 0; mt < NM; mt++) bn[e][mt] = tb[(kbl * NM + mt) * kWave + boff[kbl & 7]];
                         }
                     }
 #pragma unroll
@@ -422,14 +494,8 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                                 acc[i][mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bc[e][mt], acc[i][mt], 0, 0, 0);
                         }
                         if (REM) {
-                            // B register of member tile rmt by a bit select:
-                            // no branch (exact prefetch waits) and no indexed
-                            // register array (which would go through scratch)
                             const double ar = e ? krr[sl].y : krr[sl].x;
-                            const uint64_t b0 = __builtin_bit_cast(uint64_t, bc[e][0]);
-                            const uint64_t b1 = __builtin_bit_cast(uint64_t, bc[e][NM - 1]);
-                            const double br = __builtin_bit_cast(double, (b1 & rsel) | (b0 & ~rsel));
-                            accr = __builtin_amdgcn_mfma_f64_4x4x4f64(ar, br, accr, 0, 0, 0);
+                            accr = __builtin_amdgcn_mfma_f64_4x4x4f64(ar, bc[e][RT], accr, 0, 0, 0);
                         }
                     }
 #pragma unroll
@@ -439,12 +505,14 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     const int gn = min(g2 + KD, nkb2 - 1);
 #pragma unroll
                     for (int i = 0; i < NT; i++) kr[sl][i] = load2(ktw[i] + static_cast<size_t>(gn) * (2 * kWave) + lo);
-                    if (REM) krr[sl] = load2(krm + static_cast<size_t>(gn) * (2 * kWave) + lo);
+                    if (REM) krr[sl] = load2(krm + static_cast<size_t>(gn) * 32 + lro);
                     __builtin_amdgcn_sched_barrier(0);  // issue the refill here, a full step ahead of its use
                 }
+                step_barrier();
             }
-            step_barrier();
-        }
+        };
+        if (!REM || rmt == 0) chain_loop(std::integral_constant<int, 0>{});
+        else chain_loop(std::integral_constant<int, NM - 1>{});
         for (int c = min(4, nsteps) + nwork; c < nsteps; c++) step_barrier();
         __syncthreads();
         // E[t][m] into LDS (the chunk buffers are free now): lane l holds
@@ -681,7 +749,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                 for (int u = 0; u < kT; u++) {
                     const int t0 = NM == 2 ? td0 + 64 * u : td0 ^ (8 * u);  // position p
                     tb[t0] = v[u].x;
-                    tb[(t0 ^ 1) + 16] = v[u].y;                               // position p + 1
+                    tb[t0 + 16] = v[u].y;                                     // position p + 1
                 }
             }
             // rebuild of chunk c-1 (generation pg), both rows when either
