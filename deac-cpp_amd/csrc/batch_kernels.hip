@@ -135,6 +135,17 @@ __device__ __forceinline__ double2 load2(const gcdouble *p)
     return v;
 }
 
+// The same, marked non-temporal: the population streams through once per
+// generation and should not displace the kernel tiles every instance
+// re-reads from L2.
+__device__ __forceinline__ double2 load2_stream(const gcdouble *p)
+{
+    double2 v;
+    v.x = __builtin_nontemporal_load(p);
+    v.y = __builtin_nontemporal_load(p + 1);
+    return v;
+}
+
 // u < cr for u = (x >> 11) * 2^-53  <=>  (x >> 11) < ceil(cr * 2^53).
 __device__ __forceinline__ uint64_t cr_threshold(double cr)
 {
@@ -709,7 +720,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             if (ld) {
                 const gchar *pb = (const gchar *)popp + static_cast<size_t>(c) * C * 8;
 #pragma unroll
-                for (int u = 0; u < kT; u++) pv[u] = load2((const gcdouble *)(pb + poff[u]));
+                for (int u = 0; u < kT; u++) pv[u] = load2_stream((const gcdouble *)(pb + poff[u]));
                 if (wk < kMaskWords / kWave)
                     mw = *(const guint *)((const gchar *)maskp + static_cast<size_t>(c) * kWpc * 4 + moff);
             }
@@ -723,8 +734,8 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     if (bm[u] && ((bm[u] >> lane) & 1)) {
                         *reinterpret_cast<double2 *>(buf + own + u * 1024) = rb[u];
                         gdouble *q = (gdouble *)(gb + poff[u]);
-                        q[0] = rb[u].x;
-                        q[1] = rb[u].y;
+                        __builtin_nontemporal_store(rb[u].x, q);
+                        __builtin_nontemporal_store(rb[u].y, q + 1);
                     }
             }
             // trial of chunk c-3 (or, for the initial evaluation, the stored
@@ -825,6 +836,10 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         for (int k = 0; k < 3; k++) sh.accepted[k][m] = 0;
     }
     __syncthreads();
+
+    // Issue priority: the chain and worker waves set the step time; the
+    // random-stream waves, which run ahead of their consumers, fill in.
+    if (wave < kWalkWave) __builtin_amdgcn_s_setprio(1);
 
     // Prologue: the walk of the first generation whose masks are drawn.
     const long g0 = d.gen0;
