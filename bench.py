@@ -182,7 +182,7 @@ def main():
                      "traffic_note": "HBM bytes per launch (one launch = the timed generations, up to the "
                                      "4096-generation slice) from profiles/batch_kernel_seed256_hbm_counters.md; "
                                      "algorithmic bytes per launch: " + str(alg_launch),
-                     "kernel": "batch_kernel<1,2>", "flop_per_genome_eval": flop_per_eval,
+                     "kernel": "batch_kernel<1,2,1>", "flop_per_genome_eval": flop_per_eval,
                      "kernel_ms_per_generation": prof["evolve_ms"] / args.steps,
                      "hbm_algorithmic_GBps": evals * 2 * G * 8 / kern_s / 1e9},
     }
