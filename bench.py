@@ -20,6 +20,11 @@ T=65, seed 1) is timed beside it, and its ratio reported.
 (4*G*T flops per genome evaluation); `cpu_baseline` times the reference CPU
 executable built from source (oracle/_ref, one core) on a bounded sample of
 the plain workload.
+
+`config.uneven_batch` times the uneven batch (tests/golden/make_uneven.py:
+64 wavevectors x 16 seeds with mixed T, G, omega_max, model and
+zero-temperature kernels, population 16) for a bounded number of generations
+in one deac_batch_run, as instance-generations/s and sustained FP64 TFLOP/s.
 """
 import argparse
 import json
@@ -32,6 +37,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "deac-cpp_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
 
 METRIC = "genome-evals/sec at G=4096,T=65: batched mixed-precision vs plain FP64"
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 matrix (and vector), dense
@@ -89,6 +95,32 @@ def traffic_per_generation():
     return float(m.group(1)) if m else None
 
 
+def uneven_batch(da, gens):
+    """The uneven batch for `gens` timed generations (after one warm-up run)."""
+    import make_uneven as mu
+    insts = mu.instances()
+    probs = [da.make_problem(x["tau"], x["isf"], x["err"], da.frequency_grid(x["G"], x["omega_max"]), mu.TEMPERATURE,
+                             model=x["model"], zero_temperature=x["zero_temperature"], population_size=x["P"],
+                             normalize=True, spectra_type="bfull") for x in insts]
+    seeds = [x["seed"] for x in insts]
+    flop_per_gen = sum(x["P"] * 2 * (1 if x["zero_temperature"] else 2) * x["G"] * x["T"] for x in insts)
+    with da.Batch(probs) as b:
+        b.run(seeds, 3, 0.0, track_stats=False)
+        t0 = time.perf_counter()
+        b.run(seeds, gens + 1, 0.0, track_stats=False)
+        wall = time.perf_counter() - t0
+        prof = b.profile
+    loop_s = prof["loop_ms"] / 1e3
+    inst_gens = prof["generations"]
+    return {"workload": f"{len(insts)} instances (64 wavevectors x 16 seeds), T in {{33,65,129,257}}, "
+                        "G in {1024..8192}, omega_max in [128,512], standard/normalization/zero-T, P=16, "
+                        f"{gens} generations each, one deac_batch_run",
+            "instance_generations_per_s": inst_gens / loop_s,
+            "genome_evals_per_s": inst_gens * mu.POPULATION / loop_s,
+            "fp64_tflops": flop_per_gen * gens / loop_s / 1e12,
+            "ms": prof["loop_ms"], "wall_s_including_host_setup": wall}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +129,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=5000, help="timed generations of the plain path")
     ap.add_argument("--cpu-gens", type=int, default=2000, help="generations of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--uneven-gens", type=int, default=200, help="timed generations of the uneven batch (0: skip)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -151,6 +184,7 @@ def main():
         pout = e.run(PLAIN["seed"], args.plain_steps + 1, 0.0, track_stats=False)
     pprof = pout["profile"]
     plain_value = PLAIN["P"] * pprof["generations"] / (pprof["loop_ms"] / 1e3)
+    uneven = uneven_batch(da, args.uneven_gens) if args.uneven_gens > 0 else None
 
     line = {
         "metric": METRIC,
@@ -176,7 +210,8 @@ def main():
                    "plain_fp64": {"value": plain_value, "unit": "genome-evals/s",
                                   "workload": "single instance G=4096, T=65, P=8, seed 1",
                                   "ms_per_generation": pprof["loop_ms"] / pprof["generations"]},
-                   "batched_over_plain": value / plain_value},
+                   "batched_over_plain": value / plain_value,
+                   "uneven_batch": uneven},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "traffic_note": "HBM bytes per launch (one launch = the timed generations, up to the "

@@ -74,6 +74,11 @@ struct deac_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr;
+    // Instances of different kernel shapes run side by side: one stream per
+    // shape group (default priority), joined to `stream` by events.
+    static constexpr int kGroupStreams = 3;
+    hipStream_t gstream[kGroupStreams] = {};
+    hipEvent_t gdone[kGroupStreams] = {};
     std::vector<std::unique_ptr<BatchModel>> models;
     std::vector<int> model_of;                 // instance -> model
     std::vector<void *> allocations;
@@ -89,9 +94,15 @@ struct deac_batch {
     ~deac_batch()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipStream_t g : gstream)
+            if (g) (void)hipStreamSynchronize(g);
         for (void *p : allocations) (void)hipFree(p);
         for (hipEvent_t e : {ev0, ev1, kev0, kev1})
             if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : gdone)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t g : gstream)
+            if (g) (void)hipStreamDestroy(g);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -299,6 +310,10 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DEAC_ERR_NO_DEVICE;
     BATCH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     for (hipEvent_t *e : {&h->ev0, &h->ev1, &h->kev0, &h->kev1}) BATCH_HIP(hipEventCreate(e));
+    for (int g = 0; g < deac_batch::kGroupStreams; g++) {
+        BATCH_HIP(hipStreamCreateWithFlags(&h->gstream[g], hipStreamNonBlocking));
+        BATCH_HIP(hipEventCreateWithFlags(&h->gdone[g], hipEventDisableTiming));
+    }
     for (auto &b : h->models) {
         const deac_status su = upload_model(h.get(), *b);
         if (su != DEAC_OK) return su;
@@ -417,6 +432,14 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 groups[{b.NT, b.NM, b.nrem > 0 ? 1 : 0}].push_back(i);
             }
         if (groups.empty()) break;
+        // Longest instances first within a group (workgroups are dispatched
+        // in index order): per-generation cost ~ chunks x tiles per wave.
+        for (auto &[key, members] : groups)
+            std::stable_sort(members.begin(), members.end(), [&](size_t a, size_t b) {
+                const BatchModel &ma = *h->models[h->model_of[a]], &mb = *h->models[h->model_of[b]];
+                return static_cast<long>(ma.nchunk) * (ma.NT * 4 + ma.nrem) >
+                       static_cast<long>(mb.nchunk) * (mb.NT * 4 + mb.nrem);
+            });
         size_t slot = 0;
         std::vector<std::pair<std::tuple<int, int, int>, std::pair<size_t, size_t>>> launches;
         std::vector<size_t> order;
@@ -468,9 +491,24 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
         }
         BATCH_HIP(hipMemcpyAsync(h->descs, descs.data(), slot * sizeof(deac::BatchDesc), hipMemcpyHostToDevice, s));
         BATCH_HIP(hipEventRecord(h->kev0, s));
-        for (auto &[key, range] : launches)
+        if (launches.size() == 1) {
+            auto &[key, range] = launches[0];
             BATCH_HIP(deac::launch_batch(h->descs + range.first, static_cast<int>(range.second), std::get<0>(key),
                                          std::get<1>(key), std::get<2>(key), s));
+        } else {
+            // at most 2 x 2 x 3 shapes; groups beyond the streams share them
+            for (int g = 0; g < deac_batch::kGroupStreams; g++) BATCH_HIP(hipStreamWaitEvent(h->gstream[g], h->kev0, 0));
+            for (size_t k = 0; k < launches.size(); k++) {
+                auto &[key, range] = launches[k];
+                BATCH_HIP(deac::launch_batch(h->descs + range.first, static_cast<int>(range.second), std::get<0>(key),
+                                             std::get<1>(key), std::get<2>(key),
+                                             h->gstream[k % deac_batch::kGroupStreams]));
+            }
+            for (int g = 0; g < deac_batch::kGroupStreams; g++) {
+                BATCH_HIP(hipEventRecord(h->gdone[g], h->gstream[g]));
+                BATCH_HIP(hipStreamWaitEvent(s, h->gdone[g], 0));
+            }
+        }
         BATCH_HIP(hipEventRecord(h->kev1, s));
         std::vector<int> st(2 * n);
         BATCH_HIP(hipMemcpyAsync(st.data(), h->status, st.size() * sizeof(int), hipMemcpyDeviceToHost, s));

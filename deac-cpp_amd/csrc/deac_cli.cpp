@@ -6,6 +6,14 @@
 // runtime flags here: --model (DEAC_MODEL), --zero_temperature (the ZeroT
 // build, files prefixed deac-zT), --allow_negative_spectral_weight and
 // --single_particle_fermionic_spectral_function.
+//
+// Seed batches: --number_of_seeds K runs seeds seed .. seed+K-1 in one
+// batched launch (deac_batch_run) and writes, for each seed, exactly the
+// files and log of a separate `deac.e --seed s` run -- the workflow of
+// tools/generate_commands.sh:18-32 in one process.  --average_dsf adds the
+// seed-averaged DSF, summed in the order tools/plot_simple_average_deac_results.py:8-28
+// uses (run keys sorted as strings).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -235,6 +243,8 @@ int main(int argc, char **argv)
     prog.add("", "--zero_temperature", "Zero-temperature kernel exp(-tau*omega), positive frequencies only.", "", true);
     prog.add("", "--allow_negative_spectral_weight", "Allow negative spectral weights.", "", true);
     prog.add("", "--single_particle_fermionic_spectral_function", "Renormalize every generation when --normalize is set.", "", true);
+    prog.add("", "--number_of_seeds", "Run seeds seed, seed+1, .., seed+K-1 in one batched launch; each seed writes the files of its own run.", "1");
+    prog.add("", "--average_dsf", "With --number_of_seeds: also write the seed-averaged DSF (<prefix>_seed-average_<first>-<last>.bin).", "", true);
 
     try {
         prog.parse(argc, argv);
@@ -246,9 +256,12 @@ int main(int argc, char **argv)
 
     try {
         const unsigned long seed_arg = to_ulong(prog.get("--seed"), 0);
+        const unsigned long nseeds = to_ulong(prog.get("--number_of_seeds"), 1);
+        if (nseeds < 1) fail_with_error("number_of_seeds must be at least 1");
         std::string uuid = prog.get("--uuid").present ? prog.get("--uuid").value : "";
+        if (nseeds > 1 && !uuid.empty()) fail_with_error("--uuid cannot be combined with --number_of_seeds > 1");
         if (uuid.empty()) uuid = std::to_string(seed_arg);  // deac.cpp:2034-2037
-        std::cout << "uuid: " << uuid << std::endl;
+        if (nseeds == 1) std::cout << "uuid: " << uuid << std::endl;
 
         const std::string spectra = prog.get("--spectra_type").present ? prog.get("--spectra_type").value : "spfsf";
         const std::map<std::string, int> spectra_ids = {{"spbsf", DEAC_SPBSF}, {"spfsf", DEAC_SPFSF}, {"bfull", DEAC_BFULL}, {"ffull", DEAC_FFULL}};
@@ -278,8 +291,6 @@ int main(int argc, char **argv)
             if (err[i] <= 0.0) fail_with_error("ISF input errors must be positive");
         }
 
-        uint64_t rng_state[4];
-        deac_seed_state(seed_arg, rng_state);  // deac.cpp:2089-2091
 
         double temperature = to_double(prog.get("--temperature"), 0.0);
         if (!zero_t && (!std::isfinite(temperature) || temperature <= 0.0))
@@ -349,10 +360,12 @@ int main(int argc, char **argv)
         if (ec) fail_with_error("File error: cannot create save directory " + save_directory.string() + ": " + ec.message());
 
         const std::string prefix = std::string(zero_t ? "deac-zT" : "deac-") + spectra;  // deac.cpp:1687-1691
-        const fs::path log_path = save_directory / (prefix + "_log_" + uuid + ".dat");
-        {
-            std::ofstream log(log_path, std::ios_base::out | std::ios_base::app);  // deac.cpp:2169-2182
-            log << "uuid: " << uuid << std::endl;
+        std::vector<std::string> uuids(nseeds);
+        for (unsigned long k = 0; k < nseeds; k++) uuids[k] = nseeds == 1 ? uuid : std::to_string(seed_arg + k);
+        auto log_path_of = [&](const std::string &id) { return save_directory / (prefix + "_log_" + id + ".dat"); };
+        for (unsigned long k = 0; k < nseeds; k++) {
+            std::ofstream log(log_path_of(uuids[k]), std::ios_base::out | std::ios_base::app);  // deac.cpp:2169-2182
+            log << "uuid: " << uuids[k] << std::endl;
             log << "isf_file: " << isf_file << std::endl;
         }
 
@@ -363,77 +376,133 @@ int main(int argc, char **argv)
         }
         if (first_moment >= 0.0) std::cout << "WARNING: setting first_moment_error to 1.0." << std::endl;  // deac.cpp:627-628
 
-        deac_handle *h = nullptr;
-        const char *msg = nullptr;
-        deac_status st = deac_create(&p, &h, &msg);
-        if (st != DEAC_OK) {
-            std::cerr << "deac engine: " << deac_status_string(st) << (msg ? std::string(": ") + msg : "") << std::endl;
-            std::exit(1);
-        }
+        // One result per seed: the batch kernel when the problem fits it,
+        // else the single-instance engine seed by seed (same results).
         const size_t nout = deac_output_length(&p);
-        std::vector<double> dsf(nout), freq_out(nout), s_mean(N), s_min(N), s_sq(N);
-        deac_result r{};
-        r.dsf = dsf.data();
-        r.frequency = freq_out.data();
-        if (track_stats) {
-            r.stats_mean = s_mean.data();
-            r.stats_minimum = s_min.data();
-            r.stats_squared_mean = s_sq.data();
+        std::vector<std::vector<double>> dsf(nseeds, std::vector<double>(nout)), freq_out(nseeds, std::vector<double>(nout));
+        std::vector<std::vector<double>> s_mean(nseeds), s_min(nseeds), s_sq(nseeds);
+        std::vector<deac_result> res(nseeds);
+        std::vector<uint64_t> states(4 * nseeds);
+        for (unsigned long k = 0; k < nseeds; k++) {
+            deac_seed_state(seed_arg + k, &states[4 * k]);  // deac.cpp:2089-2091
+            res[k] = deac_result{};
+            res[k].dsf = dsf[k].data();
+            res[k].frequency = freq_out[k].data();
+            if (track_stats) {
+                s_mean[k].resize(N);
+                s_min[k].resize(N);
+                s_sq[k].resize(N);
+                res[k].stats_mean = s_mean[k].data();
+                res[k].stats_minimum = s_min[k].data();
+                res[k].stats_squared_mean = s_sq[k].data();
+            }
         }
-        deac_run_options ro{N, stop, track_stats ? 1 : 0};
-        st = deac_run(h, rng_state, &ro, &r);
-        deac_destroy(h);
-        if (st != DEAC_OK) {
-            std::cerr << "deac engine: " << deac_status_string(st) << std::endl;
-            std::exit(1);
+        const deac_run_options ro{N, stop, track_stats ? 1 : 0};
+        const char *msg = nullptr;
+        bool done = false;
+        if (nseeds > 1) {
+            std::vector<deac_problem> probs(nseeds, p);
+            deac_batch *b = nullptr;
+            deac_status st = deac_batch_create(probs.data(), nseeds, &b, &msg);
+            if (st == DEAC_OK) {
+                std::vector<deac_run_options> opts(nseeds, ro);
+                st = deac_batch_run(b, states.data(), opts.data(), res.data());
+                deac_batch_destroy(b);
+                if (st != DEAC_OK) {
+                    std::cerr << "deac engine: " << deac_status_string(st) << std::endl;
+                    std::exit(1);
+                }
+                done = true;
+            } else if (st != DEAC_ERR_UNSUPPORTED) {
+                std::cerr << "deac engine: " << deac_status_string(st) << (msg ? std::string(": ") + msg : "") << std::endl;
+                std::exit(1);
+            } else {
+                std::cout << "seeds run one after another (" << (msg ? msg : "not batchable") << ")" << std::endl;
+            }
+        }
+        if (!done) {
+            deac_handle *h = nullptr;
+            deac_status st = deac_create(&p, &h, &msg);
+            if (st != DEAC_OK) {
+                std::cerr << "deac engine: " << deac_status_string(st) << (msg ? std::string(": ") + msg : "") << std::endl;
+                std::exit(1);
+            }
+            for (unsigned long k = 0; k < nseeds && st == DEAC_OK; k++) st = deac_run(h, &states[4 * k], &ro, &res[k]);
+            deac_destroy(h);
+            if (st != DEAC_OK) {
+                std::cerr << "deac engine: " << deac_status_string(st) << std::endl;
+                std::exit(1);
+            }
         }
 
-        // Output files and log (deac.cpp:1687-1769).
-        const fs::path dsf_path = save_directory / (prefix + "_dsf_" + uuid + ".bin");
-        const fs::path freq_path = save_directory / (prefix + "_frequency_" + uuid + ".bin");
-        write_array(dsf_path, dsf.data(), nout);
-        write_array(freq_path, freq_out.data(), nout);
-        fs::path mean_path, min_path, sq_path;
-        if (track_stats) {
-            mean_path = save_directory / (prefix + "_stats_fitness-mean_" + uuid + ".bin");
-            min_path = save_directory / (prefix + "_stats_fitness-minimum_" + uuid + ".bin");
-            sq_path = save_directory / (prefix + "_stats_fitness-squared-mean_" + uuid + ".bin");
-            write_array(mean_path, s_mean.data(), r.generation + 1);
-            write_array(min_path, s_min.data(), r.generation + 1);
-            write_array(sq_path, s_sq.data(), r.generation + 1);
+        // Output files and log of each seed (deac.cpp:1687-1769).
+        for (unsigned long k = 0; k < nseeds; k++) {
+            const std::string &id = uuids[k];
+            const deac_result &r = res[k];
+            if (nseeds > 1) std::cout << "uuid: " << id << std::endl;
+            const fs::path dsf_path = save_directory / (prefix + "_dsf_" + id + ".bin");
+            const fs::path freq_path = save_directory / (prefix + "_frequency_" + id + ".bin");
+            write_array(dsf_path, dsf[k].data(), nout);
+            write_array(freq_path, freq_out[k].data(), nout);
+            fs::path mean_path, min_path, sq_path;
+            if (track_stats) {
+                mean_path = save_directory / (prefix + "_stats_fitness-mean_" + id + ".bin");
+                min_path = save_directory / (prefix + "_stats_fitness-minimum_" + id + ".bin");
+                sq_path = save_directory / (prefix + "_stats_fitness-squared-mean_" + id + ".bin");
+                write_array(mean_path, s_mean[k].data(), r.generation + 1);
+                write_array(min_path, s_min[k].data(), r.generation + 1);
+                write_array(sq_path, s_sq[k].data(), r.generation + 1);
+            }
+            std::ofstream log(log_path_of(id), std::ios_base::out | std::ios_base::app);
+            log << "build: " << model_build_name(p.model) << std::endl;
+            log << "kernel: " << spectra << std::endl;
+            log << "temperature: " << temperature << std::endl;
+            log << "number_of_generations: " << N << std::endl;
+            log << "number_of_timeslices: " << T << std::endl;
+            log << "population_size: " << P << std::endl;
+            log << "genome_size: " << frequency.size() << std::endl;
+            log << "normalize: " << normalize << std::endl;
+            log << "use_negative_first_moment: " << use_negative_first_moment << std::endl;
+            log << "first_moment: " << first_moment << std::endl;
+            log << "third_moment: " << third_moment << std::endl;
+            log << "third_moment_error: " << third_moment_error << std::endl;
+            log << "crossover_probability: " << p.crossover_probability << std::endl;
+            log << "self_adapting_crossover_probability: " << p.self_adapting_crossover_probability << std::endl;
+            log << "differential_weight: " << p.differential_weight << std::endl;
+            log << "self_adapting_differential_weight_probability: " << p.self_adapting_differential_weight_probability << std::endl;
+            log << "self_adapting_differential_weight_shift: " << shift << std::endl;
+            log << "self_adapting_differential_weight: " << sadw << std::endl;
+            log << "stop_minimum_fitness: " << stop << std::endl;
+            log << "track_stats: " << track_stats << std::endl;
+            log << "seed: " << seed_arg + k << std::endl;
+            log << "best_dsf_filename: " << dsf_path << std::endl;
+            log << "frequency_filename: " << freq_path << std::endl;
+            log << "generation: " << r.generation << std::endl;
+            std::cout << "generation: " << r.generation << std::endl;
+            log << "minimum_fitness: " << r.minimum_fitness << std::endl;
+            std::cout << "minimum_fitness: " << r.minimum_fitness << std::endl;
+            if (track_stats) {
+                log << "fitness_mean_filename: " << mean_path << std::endl;
+                log << "fitness_minimum_filename: " << min_path << std::endl;
+                log << "fitness_squared_mean_filename: " << sq_path << std::endl;
+            }
         }
-        std::ofstream log(log_path, std::ios_base::out | std::ios_base::app);
-        log << "build: " << model_build_name(p.model) << std::endl;
-        log << "kernel: " << spectra << std::endl;
-        log << "temperature: " << temperature << std::endl;
-        log << "number_of_generations: " << N << std::endl;
-        log << "number_of_timeslices: " << T << std::endl;
-        log << "population_size: " << P << std::endl;
-        log << "genome_size: " << frequency.size() << std::endl;
-        log << "normalize: " << normalize << std::endl;
-        log << "use_negative_first_moment: " << use_negative_first_moment << std::endl;
-        log << "first_moment: " << first_moment << std::endl;
-        log << "third_moment: " << third_moment << std::endl;
-        log << "third_moment_error: " << third_moment_error << std::endl;
-        log << "crossover_probability: " << p.crossover_probability << std::endl;
-        log << "self_adapting_crossover_probability: " << p.self_adapting_crossover_probability << std::endl;
-        log << "differential_weight: " << p.differential_weight << std::endl;
-        log << "self_adapting_differential_weight_probability: " << p.self_adapting_differential_weight_probability << std::endl;
-        log << "self_adapting_differential_weight_shift: " << shift << std::endl;
-        log << "self_adapting_differential_weight: " << sadw << std::endl;
-        log << "stop_minimum_fitness: " << stop << std::endl;
-        log << "track_stats: " << track_stats << std::endl;
-        log << "seed: " << seed_arg << std::endl;
-        log << "best_dsf_filename: " << dsf_path << std::endl;
-        log << "frequency_filename: " << freq_path << std::endl;
-        log << "generation: " << r.generation << std::endl;
-        std::cout << "generation: " << r.generation << std::endl;
-        log << "minimum_fitness: " << r.minimum_fitness << std::endl;
-        std::cout << "minimum_fitness: " << r.minimum_fitness << std::endl;
-        if (track_stats) {
-            log << "fitness_mean_filename: " << mean_path << std::endl;
-            log << "fitness_minimum_filename: " << min_path << std::endl;
-            log << "fitness_squared_mean_filename: " << sq_path << std::endl;
+
+        // Seed average (tools/plot_simple_average_deac_results.py:8-28): the
+        // first key's DSF plus the others in string order of the keys, then
+        // divided by the count.
+        if (prog.get("--average_dsf").present) {
+            std::vector<unsigned long> order(nseeds);
+            for (unsigned long k = 0; k < nseeds; k++) order[k] = k;
+            std::sort(order.begin(), order.end(), [&](unsigned long a, unsigned long b) { return uuids[a] < uuids[b]; });
+            std::vector<double> avg(dsf[order[0]]);
+            for (unsigned long k = 1; k < nseeds; k++)
+                for (size_t i = 0; i < nout; i++) avg[i] += dsf[order[k]][i];
+            for (size_t i = 0; i < nout; i++) avg[i] /= static_cast<double>(nseeds);
+            const fs::path avg_path = save_directory / (prefix + "_seed-average_" + std::to_string(seed_arg) + "-" +
+                                                        std::to_string(seed_arg + nseeds - 1) + ".bin");
+            write_array(avg_path, avg.data(), nout);
+            std::cout << "seed_average_filename: " << avg_path << std::endl;
         }
     } catch (const std::runtime_error &err) {
         std::cout << err.what() << std::endl;

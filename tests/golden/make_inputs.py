@@ -9,8 +9,9 @@ deac.e reads (reference: deac.cpp:2060-2074, tools/convert_isf_data.py:13-19).
   isf_c2_T65.bin   three peaks (mu,sd,w) = (3,0.4,.5), (20,4,.3), (60,10,.2)
                    on [0,512], 65 tau, sigma = 1e-4 F
   isf_c3_T257.bin  the same spectrum with 257 tau
-  uneven/          the 64 wavevector inputs of the uneven batch (see
-                   make_uneven.py)
+
+The 1024 inputs of the uneven batch are built by make_uneven.py from the
+committed uneven_table.json.
 
 F(tau) = int S(w) (e^{-tau w} + e^{-(beta-tau) w}) dw by the trapezoid rule on
 a 65536-point grid; S is normalised to unit zeroth moment on the same grid.

@@ -140,3 +140,32 @@ def test_seed_batch_full_size():
         assert np.all(o["dsf"] >= 0)
     avg = np.mean([o["dsf"] for o in outs], axis=0)
     assert np.all(np.isfinite(avg)) and avg.shape == (2 * 4096 - 1,)
+
+
+def test_uneven_batch():
+    # the uneven batch (tests/golden/make_uneven.py): 64 wavevectors x 16
+    # seeds of mixed T, G, omega_max, model and zero-temperature kernels in
+    # one deac_batch_run; a spread of 24 instances checked bit for bit
+    # against the single-instance engine, the rest by properties
+    import sys
+    sys.path.insert(0, gc.GOLDEN_DIR)
+    import make_uneven as mu
+    insts = mu.instances()
+    print(f"[gpu] uneven batch of {len(insts)} instances", flush=True)
+    N = 8
+    probs = [da.make_problem(x["tau"], x["isf"], x["err"], da.frequency_grid(x["G"], x["omega_max"]), mu.TEMPERATURE,
+                             model=x["model"], zero_temperature=x["zero_temperature"], population_size=x["P"],
+                             normalize=True, spectra_type="bfull") for x in insts]
+    with da.Batch(probs) as b:
+        outs = b.run([x["seed"] for x in insts], N, 0.0, True)
+        assert b.profile["generations"] == len(insts) * (N - 1)
+    for i in range(0, len(insts), 43):
+        x = insts[i]
+        with da.Engine(probs[i]) as e:
+            ref = e.run(x["seed"], N, 0.0)
+        assert not _compare(outs[i], ref), (i, x["k"], x["seed"], _compare(outs[i], ref))
+        assert outs[i]["rng"] == ref["rng"]
+    for x, o in zip(insts, outs):
+        nout = x["G"] if x["zero_temperature"] else 2 * x["G"] - 1
+        assert o["dsf"].shape == (nout,) and np.all(np.isfinite(o["dsf"]))
+        assert np.all(np.diff(o["minimum"]) <= 0)

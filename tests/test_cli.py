@@ -217,3 +217,70 @@ def test_cli_smoke_runs(tmp_path, case):
     if case == "track_stats":
         for k in ("mean", "minimum", "squared-mean"):
             assert (res / f"deac-spfsf_stats_fitness-{k}_{seed}.bin").stat().st_size == 3 * 8
+
+
+# ---------------------------------------------------------------- seed batches
+
+def test_number_of_seeds_must_be_positive(tmp_path):
+    fixture = tmp_path / "tiny-isf.bin"
+    write_fixture(fixture)
+    r = run(deac_command(tmp_path, fixture, extra=["--number_of_seeds", "0"]), tmp_path)
+    assert r.returncode == 1
+    assert "number_of_seeds must be at least 1" in r.stdout
+
+
+def test_uuid_with_several_seeds_is_rejected(tmp_path):
+    fixture = tmp_path / "tiny-isf.bin"
+    write_fixture(fixture)
+    r = run(deac_command(tmp_path, fixture, extra=["--number_of_seeds", "2", "--uuid", "x"]), tmp_path)
+    assert r.returncode == 1
+    assert "--uuid cannot be combined with --number_of_seeds > 1" in r.stdout
+
+
+def _log_without_paths(path):
+    return [l for l in path.read_text().splitlines() if "filename:" not in l]
+
+
+def _seed_batch_vs_single_runs(tmp_path, extra, seeds, expect_batched):
+    fixture = tmp_path / "tiny-isf.bin"
+    write_fixture(fixture)
+    batch_dir, single_dir = tmp_path / "batch", tmp_path / "single"
+    cmd = deac_command(tmp_path, fixture, N="60", seed=str(seeds[0]),
+                       extra=list(extra) + ["--track_stats", "--number_of_seeds", str(len(seeds)), "--average_dsf"])
+    cmd[cmd.index("--save_directory") + 1] = str(batch_dir)
+    r = run(cmd, tmp_path)
+    assert r.returncode == 0, r.stdout
+    assert ("seeds run one after another" not in r.stdout) == expect_batched, r.stdout
+    for s in seeds:
+        cmd = deac_command(tmp_path, fixture, N="60", seed=str(s), extra=list(extra) + ["--track_stats"])
+        cmd[cmd.index("--save_directory") + 1] = str(single_dir)
+        rs = run(cmd, tmp_path)
+        assert rs.returncode == 0, rs.stdout
+    for s in seeds:
+        for kind in ("dsf", "frequency", "stats_fitness-mean", "stats_fitness-minimum", "stats_fitness-squared-mean"):
+            a = np.fromfile(batch_dir / f"deac-spfsf_{kind}_{s}.bin")
+            b = np.fromfile(single_dir / f"deac-spfsf_{kind}_{s}.bin")
+            assert a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64)), (s, kind)
+        assert _log_without_paths(batch_dir / f"deac-spfsf_log_{s}.dat") == \
+            _log_without_paths(single_dir / f"deac-spfsf_log_{s}.dat")
+    # tools/plot_simple_average_deac_results.py:8-28 over the single runs: keys
+    # sorted as strings, the first DSF plus the others in that order, / count
+    keys = np.sort(np.array([str(s) for s in seeds]))
+    avg = np.fromfile(single_dir / f"deac-spfsf_dsf_{keys[0]}.bin")
+    for k in keys[1:]:
+        avg += np.fromfile(single_dir / f"deac-spfsf_dsf_{k}.bin")
+    avg /= keys.shape[0]
+    got = np.fromfile(batch_dir / f"deac-spfsf_seed-average_{seeds[0]}-{seeds[-1]}.bin")
+    assert np.array_equal(got.view(np.uint64), avg.view(np.uint64))
+
+
+@pytest.mark.gpu
+def test_cli_seed_batch_matches_single_runs(tmp_path):
+    # seeds 8..12 sort as strings 10, 11, 12, 8, 9
+    _seed_batch_vs_single_runs(tmp_path, ["--normalize"], list(range(8, 13)), expect_batched=True)
+
+
+@pytest.mark.gpu
+def test_cli_seed_batch_unbatchable_runs_seed_by_seed(tmp_path):
+    # moment constraints run through the single-instance engine
+    _seed_batch_vs_single_runs(tmp_path, ["--first_moment", "0.5"], [3, 4], expect_batched=False)
