@@ -25,6 +25,13 @@ the plain workload.
 64 wavevectors x 16 seeds with mixed T, G, omega_max, model and
 zero-temperature kernels, population 16) for a bounded number of generations
 in one deac_batch_run, as instance-generations/s and sustained FP64 TFLOP/s.
+
+`config.precision_sweep` evaluates K.S with deac_model_isf at FP64, FP32 and
+split-BF16x3 (chi^2 in FP64): |dchi^2|/chi^2 against FP64 for 64 genomes
+near the solution of the T=257, G=16384 hyperbolic workload (the true
+spectrum on the genome grid, 0.1 % multiplicative noise), and the
+contraction's throughput for a stack of 8192 genomes (256 seeds x 32
+members) at G=4096, T=65.
 """
 import argparse
 import json
@@ -121,6 +128,45 @@ def uneven_batch(da, gens):
             "ms": prof["loop_ms"], "wall_s_including_host_setup": wall}
 
 
+def precision_sweep(da):
+    import numpy as np
+    import make_inputs as mi
+    out = {}
+    # near-solution genomes of the T=257, G=16384 hyperbolic workload:
+    # genome = 2 S(w) e^{beta w / 2} on both signs reproduces the ISF
+    tau, f, s = np.fromfile(os.path.join(ROOT, "tests", "golden", "isf_c3_T257.bin")).reshape(3, -1)
+    G3, beta = 16384, 1.0 / TEMPERATURE
+    w = da.frequency_grid(G3, OMEGA_MAX)
+    grid = np.linspace(0.0, OMEGA_MAX, 65536)
+    norm = np.trapezoid(sum(wt * mi.gaussian(grid, mu, sd) for mu, sd, wt in mi.CONFIG2_PEAKS), grid)
+    S = sum(wt * mi.gaussian(w, mu, sd) for mu, sd, wt in mi.CONFIG2_PEAKS) / norm
+    g0 = np.concatenate([2.0 * S * np.exp(0.5 * beta * w)] * 2)
+    pop = g0 * (1.0 + 1e-3 * np.random.default_rng(3).standard_normal((64, 2 * G3)))
+    prob = da.make_problem(tau, f, s, w, TEMPERATURE, model="hyperbolic", spectra_type="bfull", population_size=64,
+                           normalize=True)
+    E64, c64, _ = da.model_isf(prob, pop, "fp64")
+    near = {"chi2_fp64_median": float(np.median(c64))}
+    for prec in ("fp32", "bf16x3"):
+        E, c, _ = da.model_isf(prob, pop, prec)
+        rel = np.abs(c - c64) / c64
+        near[prec] = {"dchi2_rel_median": float(np.median(rel)), "dchi2_rel_max": float(np.max(rel)),
+                      "model_isf_rel_max": float(np.max(np.abs(E - E64) / np.abs(E64)))}
+    out["near_solution_T257_G16384_hyperbolic"] = near
+    # stacked contraction at G=4096, T=65: 8192 genomes
+    t2, f2, s2 = load_isf()
+    p2 = da.make_problem(t2, f2, s2, da.frequency_grid(G, OMEGA_MAX), TEMPERATURE, population_size=32, normalize=True,
+                         spectra_type="bfull")
+    genomes = np.random.default_rng(4).uniform(0.0, 1.0, (8192, 2 * G)) / G
+    flop = 2.0 * T * (2 * G) * genomes.shape[0]
+    thr = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        da.model_isf(p2, genomes, prec)
+        _, _, ms = da.model_isf(p2, genomes, prec)
+        thr[prec] = {"ms": ms, "tflops": flop / (ms / 1e3) / 1e12}
+    out["stacked_8192_genomes_G4096_T65"] = thr
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +176,7 @@ def main():
     ap.add_argument("--cpu-gens", type=int, default=2000, help="generations of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--uneven-gens", type=int, default=200, help="timed generations of the uneven batch (0: skip)")
+    ap.add_argument("--no-precision-sweep", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -185,6 +232,7 @@ def main():
     pprof = pout["profile"]
     plain_value = PLAIN["P"] * pprof["generations"] / (pprof["loop_ms"] / 1e3)
     uneven = uneven_batch(da, args.uneven_gens) if args.uneven_gens > 0 else None
+    sweep = None if args.no_precision_sweep else precision_sweep(da)
 
     line = {
         "metric": METRIC,
@@ -211,7 +259,8 @@ def main():
                                   "workload": "single instance G=4096, T=65, P=8, seed 1",
                                   "ms_per_generation": pprof["loop_ms"] / pprof["generations"]},
                    "batched_over_plain": value / plain_value,
-                   "uneven_batch": uneven},
+                   "uneven_batch": uneven,
+                   "precision_sweep": sweep},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "traffic_note": "HBM bytes per launch (one launch = the timed generations, up to the "

@@ -21,7 +21,7 @@ STATUS = {0: "success", 1: "invalid argument", 2: "no usable gfx950 device", 3: 
 
 EXPORTED = ["deac_create", "deac_run", "deac_destroy", "deac_output_length", "deac_seed_state",
             "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_run_profile",
-            "deac_batch_create", "deac_batch_run", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile"]
+            "deac_batch_create", "deac_batch_run", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile", "deac_genome_length", "deac_model_isf"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -116,6 +116,11 @@ def lib():
         L.deac_batch_size.restype = C.c_size_t
         L.deac_batch_last_profile.argtypes = [C.c_void_p, C.POINTER(RunProfile)]
         L.deac_batch_last_profile.restype = C.c_int
+        L.deac_genome_length.argtypes = [C.POINTER(Problem)]
+        L.deac_genome_length.restype = C.c_size_t
+        L.deac_model_isf.argtypes = [C.POINTER(Problem), _dp, C.c_size_t, C.c_int, _dp, _dp,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_char_p)]
+        L.deac_model_isf.restype = C.c_int
         _lib = L
     return _lib
 
@@ -298,3 +303,24 @@ class Batch:
 def run(problem, seed, number_of_generations, stop_minimum_fitness=1.0, track_stats=True):
     with Engine(problem) as e:
         return e.run(seed, number_of_generations, stop_minimum_fitness, track_stats)
+
+
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+
+def model_isf(problem, genomes, precision="fp64"):
+    """deac_model_isf: model ISF (count x T) and reduced chi^2 (count) of a
+    stack of genomes at a K.S precision, plus the contraction's device time
+    in ms."""
+    L = lib().deac_genome_length(C.byref(problem))
+    g = np.ascontiguousarray(genomes, dtype=np.float64).reshape(-1, L)
+    n, T = g.shape[0], problem.number_of_timeslices
+    e = np.zeros((n, T))
+    chi = np.zeros(n)
+    ms = C.c_double(0.0)
+    msg = C.c_char_p()
+    st = lib().deac_model_isf(C.byref(problem), _ptr(g), n, PRECISIONS[precision], _ptr(e), _ptr(chi),
+                              C.byref(ms), C.byref(msg))
+    if st:
+        raise DeacError(st, msg.value.decode() if msg.value else None)
+    return e, chi, ms.value

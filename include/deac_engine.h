@@ -176,6 +176,36 @@ size_t deac_batch_size(const deac_batch *batch);
  * (initial evaluation included), generations is summed over instances. */
 deac_status deac_batch_last_profile(const deac_batch *batch, deac_run_profile *profile);
 
+/* ---- K.S precision sweep ------------------------------------------------
+ * The model ISF E = K.S of a stack of genomes, the contraction of the
+ * reference's fitness evaluation (deac.cpp:1385-1395, matrix_multiply_LxM_by_MxN
+ * at deac.cpp:110-118), at a chosen precision of K.S:
+ *   DEAC_PRECISION_FP64    the engine's exact evaluation (bit for bit the
+ *                          reference's FMA chains);
+ *   DEAC_PRECISION_FP32    K scaled per tau row by a power of two, K and S in
+ *                          FP32 on v_mfma_f32_16x16x4f32, FP32 accumulation;
+ *   DEAC_PRECISION_BF16X3  K and S split into bfloat16 pairs, three products
+ *                          per element on v_mfma_f32_16x16x32_bf16, FP32
+ *                          accumulation.
+ * It has no counterpart among the reference's functions: it exists to report
+ * |dchi^2|/chi^2 of the mixed-precision forms against FP64.
+ *
+ * genomes: count rows of deac_genome_length(problem) doubles (positive
+ * frequencies then negative, the population layout of deac.cpp:430-458).
+ * Outputs (each may be NULL): model_isf, count x T; chi2, count reduced
+ * chi^2 values (deac.cpp:120-126, 1540-1542; FP64 in the reference's order,
+ * no moment terms); kernel_ms, device time of the contraction. */
+typedef enum deac_precision {
+    DEAC_PRECISION_FP64 = 0,
+    DEAC_PRECISION_FP32 = 1,
+    DEAC_PRECISION_BF16X3 = 2
+} deac_precision;
+
+size_t deac_genome_length(const deac_problem *problem);
+
+deac_status deac_model_isf(const deac_problem *problem, const double *genomes, size_t count, int precision,
+                           double *model_isf, double *chi2, double *kernel_ms, const char **message);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,3 +140,22 @@ def test_batch_create_without_device_reports_no_device():
     with pytest.raises(da.DeacError) as e:
         da.Batch([_batch_problem()] * 3)
     assert e.value.status == 2
+
+
+def test_model_isf_rejects_bad_arguments_before_the_device():
+    import ctypes as C
+    tau = np.linspace(0.0, 0.3, 4)
+    p = da.make_problem(tau, np.exp(-tau), 1e-3 * np.exp(-tau), da.frequency_grid(8, 4.0), 1.0,
+                        spectra_type="bfull")
+    assert da.lib().deac_genome_length(C.byref(p)) == 16
+    g = np.zeros((2, 16))
+    msg = C.c_char_p()
+    st = da.lib().deac_model_isf(C.byref(p), g.ctypes.data_as(C.POINTER(C.c_double)), 2, 7, None, None, None,
+                                 C.byref(msg))
+    assert st == 1 and b"precision" in msg.value
+    st = da.lib().deac_model_isf(C.byref(p), g.ctypes.data_as(C.POINTER(C.c_double)), 0, 0, None, None, None,
+                                 C.byref(msg))
+    assert st == 1
+    pz = da.make_problem(tau, np.exp(-tau), 1e-3 * np.exp(-tau), da.frequency_grid(8, 4.0), 0.0,
+                         spectra_type="bfull", zero_temperature=True)
+    assert da.lib().deac_genome_length(C.byref(pz)) == 8
