@@ -13,7 +13,7 @@
 //   * evolve_kernel - a persistent kernel (all workgroups co-resident) that runs a batch of
 //     generations: best-fitness/stats/stop test, trial vectors (DE/rand/1,
 //     binomial crossover, |.|, zero-frequency match), the model ISF
-//     E = K.S as exact FP64 FMA chains, chi^2 in the reference's summation
+//     E = K.S as exact FP64 FMA chains (4x4x4 FP64 MFMAs), chi^2 in the reference's summation
 //     order, greedy selection.  Grid barriers separate the phases.
 // The file is compiled with -ffp-contract=off: every fused multiply-add is
 // written as fma().
@@ -285,26 +285,6 @@ __device__ __forceinline__ void wait_copies(int wid, int pending)
     }
 }
 
-#define DEAC_FMAC_DPP(Q, K) "v_fmac_f64_dpp %0, %1, %" #K " row_newbcast:" #Q " row_mask:0xf bank_mask:0xf\n"
-
-// 16 chain steps: acc += s[row, q] * k[q] for q = 0..15 in order, where s
-// holds the row's 16 genome values (lane q of the row) and reaches every lane
-// of the row by DPP row broadcast.  One rounding per step (fused), the
-// reference's FMA chain (deac.cpp:110-118).  The s_nop covers the VALU-write
-// to DPP-read hazard on s.
-__device__ __forceinline__ double fmac16(double acc, double s, const double (&k)[16])
-{
-    asm volatile("s_nop 1\n"
-                 DEAC_FMAC_DPP(0, 2) DEAC_FMAC_DPP(1, 3) DEAC_FMAC_DPP(2, 4) DEAC_FMAC_DPP(3, 5)
-                 DEAC_FMAC_DPP(4, 6) DEAC_FMAC_DPP(5, 7) DEAC_FMAC_DPP(6, 8) DEAC_FMAC_DPP(7, 9)
-                 DEAC_FMAC_DPP(8, 10) DEAC_FMAC_DPP(9, 11) DEAC_FMAC_DPP(10, 12) DEAC_FMAC_DPP(11, 13)
-                 DEAC_FMAC_DPP(12, 14) DEAC_FMAC_DPP(13, 15) DEAC_FMAC_DPP(14, 16) DEAC_FMAC_DPP(15, 17)
-                 : "+v"(acc)
-                 : "v"(s), "v"(k[0]), "v"(k[1]), "v"(k[2]), "v"(k[3]), "v"(k[4]), "v"(k[5]), "v"(k[6]), "v"(k[7]),
-                   "v"(k[8]), "v"(k[9]), "v"(k[10]), "v"(k[11]), "v"(k[12]), "v"(k[13]), "v"(k[14]), "v"(k[15]));
-    return acc;
-}
-
 // Model ISF E[i][t] of one (tau block, member group) task; the whole
 // workgroup takes part: wave 0 runs the chains, waves 1-3 stage chunks.
 // E = sum over the concatenated genome (positive sign, then negative sign)
@@ -313,7 +293,6 @@ __device__ __forceinline__ double fmac16(double acc, double s, const double (&k)
 __device__ void chain_task(const EvoArgs &a, double *lds, int tb, int mg, bool init)
 {
     const int wid = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    const int r = lane >> 4, c = lane & 15;
     __shared__ const double *srows[kChainRows];
     if (threadIdx.x < kChainRows) srows[threadIdx.x] = chain_row(a, mg * kChainRows + threadIdx.x, init);
     __syncthreads();
@@ -328,33 +307,17 @@ __device__ void chain_task(const EvoArgs &a, double *lds, int tb, int mg, bool i
     double acc = 0.0;
     for (int ch = 0; ch < nchunk; ch++) {
         if (wid == 0) {
+            // 64 chains (16 tau x 4 members) advanced 4 positions per
+            // v_mfma_f64_4x4x4_4b_f64, which is bit for bit four fused
+            // multiply-adds in ascending position order: block b holds tau
+            // rows 4b..4b+3 of the task, every block the same 4 members.
+            // A: lane l reads tau row l&15 at position j + (l>>4); B: member
+            // l&3 at the same position (a broadcast across the blocks).
             const int st = ch % kStages;
-            const double *kb = lds + st * kKBuf + c * kRowPad;
-            const double *sb = lds + kStages * kKBuf + st * kSBuf + r * kRowPad + c;
-            // Operands of group g+1 are read while group g runs.
-            double k[16], kn[16];
-            double sv = sb[0], svn;
+            const double *kb = lds + st * kKBuf + (lane & 15) * kRowPad + (lane >> 4);
+            const double *sb = lds + kStages * kKBuf + st * kSBuf + (lane & 3) * kRowPad + (lane >> 4);
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const double2 v = *reinterpret_cast<const double2 *>(kb + 2 * u);
-                k[2 * u] = v.x;
-                k[2 * u + 1] = v.y;
-            }
-#pragma unroll
-            for (int j0 = 0; j0 < kChunk; j0 += 16) {
-                const int jn = j0 + 16 < kChunk ? j0 + 16 : j0;
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const double2 v = *reinterpret_cast<const double2 *>(kb + jn + 2 * u);
-                    kn[2 * u] = v.x;
-                    kn[2 * u + 1] = v.y;
-                }
-                svn = sb[jn];
-                acc = fmac16(acc, sv, k);
-#pragma unroll
-                for (int u = 0; u < 16; u++) k[u] = kn[u];
-                sv = svn;
-            }
+            for (int j = 0; j < kChunk; j += 4) acc = __builtin_amdgcn_mfma_f64_4x4x4f64(kb[j], sb[j], acc, 0, 0, 0);
             DEAC_WAIT_LDS();  // the stage may be refilled after the barrier
         } else {
             // Keep kStages-1 chunks ahead of the compute wave; before the
@@ -367,7 +330,8 @@ __device__ void chain_task(const EvoArgs &a, double *lds, int tb, int mg, bool i
         chunk_barrier();
     }
     if (wid == 0) {
-        const int i = mg * kChainRows + r, t = tb * kChainTau + c;
+        // 4x4x4_4b output: lane l holds row l>>4 of block (l>>2)&3, column l&3
+        const int i = mg * kChainRows + (lane & 3), t = tb * kChainTau + 4 * ((lane >> 2) & 3) + (lane >> 4);
         if (i < a.P && t < a.T) a.emod[static_cast<size_t>(i) * a.T + t] = acc;
     }
     __syncthreads();
