@@ -27,29 +27,41 @@ struct DeviceBuffer {
 struct deac_handle {
     deac::Model m;
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // evolution
+    hipStream_t draw_stream = nullptr;  // RNG walk, batches ahead of the evolution
     int MG = 0, TB = 0, stride = 0, nwg = 0, chunk = 0, batch = 0;
+    bool overlap = false;               // walk beside the evolution (a CU is left free for it)
     std::vector<void *> allocations;
     double *kmat = nullptr, *nterm = nullptr, *fterm = nullptr, *tterm = nullptr, *obs = nullptr, *sig = nullptr;
     double *pop0 = nullptr, *pop1 = nullptr, *fitness = nullptr, *cr_old = nullptr, *f_old = nullptr;
     double *emod = nullptr, *mom = nullptr;
-    int *cur = nullptr, *status = nullptr;
-    unsigned *bar = nullptr;
-    deac::MemberCtl *ctl = nullptr;
-    uint64_t *raw = nullptr, *gen_start = nullptr, *state = nullptr, *skip_rows = nullptr, *lane_tab = nullptr;
+    int *cur = nullptr;
+    // Two buffer sets, used by batches of even and odd index: grid barrier
+    // counter ([0]) and status ([4..6]) per set, and the batch's draws.
+    unsigned *sync = nullptr;           // [2][8]
+    int *host_status = nullptr;         // [3][4], pinned: status of the sets, then of the initial fitness
+    deac::MemberCtl *ctl[2] = {};
+    uint64_t *raw[2] = {}, *gen_start[2] = {};
+    uint64_t *state = nullptr, *skip_rows = nullptr, *lane_tab = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t kev[6] = {};      // walk, mask, evolve boundaries of one batch
+    // per set: walk start, walk end, mask start, mask end, evolve start,
+    // evolve end (draws free), status copied
+    hipEvent_t kev[2][7] = {};
     deac_run_profile last{};
 
     ~deac_handle()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (draw_stream) (void)hipStreamSynchronize(draw_stream);
         for (void *p : allocations) (void)hipFree(p);
+        if (host_status) (void)hipHostFree(host_status);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        for (hipEvent_t e : kev)
-            if (e) (void)hipEventDestroy(e);
+        for (auto &set : kev)
+            for (hipEvent_t e : set)
+                if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
+        if (draw_stream) (void)hipStreamDestroy(draw_stream);
     }
 
     template <class T>
@@ -158,17 +170,25 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
         delete h;
         return s;
     };
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(DEAC_ERR_DEVICE);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->draw_stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(DEAC_ERR_DEVICE);
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return fail(DEAC_ERR_DEVICE);
-    for (hipEvent_t &e : h->kev)
-        if (hipEventCreate(&e) != hipSuccess) return fail(DEAC_ERR_DEVICE);
+    for (auto &set : h->kev)
+        for (hipEvent_t &e : set)
+            if (hipEventCreate(&e) != hipSuccess) return fail(DEAC_ERR_DEVICE);
+    if (hipHostMalloc(reinterpret_cast<void **>(&h->host_status), 12 * sizeof(int)) != hipSuccess) {
+        h->host_status = nullptr;
+        return fail(DEAC_ERR_OUT_OF_MEMORY);
+    }
 
     const int T = static_cast<int>(m.T), P = static_cast<int>(m.P), Kc = static_cast<int>(m.Kc);
     h->MG = (P + deac::kChainRows - 1) / deac::kChainRows;
     h->TB = (T + deac::kChainTau - 1) / deac::kChainTau;
     h->stride = round_up(Kc, deac::kChunk);
     h->chunk = (Kc + 63) / 64;
-    // RNG batch: generations whose mask draws are buffered at once (<= 256 MiB).
+    // RNG batch: generations whose mask draws are buffered at once (<= 256 MiB
+    // per buffer set).
     const size_t per_gen = static_cast<size_t>(P) * Kc * sizeof(uint64_t);
     h->batch = static_cast<int>(std::clamp<size_t>((size_t(256) << 20) / per_gen, 1, 1024));
 
@@ -177,6 +197,7 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
     int nwg = std::max(h->MG * h->TB, 32);
     if (const char *e = std::getenv("DEAC_EVOLVE_WORKGROUPS")) nwg = std::atoi(e);
     h->nwg = std::clamp(nwg, 1, maxwg);
+    h->overlap = h->nwg < maxwg;
 
     // Device buffers.
     const size_t kmat_n = static_cast<size_t>(h->TB) * deac::kChainTau * h->stride;
@@ -196,11 +217,12 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
     A(h->f_old, static_cast<size_t>(P) * m.nsign);
     A(h->emod, static_cast<size_t>(P) * T);
     A(h->mom, static_cast<size_t>(P) * 4);
-    A(h->status, 3);
-    A(h->bar, 1);
-    A(h->ctl, static_cast<size_t>(h->batch) * P);
-    A(h->raw, static_cast<size_t>(h->batch) * P * Kc);
-    A(h->gen_start, static_cast<size_t>(h->batch) * 4);
+    A(h->sync, 16);
+    for (int k = 0; k < 2; k++) {
+        A(h->ctl[k], static_cast<size_t>(h->batch) * P);
+        A(h->raw[k], static_cast<size_t>(h->batch) * P * Kc);
+        A(h->gen_start[k], static_cast<size_t>(h->batch) * 4);
+    }
     A(h->state, 4);
     A(h->skip_rows, 256 * 4);
     A(h->lane_tab, static_cast<size_t>(64) * 64 * 16 * 4);
@@ -280,8 +302,6 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
     DEAC_HIP(hipMemcpyAsync(h->cr_old, cr.data(), cr.size() * sizeof(double), hipMemcpyHostToDevice, s));
     DEAC_HIP(hipMemcpyAsync(h->f_old, fw.data(), fw.size() * sizeof(double), hipMemcpyHostToDevice, s));
     DEAC_HIP(hipMemcpyAsync(h->state, rng.s, sizeof rng.s, hipMemcpyHostToDevice, s));
-    const int zero3[3] = {0, 0, 0};
-    DEAC_HIP(hipMemcpyAsync(h->status, zero3, sizeof zero3, hipMemcpyHostToDevice, s));
 
     deac::EvoArgs ea{};
     ea.P = static_cast<int>(P);
@@ -316,25 +336,23 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
     ea.f_old = h->f_old;
     ea.emod = h->emod;
     ea.mom = h->mom;
-    ea.ctl = h->ctl;
-    ea.raw = h->raw;
     ea.stats = stats;
     ea.stats_len = N;
-    ea.bar = h->bar;
-    ea.status = h->status;
 
-    // Initial fitness (deac.cpp:857-976).
+    // Initial fitness (deac.cpp:857-976), on buffer set 1; its status goes
+    // to host slot 2.
     ea.init_mode = 1;
     ea.gen0 = 0;
     ea.ngen = 0;
-    DEAC_HIP(deac::launch_zero_u32(h->bar, 1, s));
+    ea.bar = h->sync + 8;
+    ea.status = reinterpret_cast<int *>(h->sync + 12);
+    DEAC_HIP(deac::launch_zero_u32(h->sync + 8, 8, s));
     DEAC_HIP(deac::launch_evolve(ea, h->nwg, s));
+    DEAC_HIP(hipMemcpyAsync(h->host_status + 8, h->sync + 12, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
     ea.init_mode = 0;
 
     deac::WalkArgs wa{};
     wa.state = h->state;
-    wa.gen_start = h->gen_start;
-    wa.ctl = h->ctl;
     wa.skip_rows = h->skip_rows;
     wa.P = static_cast<int>(P);
     wa.nsign = nsign;
@@ -342,59 +360,121 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
     wa.tau_cr = m.tau_cr;
     wa.tau_f = m.tau_f;
     deac::MaskArgs ma{};
-    ma.ctl = h->ctl;
     ma.lane_tab = h->lane_tab;
-    ma.raw = h->raw;
     ma.P = static_cast<int>(P);
     ma.Kc = static_cast<int>(Kc);
     ma.chunk = h->chunk;
 
-    // Generation loop: ii = 0 .. N-2 (deac.cpp:1138), in batches.
+    // Generation loop: ii = 0 .. N-2 (deac.cpp:1138), in batches.  Batch b
+    // uses buffer set b%2.  Its walk runs on the draw stream while batch b-1
+    // evolves (on a CU of its own: the walk kernel holds enough LDS that it
+    // cannot share one with an evolution workgroup), then its mask draws and
+    // its evolution run on the evolution stream; the mask kernel spreads over
+    // every CU and costs less alone than beside the evolution.  The evolution
+    // of b is queued before the host has read batch b-1's status.  Behind a
+    // stop it changes nothing: the stop test sits at the top of a generation,
+    // so the fitnesses it finds are those that stopped batch b-1, and it
+    // stops at its first generation (its status and its statistics entry,
+    // past the stop, are never read).  Batches start at 8
+    // generations and double up to the buffer size, so that the first walk,
+    // which nothing hides, is short.  When the evolution fills every CU the
+    // walk runs in line on the evolution stream.
+    const size_t total = N - 1;
+    std::vector<std::pair<size_t, int>> plan;  // first generation, generations
+    for (size_t g = 0, len = std::min(8, h->batch); g < total;) {
+        const int nb = static_cast<int>(std::min(len, total - g));
+        plan.emplace_back(g, nb);
+        g += static_cast<size_t>(nb);
+        len = std::min(2 * len, static_cast<size_t>(h->batch));
+    }
+    hipStream_t d = h->overlap ? h->draw_stream : s;
     DEAC_HIP(hipEventRecord(h->ev0, s));
+    DEAC_HIP(hipStreamWaitEvent(d, h->ev0, 0));  // RNG state uploaded
+    auto draws = [&](size_t b) -> deac_status {
+        const int k = static_cast<int>(b & 1);
+        hipEvent_t *e = h->kev[k];
+        if (b >= 2) DEAC_HIP(hipStreamWaitEvent(d, e[5], 0));  // batch b-2 done with the set
+        wa.gen_start = h->gen_start[k];
+        wa.ctl = h->ctl[k];
+        wa.ngen = plan[b].second;
+        DEAC_HIP(hipEventRecord(e[0], d));
+        DEAC_HIP(deac::launch_walk(wa, d));
+        DEAC_HIP(hipEventRecord(e[1], d));
+        return DEAC_OK;
+    };
+    auto evolve = [&](size_t b) -> deac_status {
+        const int k = static_cast<int>(b & 1);
+        hipEvent_t *e = h->kev[k];
+        unsigned *sy = h->sync + 8 * k;
+        ea.ctl = h->ctl[k];
+        ea.raw = h->raw[k];
+        ea.bar = sy;
+        ea.status = reinterpret_cast<int *>(sy + 4);
+        ea.gen0 = static_cast<long>(plan[b].first);
+        ea.ngen = plan[b].second;
+        ma.ctl = h->ctl[k];
+        ma.raw = h->raw[k];
+        ma.ngen = plan[b].second;
+        DEAC_HIP(hipStreamWaitEvent(s, e[1], 0));  // walk done
+        DEAC_HIP(hipEventRecord(e[2], s));
+        DEAC_HIP(deac::launch_mask(ma, s));
+        DEAC_HIP(hipEventRecord(e[3], s));
+        DEAC_HIP(deac::launch_zero_u32(sy, 8, s));
+        DEAC_HIP(hipEventRecord(e[4], s));
+        DEAC_HIP(deac::launch_evolve(ea, h->nwg, s));
+        DEAC_HIP(hipEventRecord(e[5], s));
+        DEAC_HIP(hipMemcpyAsync(h->host_status + 4 * k, sy + 4, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+        DEAC_HIP(hipEventRecord(e[6], s));
+        return DEAC_OK;
+    };
     deac_run_profile prof{};
     size_t done = 0;          // generations evolved
     long stopped_at = -1;     // generation index of the stop, if any
-    const size_t total = N - 1;
-    while (done < total) {
-        const int nb = static_cast<int>(std::min<size_t>(h->batch, total - done));
-        wa.ngen = nb;
-        ma.ngen = nb;
-        ea.gen0 = static_cast<long>(done);
-        ea.ngen = nb;
-        DEAC_HIP(hipEventRecord(h->kev[0], s));
-        DEAC_HIP(deac::launch_walk(wa, s));
-        DEAC_HIP(hipEventRecord(h->kev[1], s));
-        DEAC_HIP(deac::launch_mask(ma, s));
-        DEAC_HIP(hipEventRecord(h->kev[2], s));
-        DEAC_HIP(deac::launch_zero_u32(h->bar, 1, s));
-        DEAC_HIP(hipEventRecord(h->kev[3], s));
-        DEAC_HIP(deac::launch_evolve(ea, h->nwg, s));
-        DEAC_HIP(hipEventRecord(h->kev[4], s));
-        int status[3];
-        DEAC_HIP(hipMemcpyAsync(status, h->status, sizeof status, hipMemcpyDeviceToHost, s));
-        DEAC_HIP(hipStreamSynchronize(s));
-        float t_walk = 0, t_mask = 0, t_evo = 0;
-        DEAC_HIP(hipEventElapsedTime(&t_walk, h->kev[0], h->kev[1]));
-        DEAC_HIP(hipEventElapsedTime(&t_mask, h->kev[1], h->kev[2]));
-        DEAC_HIP(hipEventElapsedTime(&t_evo, h->kev[3], h->kev[4]));
-        prof.walk_ms += t_walk;
-        prof.mask_ms += t_mask;
-        prof.evolve_ms += t_evo;
-        prof.evolve_launches += 1;
-        if (status[2]) return DEAC_ERR_DEVICE;  // a grid barrier timed out
-        if (status[0]) {
-            stopped_at = static_cast<long>(done) + status[1];
-            uint64_t st4[4];
-            DEAC_HIP(hipMemcpy(st4, h->gen_start + static_cast<size_t>(status[1]) * 4, sizeof st4,
-                               hipMemcpyDeviceToHost));
-            std::memcpy(rng.s, st4, sizeof st4);
-            done += static_cast<size_t>(status[1]);
-            break;
+    auto pipeline = [&]() -> deac_status {
+        DEAC_TRY(draws(0));
+        if (plan.size() > 1) DEAC_TRY(draws(1));
+        DEAC_TRY(evolve(0));
+        for (size_t b = 0; b < plan.size(); b++) {
+            if (b + 1 < plan.size()) DEAC_TRY(evolve(b + 1));
+            const int k = static_cast<int>(b & 1);
+            hipEvent_t *e = h->kev[k];
+            DEAC_HIP(hipEventSynchronize(e[6]));
+            if (b == 0 && h->host_status[8 + 2]) return DEAC_ERR_DEVICE;  // initial fitness: barrier timeout
+            float t_walk = 0, t_mask = 0, t_evo = 0;
+            DEAC_HIP(hipEventElapsedTime(&t_walk, e[0], e[1]));
+            DEAC_HIP(hipEventElapsedTime(&t_mask, e[2], e[3]));
+            DEAC_HIP(hipEventElapsedTime(&t_evo, e[4], e[5]));
+            prof.walk_ms += t_walk;
+            prof.mask_ms += t_mask;
+            prof.evolve_ms += t_evo;
+            prof.evolve_launches += 1;
+            const int *status = h->host_status + 4 * k;
+            if (status[2]) return DEAC_ERR_DEVICE;  // a grid barrier timed out
+            if (status[0]) {
+                // RNG state at the top of the stopping generation; the set is
+                // not rewritten before this copy (draws of b+2 not queued)
+                stopped_at = static_cast<long>(plan[b].first) + status[1];
+                uint64_t st4[4];
+                DEAC_HIP(hipMemcpy(st4, h->gen_start[k] + static_cast<size_t>(status[1]) * 4, sizeof st4,
+                                   hipMemcpyDeviceToHost));
+                std::memcpy(rng.s, st4, sizeof st4);
+                done += static_cast<size_t>(status[1]);
+                return DEAC_OK;
+            }
+            done += static_cast<size_t>(plan[b].second);
+            if (b + 2 < plan.size()) DEAC_TRY(draws(b + 2));
         }
-        done += static_cast<size_t>(nb);
+        return DEAC_OK;
+    };
+    const deac_status ps = pipeline();
+    // nothing stays in flight past this call, whatever the outcome
+    const bool drained = hipEventRecord(h->ev1, s) == hipSuccess && hipStreamSynchronize(s) == hipSuccess &&
+                         hipStreamSynchronize(d) == hipSuccess;
+    if (ps != DEAC_OK) return ps;
+    if (!drained) {
+        (void)hipGetLastError();
+        return DEAC_ERR_DEVICE;
     }
-    DEAC_HIP(hipEventRecord(h->ev1, s));
-    DEAC_HIP(hipEventSynchronize(h->ev1));
     float ms = 0.0f;
     DEAC_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     prof.loop_ms = ms;

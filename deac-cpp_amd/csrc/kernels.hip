@@ -568,9 +568,16 @@ __global__ void zero_u32_kernel(unsigned *p, size_t n)
 
 // ---------------------------------------------------------------- launchers
 
+// LDS the walk kernel holds beyond its own table: with it the walk never
+// shares a CU (160 KiB of LDS) with an evolution workgroup, whose FP64 MFMA
+// chains would otherwise lose issue cycles to the walk's VALU work.
+constexpr size_t kWalkLdsReserve = 32 * 1024;
+static_assert(kChainLds * sizeof(double) + sizeof(uint64_t) * 256 * 4 + kWalkLdsReserve > 160 * 1024,
+              "the walk kernel must not fit on a CU beside an evolution workgroup");
+
 hipError_t launch_walk(const WalkArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(walk_kernel, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(walk_kernel, dim3(1), dim3(64), kWalkLdsReserve, s, a);
     return hipGetLastError();
 }
 

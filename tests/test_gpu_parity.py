@@ -102,6 +102,33 @@ def test_rng_continuation_matches_oracle():
     assert got["generation"] == o["generation"] == case["meta"]["generation"]
 
 
+@pytest.mark.parametrize("target", [0, 5, 8, 22, 119, 120])
+def test_stop_at_batch_edges_matches_oracle(target):
+    # the engine evolves batches of 8, 16, 32, 64, ... generations and queues
+    # the next batch before it has read the current one's stop flag; stops on
+    # the first generation (0, 8, 120), the last (119) and inside a batch (5,
+    # 22) give the oracle's generation, RNG state, statistics and DSF.  The
+    # thresholds are the oracle's best fitness where it drops at `target`.
+    print(f"[gpu] stop at generation {target}", flush=True)
+    case = gc.load("smoke_stop")
+    a = case["meta"]["args"]
+    freq = gc.frequency_of(case, ob.frequency_grid)
+
+    def oracle(stop):
+        return ob.run(ob.make_problem(case["tau"], case["isf"], case["err"], freq, a["T"], number_of_generations=400,
+                                      stop_minimum_fitness=stop, **gc.problem_kwargs(case)), a["seed"])
+
+    m = oracle(0.0)["minimum"]
+    assert target == 0 or m[target] < m[target - 1]
+    o = oracle(m[target])
+    assert o["generation"] == target
+    got = engine_run(case, N=400, stop=m[target])
+    assert got["generation"] == target
+    assert got["rng"] == o["rng"]
+    for k in ("minimum", "mean", "squared_mean", "dsf", "frequency"):
+        assert gc.bit_diff(got[k], o[k]) is None, k
+
+
 def test_full_size_properties_config2():
     # config 2 at its full genome: best fitness never increases, the DSF is
     # non-negative and mirrored, and the reported fitness is the reduced
