@@ -167,10 +167,7 @@ extern "C" deac_status deac_model_isf(const deac_problem *p, const double *genom
         if (!buf.alloc(dA, A.size()) || !buf.alloc(dB, B.size()) || !buf.alloc(dE, E.size())) return DEAC_ERR_OUT_OF_MEMORY;
         PREC_HIP(hipMemcpyAsync(dA, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, s));
         PREC_HIP(hipMemcpyAsync(dB, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice, s));
-        PREC_HIP(hipStreamSynchronize(s));  // uploads done: the events time the kernels only
-        PREC_HIP(hipEventRecord(e0, s));
-        PREC_HIP(deac::launch_ks_fp64(dA, dB, dE, T, N, Mt, Nt, static_cast<int>(Kb), s));
-        PREC_HIP(hipEventRecord(e1, s));
+        PREC_HIP(deac::launch_ks_fp64(dA, dB, dE, T, N, Mt, Nt, static_cast<int>(Kb), s, e0, e1));
         PREC_HIP(hipMemcpyAsync(E.data(), dE, E.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         PREC_HIP(hipStreamSynchronize(s));
     } else {
@@ -223,9 +220,9 @@ extern "C" deac_status deac_model_isf(const deac_problem *p, const double *genom
             if (!buf.alloc(dA, asz) || !buf.alloc(dB, bsz)) return DEAC_ERR_OUT_OF_MEMORY;
             PREC_HIP(hipMemcpyAsync(dA, A.data(), asz * sizeof(float), hipMemcpyHostToDevice, s));
             PREC_HIP(hipMemcpyAsync(dB, B.data(), bsz * sizeof(float), hipMemcpyHostToDevice, s));
-            PREC_HIP(hipStreamSynchronize(s));  // uploads done: the events time the kernels only
-        PREC_HIP(hipEventRecord(e0, s));
-            PREC_HIP(deac::launch_ks_fp32(dA, dB, part, Mt, Nt, static_cast<int>(Kb), kb_per_slice, slices, s));
+            PREC_HIP(deac::launch_ks_fp32(dA, dB, part, Mt, Nt, static_cast<int>(Kb), kb_per_slice, slices, s, e0,
+                                          nullptr));
+            PREC_HIP(deac::launch_reduce_slices(part, out, slices, Mpad * Npad, s, nullptr, e1));
         } else {
             // x = hi + lo: hi = bf16(x), lo = bf16(x - hi) (the remainder in FP32)
             std::vector<uint16_t> Ah(asz, 0), Al(asz, 0), Bh(bsz, 0), Bl(bsz, 0);
@@ -263,12 +260,10 @@ extern "C" deac_status deac_model_isf(const deac_problem *p, const double *genom
             PREC_HIP(hipMemcpyAsync(dAl, Al.data(), asz * 2, hipMemcpyHostToDevice, s));
             PREC_HIP(hipMemcpyAsync(dBh, Bh.data(), bsz * 2, hipMemcpyHostToDevice, s));
             PREC_HIP(hipMemcpyAsync(dBl, Bl.data(), bsz * 2, hipMemcpyHostToDevice, s));
-            PREC_HIP(hipStreamSynchronize(s));  // uploads done: the events time the kernels only
-        PREC_HIP(hipEventRecord(e0, s));
-            PREC_HIP(deac::launch_ks_bf16x3(dAh, dAl, dBh, dBl, part, Mt, Nt, static_cast<int>(Kb), kb_per_slice, slices, s));
+            PREC_HIP(deac::launch_ks_bf16x3(dAh, dAl, dBh, dBl, part, Mt, Nt, static_cast<int>(Kb), kb_per_slice,
+                                            slices, s, e0, nullptr));
+            PREC_HIP(deac::launch_reduce_slices(part, out, slices, Mpad * Npad, s, nullptr, e1));
         }
-        PREC_HIP(deac::launch_reduce_slices(part, out, slices, Mpad * Npad, s));
-        PREC_HIP(hipEventRecord(e1, s));
         std::vector<float> E32(Mpad * Npad);
         PREC_HIP(hipMemcpyAsync(E32.data(), out, E32.size() * sizeof(float), hipMemcpyDeviceToHost, s));
         PREC_HIP(hipStreamSynchronize(s));

@@ -20,6 +20,7 @@
 //                          B likewise with S.
 // Outputs: FP64 E[n][t] directly; FP32/BF16 partial tiles [slice][Mpad][Npad]
 // reduced in slice order by reduce_slices.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -160,33 +161,36 @@ __global__ __launch_bounds__(256) void reduce_slices_kernel(const float *__restr
 }  // namespace
 
 hipError_t launch_ks_fp64(const double *A, const double *B, double *E, int T, int N, int Mt, int Nt, int Kb,
-                          hipStream_t s)
+                          hipStream_t s, hipEvent_t start, hipEvent_t stop)
 {
-    hipLaunchKernelGGL(ks_fp64_kernel, dim3(Mt, (Nt + 3) / 4), dim3(256), 0, s, A, B, E, T, N, Mt, Nt, Kb);
+    hipExtLaunchKernelGGL(ks_fp64_kernel, dim3(Mt, (Nt + 3) / 4), dim3(256), 0, s, start, stop, 0, A, B, E, T, N, Mt,
+                          Nt, Kb);
     return hipGetLastError();
 }
 
 hipError_t launch_ks_fp32(const float *A, const float *B, float *part, int Mt, int Nt, int Kb, int kb_per_slice,
-                          int slices, hipStream_t s)
+                          int slices, hipStream_t s, hipEvent_t start, hipEvent_t stop)
 {
-    hipLaunchKernelGGL(ks_fp32_kernel, dim3(Mt, (Nt + 3) / 4, slices), dim3(256), 0, s, A, B, part, Mt, Nt, Kb,
-                       kb_per_slice);
+    hipExtLaunchKernelGGL(ks_fp32_kernel, dim3(Mt, (Nt + 3) / 4, slices), dim3(256), 0, s, start, stop, 0, A, B, part,
+                          Mt, Nt, Kb, kb_per_slice);
     return hipGetLastError();
 }
 
 hipError_t launch_ks_bf16x3(const void *Ah, const void *Al, const void *Bh, const void *Bl, float *part, int Mt,
-                            int Nt, int Kb, int kb_per_slice, int slices, hipStream_t s)
+                            int Nt, int Kb, int kb_per_slice, int slices, hipStream_t s, hipEvent_t start,
+                            hipEvent_t stop)
 {
-    hipLaunchKernelGGL(ks_bf16x3_kernel, dim3(Mt, (Nt + 3) / 4, slices), dim3(256), 0, s, static_cast<const bf8 *>(Ah),
-                       static_cast<const bf8 *>(Al), static_cast<const bf8 *>(Bh), static_cast<const bf8 *>(Bl), part,
-                       Mt, Nt, Kb, kb_per_slice);
+    hipExtLaunchKernelGGL(ks_bf16x3_kernel, dim3(Mt, (Nt + 3) / 4, slices), dim3(256), 0, s, start, stop, 0,
+                          static_cast<const bf8 *>(Ah), static_cast<const bf8 *>(Al), static_cast<const bf8 *>(Bh),
+                          static_cast<const bf8 *>(Bl), part, Mt, Nt, Kb, kb_per_slice);
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_slices(const float *part, float *out, int slices, size_t mn, hipStream_t s)
+hipError_t launch_reduce_slices(const float *part, float *out, int slices, size_t mn, hipStream_t s, hipEvent_t start,
+                                hipEvent_t stop)
 {
     const unsigned blocks = static_cast<unsigned>((mn + 255) / 256);
-    hipLaunchKernelGGL(reduce_slices_kernel, dim3(blocks), dim3(256), 0, s, part, out, slices, mn);
+    hipExtLaunchKernelGGL(reduce_slices_kernel, dim3(blocks), dim3(256), 0, s, start, stop, 0, part, out, slices, mn);
     return hipGetLastError();
 }
 

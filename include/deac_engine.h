@@ -194,7 +194,9 @@ deac_status deac_batch_last_profile(const deac_batch *batch, deac_run_profile *p
  * frequencies then negative, the population layout of deac.cpp:430-458).
  * Outputs (each may be NULL): model_isf, count x T; chi2, count reduced
  * chi^2 values (deac.cpp:120-126, 1540-1542; FP64 in the reference's order,
- * no moment terms); kernel_ms, device time of the contraction. */
+ * no moment terms); kernel_ms, device time of the contraction, from the
+ * start of its kernel to the end of the last one (for FP32 and BF16X3, the
+ * FP32 sum over genome slices), host launch latency excluded. */
 typedef enum deac_precision {
     DEAC_PRECISION_FP64 = 0,
     DEAC_PRECISION_FP32 = 1,
