@@ -368,6 +368,11 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
                               "dsf and frequency outputs are required";
             return DEAC_ERR_INVALID_ARGUMENT;
         }
+        if (options[i].ks_precision != DEAC_PRECISION_FP64 || options[i].split_loop) {
+            g_batch_message = "instance " + std::to_string(i) + ": the batch kernel evaluates K.S in exact FP64 "
+                              "(ks_precision 0, no split loop); run mixed precision through deac_run";
+            return DEAC_ERR_UNSUPPORTED;
+        }
     }
     BATCH_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;

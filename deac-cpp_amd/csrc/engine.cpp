@@ -12,6 +12,9 @@
 #include "deac_engine.h"
 #include "host_model.hpp"
 #include "kernels.hpp"
+#include "ks_layout.hpp"
+#include "precision_kernels.hpp"
+#include "split_kernels.hpp"
 
 namespace {
 
@@ -48,6 +51,15 @@ struct deac_handle {
     // evolve end (draws free), status copied
     hipEvent_t kev[2][7] = {};
     deac_run_profile last{};
+    // Split loop: contraction operands of the precision it last ran at.
+    struct Split {
+        int precision = -1;
+        deac::KsLayout y;
+        void *A = nullptr, *Al = nullptr;  // kernel matrix (BF16X3: hi, lo)
+        void *B = nullptr, *Bl = nullptr;  // genome operand
+        float *part = nullptr;             // FP32 genome slices
+        int *ex = nullptr;                 // per-row exponents
+    } split;
 
     ~deac_handle()
     {
@@ -72,6 +84,14 @@ struct deac_handle {
         allocations.push_back(p);
         out = static_cast<T *>(p);
         return DEAC_OK;
+    }
+
+    void release(void *&p)
+    {
+        if (!p) return;
+        allocations.erase(std::find(allocations.begin(), allocations.end(), p));
+        (void)hipFree(p);
+        p = nullptr;
     }
 };
 
@@ -98,6 +118,65 @@ void pad_rows(const std::vector<double> &pop, size_t P, size_t Kc, int MG, int s
 {
     out.assign(static_cast<size_t>(MG) * deac::kChainRows * stride, 0.0);
     for (size_t i = 0; i < P; i++) std::memcpy(&out[i * stride], &pop[i * Kc], Kc * sizeof(double));
+}
+
+// Contraction operands of the split loop at `precision` (kept until another
+// precision is asked for): the kernel matrix in the contraction's lane order,
+// the genome operand and slice buffers for the population.
+deac_status prepare_split(deac_handle *h, int precision)
+{
+    deac_handle::Split &sp = h->split;
+    if (sp.precision == precision) return DEAC_OK;
+    for (void **p : {&sp.A, &sp.Al, &sp.B, &sp.Bl, reinterpret_cast<void **>(&sp.part),
+                     reinterpret_cast<void **>(&sp.ex)})
+        h->release(*p);
+    sp.precision = -1;
+    const deac::Model &m = h->m;
+    sp.y = deac::ks_layout(m, precision, m.P);
+    const deac::KsLayout &y = sp.y;
+    auto up = [](void *dst, const void *src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? DEAC_OK : DEAC_ERR_DEVICE;
+    };
+    if (precision == DEAC_PRECISION_FP64) {
+        std::vector<double> A;
+        deac::ks_kernel_fp64(m, y, A);
+        double *a = nullptr, *b = nullptr;
+        DEAC_TRY(h->alloc(a, A.size()));
+        sp.A = a;
+        DEAC_TRY(h->alloc(b, y.b_count()));
+        sp.B = b;
+        DEAC_TRY(up(a, A.data(), A.size() * sizeof(double)));
+    } else if (precision == DEAC_PRECISION_FP32) {
+        std::vector<float> A;
+        deac::ks_kernel_fp32(m, y, A);
+        float *a = nullptr, *b = nullptr;
+        DEAC_TRY(h->alloc(a, A.size()));
+        sp.A = a;
+        DEAC_TRY(h->alloc(b, y.b_count()));
+        sp.B = b;
+        DEAC_TRY(up(a, A.data(), A.size() * sizeof(float)));
+    } else {
+        std::vector<uint16_t> Ah, Al;
+        deac::ks_kernel_bf16x3(m, y, Ah, Al);
+        uint16_t *ah = nullptr, *al = nullptr, *bh = nullptr, *bl = nullptr;
+        DEAC_TRY(h->alloc(ah, Ah.size()));
+        sp.A = ah;
+        DEAC_TRY(h->alloc(al, Al.size()));
+        sp.Al = al;
+        DEAC_TRY(h->alloc(bh, y.b_count()));
+        sp.B = bh;
+        DEAC_TRY(h->alloc(bl, y.b_count()));
+        sp.Bl = bl;
+        DEAC_TRY(up(ah, Ah.data(), Ah.size() * sizeof(uint16_t)));
+        DEAC_TRY(up(al, Al.data(), Al.size() * sizeof(uint16_t)));
+    }
+    if (precision != DEAC_PRECISION_FP64) {
+        DEAC_TRY(h->alloc(sp.part, static_cast<size_t>(y.slices) * y.mpad() * y.npad()));
+        DEAC_TRY(h->alloc(sp.ex, y.T));
+        DEAC_TRY(up(sp.ex, y.ex.data(), y.T * sizeof(int)));
+    }
+    sp.precision = precision;
+    return DEAC_OK;
 }
 
 }  // namespace
@@ -276,11 +355,18 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         g_message = "number_of_generations must be at least 2";
         return DEAC_ERR_INVALID_ARGUMENT;
     }
+    if (o->ks_precision != DEAC_PRECISION_FP64 && o->ks_precision != DEAC_PRECISION_FP32 &&
+        o->ks_precision != DEAC_PRECISION_BF16X3) {
+        g_message = "ks_precision must be DEAC_PRECISION_FP64, DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3";
+        return DEAC_ERR_INVALID_ARGUMENT;
+    }
+    const bool split = o->ks_precision != DEAC_PRECISION_FP64 || o->split_loop != 0;
     const deac::Model &m = h->m;
     const size_t N = o->number_of_generations, P = m.P, Kc = m.Kc;
     const int nsign = m.nsign;
     hipStream_t s = h->stream;
     DEAC_HIP(hipSetDevice(h->device));
+    if (split) DEAC_TRY(prepare_split(h, o->ks_precision));
 
     // Initial population and normalization on the host (deac.cpp:430-603).
     deac::Xoshiro rng{{rng_state[0], rng_state[1], rng_state[2], rng_state[3]}};
@@ -339,18 +425,6 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
     ea.stats = stats;
     ea.stats_len = N;
 
-    // Initial fitness (deac.cpp:857-976), on buffer set 1; its status goes
-    // to host slot 2.
-    ea.init_mode = 1;
-    ea.gen0 = 0;
-    ea.ngen = 0;
-    ea.bar = h->sync + 8;
-    ea.status = reinterpret_cast<int *>(h->sync + 12);
-    DEAC_HIP(deac::launch_zero_u32(h->sync + 8, 8, s));
-    DEAC_HIP(deac::launch_evolve(ea, h->nwg, s));
-    DEAC_HIP(hipMemcpyAsync(h->host_status + 8, h->sync + 12, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-    ea.init_mode = 0;
-
     deac::WalkArgs wa{};
     wa.state = h->state;
     wa.skip_rows = h->skip_rows;
@@ -387,9 +461,7 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         g += static_cast<size_t>(nb);
         len = std::min(2 * len, static_cast<size_t>(h->batch));
     }
-    hipStream_t d = h->overlap ? h->draw_stream : s;
-    DEAC_HIP(hipEventRecord(h->ev0, s));
-    DEAC_HIP(hipStreamWaitEvent(d, h->ev0, 0));  // RNG state uploaded
+    hipStream_t d = split || !h->overlap ? s : h->draw_stream;
     auto draws = [&](size_t b) -> deac_status {
         const int k = static_cast<int>(b & 1);
         hipEvent_t *e = h->kev[k];
@@ -431,6 +503,19 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
     size_t done = 0;          // generations evolved
     long stopped_at = -1;     // generation index of the stop, if any
     auto pipeline = [&]() -> deac_status {
+        // Initial fitness (deac.cpp:857-976), on buffer set 1; its status
+        // goes to host slot 2.
+        ea.init_mode = 1;
+        ea.gen0 = 0;
+        ea.ngen = 0;
+        ea.bar = h->sync + 8;
+        ea.status = reinterpret_cast<int *>(h->sync + 12);
+        DEAC_HIP(deac::launch_zero_u32(h->sync + 8, 8, s));
+        DEAC_HIP(deac::launch_evolve(ea, h->nwg, s));
+        DEAC_HIP(hipMemcpyAsync(h->host_status + 8, h->sync + 12, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+        ea.init_mode = 0;
+        DEAC_HIP(hipEventRecord(h->ev0, s));
+        DEAC_HIP(hipStreamWaitEvent(d, h->ev0, 0));  // RNG state uploaded
         DEAC_TRY(draws(0));
         if (plan.size() > 1) DEAC_TRY(draws(1));
         DEAC_TRY(evolve(0));
@@ -466,7 +551,100 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         }
         return DEAC_OK;
     };
-    const deac_status ps = pipeline();
+    // The split loop: the same generations as separate kernels on one
+    // stream, batch by batch (walk and mask draws, then per generation top,
+    // trial, renormalization, the contraction at the chosen precision,
+    // moments and selection), the status read after each batch.
+    auto split_loop = [&]() -> deac_status {
+        const deac_handle::Split &sp = h->split;
+        const deac::KsLayout &y = sp.y;
+        const int Kb = static_cast<int>(y.Kb);
+        ea.ctl = h->ctl[0];
+        ea.raw = h->raw[0];
+        ea.bar = h->sync;
+        ea.status = reinterpret_cast<int *>(h->sync + 4);
+        const int *halt = ea.status;
+        // model ISF (and moment rows) of the contraction's genomes: the
+        // current ones for the initial fitness, the trials otherwise
+        auto evaluate = [&](int init) -> deac_status {
+            if (y.precision == DEAC_PRECISION_FP64) {
+                double *B = static_cast<double *>(sp.B);
+                DEAC_HIP(deac::launch_split_pack_fp64(ea, init, B, y.Nt, Kb, s));
+                DEAC_HIP(deac::launch_ks_fp64(static_cast<const double *>(sp.A), B, ea.emod, ea.T, ea.P, y.Mt, y.Nt,
+                                              Kb, s, nullptr, nullptr, halt));
+            } else {
+                if (y.precision == DEAC_PRECISION_FP32) {
+                    float *B = static_cast<float *>(sp.B);
+                    DEAC_HIP(deac::launch_split_pack_fp32(ea, init, B, y.Nt, Kb, s));
+                    DEAC_HIP(deac::launch_ks_fp32(static_cast<const float *>(sp.A), B, sp.part, y.Mt, y.Nt, Kb,
+                                                  y.kb_per_slice, y.slices, s, nullptr, nullptr, halt));
+                } else {
+                    DEAC_HIP(deac::launch_split_pack_bf16x3(ea, init, sp.B, sp.Bl, y.Nt, Kb, s));
+                    DEAC_HIP(deac::launch_ks_bf16x3(sp.A, sp.Al, sp.B, sp.Bl, sp.part, y.Mt, y.Nt, Kb, y.kb_per_slice,
+                                                    y.slices, s, nullptr, nullptr, halt));
+                }
+                DEAC_HIP(deac::launch_split_model(ea, sp.part, y.slices, static_cast<int>(y.npad()),
+                                                  y.mpad() * y.npad(), sp.ex, s));
+            }
+            if (ea.use_first || ea.use_third) DEAC_HIP(deac::launch_split_moments(ea, init, s));
+            return DEAC_OK;
+        };
+        // Initial fitness (deac.cpp:857-976).
+        DEAC_HIP(deac::launch_zero_u32(h->sync, 8, s));
+        DEAC_TRY(evaluate(1));
+        DEAC_HIP(deac::launch_split_select(ea, 0, 1, s));
+        DEAC_HIP(hipEventRecord(h->ev0, s));
+        hipEvent_t *e = h->kev[0];
+        for (size_t first = 0; first < total;) {
+            const int nb = static_cast<int>(std::min<size_t>(h->batch, total - first));
+            wa.gen_start = h->gen_start[0];
+            wa.ctl = h->ctl[0];
+            wa.ngen = nb;
+            ma.ctl = h->ctl[0];
+            ma.raw = h->raw[0];
+            ma.ngen = nb;
+            ea.gen0 = static_cast<long>(first);
+            ea.ngen = nb;
+            DEAC_HIP(hipEventRecord(e[0], s));
+            DEAC_HIP(deac::launch_walk(wa, s));
+            DEAC_HIP(hipEventRecord(e[1], s));
+            DEAC_HIP(deac::launch_mask(ma, s));
+            DEAC_HIP(hipEventRecord(e[2], s));
+            for (int g = 0; g < nb; g++) {
+                DEAC_HIP(deac::launch_split_top(ea, g, s));
+                DEAC_HIP(deac::launch_split_trial(ea, g, s));
+                if (ea.renorm) DEAC_HIP(deac::launch_split_renorm(ea, s));
+                DEAC_TRY(evaluate(0));
+                DEAC_HIP(deac::launch_split_select(ea, g, 0, s));
+            }
+            DEAC_HIP(hipEventRecord(e[3], s));
+            DEAC_HIP(hipMemcpyAsync(h->host_status, h->sync + 4, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+            DEAC_HIP(hipEventRecord(e[4], s));
+            DEAC_HIP(hipEventSynchronize(e[4]));
+            float t_walk = 0, t_mask = 0, t_evo = 0;
+            DEAC_HIP(hipEventElapsedTime(&t_walk, e[0], e[1]));
+            DEAC_HIP(hipEventElapsedTime(&t_mask, e[1], e[2]));
+            DEAC_HIP(hipEventElapsedTime(&t_evo, e[2], e[3]));
+            prof.walk_ms += t_walk;
+            prof.mask_ms += t_mask;
+            prof.evolve_ms += t_evo;
+            prof.evolve_launches += 1;
+            const int *status = h->host_status;
+            if (status[0]) {
+                stopped_at = static_cast<long>(first) + status[1];
+                uint64_t st4[4];
+                DEAC_HIP(hipMemcpy(st4, h->gen_start[0] + static_cast<size_t>(status[1]) * 4, sizeof st4,
+                                   hipMemcpyDeviceToHost));
+                std::memcpy(rng.s, st4, sizeof st4);
+                done += static_cast<size_t>(status[1]);
+                return DEAC_OK;
+            }
+            done += static_cast<size_t>(nb);
+            first += static_cast<size_t>(nb);
+        }
+        return DEAC_OK;
+    };
+    const deac_status ps = split ? split_loop() : pipeline();
     // nothing stays in flight past this call, whatever the outcome
     const bool drained = hipEventRecord(h->ev1, s) == hipSuccess && hipStreamSynchronize(s) == hipSuccess &&
                          hipStreamSynchronize(d) == hipSuccess;

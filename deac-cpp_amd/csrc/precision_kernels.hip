@@ -41,8 +41,10 @@ constexpr int kDepth16 = kSweepDepth16;  // k-blocks in flight, 16x16x32 operand
 // One wave per (tau tile, genome tile); the four waves of a workgroup take
 // four genome tiles.
 __global__ __launch_bounds__(256) void ks_fp64_kernel(const double *__restrict__ A, const double *__restrict__ B,
-                                                      double *__restrict__ E, int T, int N, int Mt, int Nt, int Kb)
+                                                      double *__restrict__ E, int T, int N, int Mt, int Nt, int Kb,
+                                                      const int *halt)
 {
+    if (halt && *halt) return;
     const int lane = threadIdx.x & 63;
     const int mt = blockIdx.x, nt = blockIdx.y * 4 + static_cast<int>(threadIdx.x >> 6);
     if (mt >= Mt || nt >= Nt) return;
@@ -84,8 +86,10 @@ __device__ __forceinline__ void store_partial(float *part, int slice, int Mpad, 
 
 // grid (Mt, ceil(Nt/4), slices): one output tile per wave over one slice
 __global__ __launch_bounds__(256) void ks_fp32_kernel(const float *__restrict__ A, const float *__restrict__ B,
-                                                      float *__restrict__ part, int Mt, int Nt, int Kb, int kb_per_slice)
+                                                      float *__restrict__ part, int Mt, int Nt, int Kb, int kb_per_slice,
+                                                      const int *halt)
 {
+    if (halt && *halt) return;
     const int lane = threadIdx.x & 63;
     const int mt = blockIdx.x, nt = blockIdx.y * 4 + static_cast<int>(threadIdx.x >> 6), slice = blockIdx.z;
     if (nt >= Nt) return;
@@ -114,8 +118,9 @@ __global__ __launch_bounds__(256) void ks_fp32_kernel(const float *__restrict__ 
 __global__ __launch_bounds__(256) void ks_bf16x3_kernel(const bf8 *__restrict__ Ah, const bf8 *__restrict__ Al,
                                                         const bf8 *__restrict__ Bh, const bf8 *__restrict__ Bl,
                                                         float *__restrict__ part, int Mt, int Nt, int Kb,
-                                                        int kb_per_slice)
+                                                        int kb_per_slice, const int *halt)
 {
+    if (halt && *halt) return;
     const int lane = threadIdx.x & 63;
     const int mt = blockIdx.x, nt = blockIdx.y * 4 + static_cast<int>(threadIdx.x >> 6), slice = blockIdx.z;
     if (nt >= Nt) return;
@@ -161,28 +166,28 @@ __global__ __launch_bounds__(256) void reduce_slices_kernel(const float *__restr
 }  // namespace
 
 hipError_t launch_ks_fp64(const double *A, const double *B, double *E, int T, int N, int Mt, int Nt, int Kb,
-                          hipStream_t s, hipEvent_t start, hipEvent_t stop)
+                          hipStream_t s, hipEvent_t start, hipEvent_t stop, const int *halt)
 {
     hipExtLaunchKernelGGL(ks_fp64_kernel, dim3(Mt, (Nt + 3) / 4), dim3(256), 0, s, start, stop, 0, A, B, E, T, N, Mt,
-                          Nt, Kb);
+                          Nt, Kb, halt);
     return hipGetLastError();
 }
 
 hipError_t launch_ks_fp32(const float *A, const float *B, float *part, int Mt, int Nt, int Kb, int kb_per_slice,
-                          int slices, hipStream_t s, hipEvent_t start, hipEvent_t stop)
+                          int slices, hipStream_t s, hipEvent_t start, hipEvent_t stop, const int *halt)
 {
     hipExtLaunchKernelGGL(ks_fp32_kernel, dim3(Mt, (Nt + 3) / 4, slices), dim3(256), 0, s, start, stop, 0, A, B, part,
-                          Mt, Nt, Kb, kb_per_slice);
+                          Mt, Nt, Kb, kb_per_slice, halt);
     return hipGetLastError();
 }
 
 hipError_t launch_ks_bf16x3(const void *Ah, const void *Al, const void *Bh, const void *Bl, float *part, int Mt,
                             int Nt, int Kb, int kb_per_slice, int slices, hipStream_t s, hipEvent_t start,
-                            hipEvent_t stop)
+                            hipEvent_t stop, const int *halt)
 {
     hipExtLaunchKernelGGL(ks_bf16x3_kernel, dim3(Mt, (Nt + 3) / 4, slices), dim3(256), 0, s, start, stop, 0,
                           static_cast<const bf8 *>(Ah), static_cast<const bf8 *>(Al), static_cast<const bf8 *>(Bh),
-                          static_cast<const bf8 *>(Bl), part, Mt, Nt, Kb, kb_per_slice);
+                          static_cast<const bf8 *>(Bl), part, Mt, Nt, Kb, kb_per_slice, halt);
     return hipGetLastError();
 }
 

@@ -54,7 +54,7 @@ class Problem(C.Structure):
 
 class RunOptions(C.Structure):
     _fields_ = [("number_of_generations", C.c_size_t), ("stop_minimum_fitness", C.c_double),
-                ("track_stats", C.c_int)]
+                ("track_stats", C.c_int), ("ks_precision", C.c_int), ("split_loop", C.c_int)]
 
 
 class Result(C.Structure):
@@ -193,8 +193,13 @@ class Engine:
         except Exception:
             pass
 
-    def run(self, seed, number_of_generations, stop_minimum_fitness=1.0, track_stats=True):
-        """Run from --seed (int) or an explicit 4-word RNG state."""
+    def run(self, seed, number_of_generations, stop_minimum_fitness=1.0, track_stats=True, precision="fp64",
+            split_loop=False):
+        """Run from --seed (int) or an explicit 4-word RNG state.
+
+        precision: K.S of the model ISF, "fp64" (exact), "fp32" or "bf16x3"
+        (mixed, through the split generation loop); split_loop=True runs
+        fp64 through the split loop as well."""
         p = self.problem
         N = number_of_generations
         nout = lib().deac_output_length(C.byref(p))
@@ -206,7 +211,7 @@ class Engine:
             s = seed_state(seed)
         else:
             s = (C.c_uint64 * 4)(*[int(v) for v in seed])
-        o = RunOptions(N, stop_minimum_fitness, int(track_stats))
+        o = RunOptions(N, stop_minimum_fitness, int(track_stats), PRECISIONS[precision], int(split_loop))
         st = lib().deac_run(self.handle, s, C.byref(o), C.byref(r))
         if st:
             raise DeacError(st)

@@ -80,10 +80,33 @@ typedef struct deac_problem {
     double self_adapting_differential_weight_probability; /* -v, tau_F */
 } deac_problem;
 
+/* Precision of the K.S contraction of the model ISF (deac.cpp:1385-1395):
+ *   DEAC_PRECISION_FP64    exact: bit for bit the reference's FMA chains;
+ *   DEAC_PRECISION_FP32    K scaled per tau row by a power of two, K and S in
+ *                          FP32 on v_mfma_f32_16x16x4f32, FP32 accumulation;
+ *   DEAC_PRECISION_BF16X3  K and S split into bfloat16 pairs, three products
+ *                          per element on v_mfma_f32_16x16x32_bf16, FP32
+ *                          accumulation. */
+typedef enum deac_precision {
+    DEAC_PRECISION_FP64 = 0,
+    DEAC_PRECISION_FP32 = 1,
+    DEAC_PRECISION_BF16X3 = 2
+} deac_precision;
+
 typedef struct deac_run_options {
     size_t number_of_generations;         /* N >= 2; at most N-1 evolution steps */
     double stop_minimum_fitness;          /* stop when the best fitness <= this */
     int track_stats;                      /* fill the three stats arrays */
+    /* deac_run only (deac_batch_run requires FP64 and no split loop):
+     * ks_precision, a deac_precision.  FP64 (0) runs the fused exact kernel.
+     * FP32 and BF16X3 run the split generation loop, one kernel per step,
+     * with the model ISF from the mixed-precision contraction; trial vectors,
+     * chi^2, selection, statistics and constraints stay FP64 in the
+     * reference's order, and the random draws are the reference's.
+     * split_loop nonzero: FP64 runs through the split loop as well, with the
+     * exact contraction (results identical to the fused kernel). */
+    int ks_precision;
+    int split_loop;
 } deac_run_options;
 
 /* Outputs of one reconstruction.  All arrays are caller-allocated. */
@@ -179,14 +202,7 @@ deac_status deac_batch_last_profile(const deac_batch *batch, deac_run_profile *p
 /* ---- K.S precision sweep ------------------------------------------------
  * The model ISF E = K.S of a stack of genomes, the contraction of the
  * reference's fitness evaluation (deac.cpp:1385-1395, matrix_multiply_LxM_by_MxN
- * at deac.cpp:110-118), at a chosen precision of K.S:
- *   DEAC_PRECISION_FP64    the engine's exact evaluation (bit for bit the
- *                          reference's FMA chains);
- *   DEAC_PRECISION_FP32    K scaled per tau row by a power of two, K and S in
- *                          FP32 on v_mfma_f32_16x16x4f32, FP32 accumulation;
- *   DEAC_PRECISION_BF16X3  K and S split into bfloat16 pairs, three products
- *                          per element on v_mfma_f32_16x16x32_bf16, FP32
- *                          accumulation.
+ * at deac.cpp:110-118), at a chosen deac_precision of K.S (above).
  * It has no counterpart among the reference's functions: it exists to report
  * |dchi^2|/chi^2 of the mixed-precision forms against FP64.
  *
@@ -197,12 +213,6 @@ deac_status deac_batch_last_profile(const deac_batch *batch, deac_run_profile *p
  * no moment terms); kernel_ms, device time of the contraction, from the
  * start of its kernel to the end of the last one (for FP32 and BF16X3, the
  * FP32 sum over genome slices), host launch latency excluded. */
-typedef enum deac_precision {
-    DEAC_PRECISION_FP64 = 0,
-    DEAC_PRECISION_FP32 = 1,
-    DEAC_PRECISION_BF16X3 = 2
-} deac_precision;
-
 size_t deac_genome_length(const deac_problem *problem);
 
 deac_status deac_model_isf(const deac_problem *problem, const double *genomes, size_t count, int precision,
