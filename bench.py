@@ -26,6 +26,12 @@ the plain workload.
 zero-temperature kernels, population 16) for a bounded number of generations
 in one deac_batch_run, as instance-generations/s and sustained FP64 TFLOP/s.
 
+`config.precision_trajectories` runs the T=257, G=16384, P=64 hyperbolic
+workload for a bounded number of generations (--trajectory-gens) with K.S in exact
+FP64 (the fused kernel) and in FP32 and split-BF16x3 (the split loop, chi^2
+and selection in FP64), and reports each mixed run's final |dchi^2|/chi^2 and
+DSF L2 error against the FP64 run.
+
 `config.precision_sweep` evaluates K.S with deac_model_isf at FP64, FP32 and
 split-BF16x3 (chi^2 in FP64): |dchi^2|/chi^2 against FP64 for 64 genomes
 near the solution of the T=257, G=16384 hyperbolic workload (the true
@@ -167,6 +173,44 @@ def precision_sweep(da):
     return out
 
 
+def precision_trajectories(da, gens):
+    """Precision trajectories: the T=257, G=16384, P=64 hyperbolic workload run
+    for `gens` generations (seed 1) with K.S in exact FP64 (the fused kernel,
+    the reference's trajectory) and in FP32 / BF16x3 (the split loop, chi^2
+    and selection in FP64).  Each mixed run reports its final chi^2, that of
+    its DSF re-evaluated in FP64, |dchi^2|/chi^2 and the DSF L2 error against
+    the FP64 run, and its time per generation."""
+    import numpy as np
+    tau, f, s = np.fromfile(os.path.join(ROOT, "tests", "golden", "isf_c3_T257.bin")).reshape(3, -1)
+    G3, beta = 16384, 1.0 / TEMPERATURE
+    w = da.frequency_grid(G3, OMEGA_MAX)
+    prob = da.make_problem(tau, f, s, w, TEMPERATURE, model="hyperbolic", spectra_type="bfull", population_size=64)
+    out = {"workload": f"T=257, G=16384, P=64, omega_max 512, hyperbolic, bfull, seed 1, {gens} generations",
+           "runs": {}}
+    runs = {}
+    with da.Engine(prob) as e:
+        for prec in ("fp64", "fp32", "bf16x3"):
+            r = e.run(1, gens + 1, 0.0, track_stats=False, precision=prec)
+            runs[prec] = r
+            out["runs"][prec] = {"chi2": r["minimum_fitness"],
+                                 "ms_per_generation": r["profile"]["loop_ms"] / r["loop_generations"],
+                                 "path": "fused kernel, exact" if prec == "fp64" else "split loop"}
+    ref = runs["fp64"]
+    # hyperbolic bfull DSF: 0.5 S+ e^{beta w / 2} at G-1+i, 0.5 S- e^{-beta w / 2} at G-1-i
+    for prec in ("fp32", "bf16x3"):
+        r = runs[prec]
+        dsf = r["dsf"]
+        genome = np.concatenate([2.0 * dsf[G3 - 1:] / np.exp(0.5 * beta * w),
+                                 2.0 * dsf[G3 - 1::-1] / np.exp(-0.5 * beta * w)])
+        _, chi64, _ = da.model_isf(prob, genome, "fp64")
+        out["runs"][prec].update(
+            chi2_fp64_reevaluated=float(chi64[0]),
+            dchi2_rel=abs(r["minimum_fitness"] - ref["minimum_fitness"]) / ref["minimum_fitness"],
+            dchi2_rel_fp64_reevaluated=abs(float(chi64[0]) - ref["minimum_fitness"]) / ref["minimum_fitness"],
+            dsf_l2_rel=float(np.linalg.norm(dsf - ref["dsf"]) / np.linalg.norm(ref["dsf"])))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +221,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--uneven-gens", type=int, default=200, help="timed generations of the uneven batch (0: skip)")
     ap.add_argument("--no-precision-sweep", action="store_true")
+    ap.add_argument("--trajectory-gens", type=int, default=2000,
+                    help="generations of the T=257, G=16384 precision trajectories (0: skip)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -233,6 +279,7 @@ def main():
     plain_value = PLAIN["P"] * pprof["generations"] / (pprof["loop_ms"] / 1e3)
     uneven = uneven_batch(da, args.uneven_gens) if args.uneven_gens > 0 else None
     sweep = None if args.no_precision_sweep else precision_sweep(da)
+    trajectories = precision_trajectories(da, args.trajectory_gens) if args.trajectory_gens > 0 else None
 
     line = {
         "metric": METRIC,
@@ -260,7 +307,8 @@ def main():
                                   "ms_per_generation": pprof["loop_ms"] / pprof["generations"]},
                    "batched_over_plain": value / plain_value,
                    "uneven_batch": uneven,
-                   "precision_sweep": sweep},
+                   "precision_sweep": sweep,
+                   "precision_trajectories": trajectories},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "traffic_note": "HBM bytes per launch (one launch = the timed generations, up to the "
