@@ -18,8 +18,9 @@ T=65, seed 1) is timed beside it, and its ratio reported.
 
 `roofline` prices the batch kernel against MI355X FP64 matrix peak
 (4*G*T flops per genome evaluation); `cpu_baseline` times the reference CPU
-executable built from source (oracle/_ref, one core) on a bounded sample of
-the plain workload.
+executable built from source (oracle/_ref) on a bounded sample of the seed
+batch, one process per seed on the host cores at once, as the reference is
+run for many seeds.
 
 `config.uneven_batch` times the uneven batch (tests/golden/make_uneven.py:
 64 wavevectors x 16 seeds with mixed T, G, omega_max, model and
@@ -68,31 +69,40 @@ def load_isf():
 
 
 def cpu_baseline(gens):
-    """Reference CPU build on the plain workload for `gens` generations (1 core)."""
+    """Reference CPU build on the seed-batch workload, used as the reference
+    is used for many seeds: one process per seed (seeds 1..n), n = the
+    host cores available (at most 16, the GPU's share of the box), all at
+    once, `gens` generations each; aggregate genome-evals/s over the wall
+    time.  Without the reference build, the oracle (one seed, one core)."""
+    P = BATCH["P"]
     exe = os.path.join(ROOT, "oracle", "_ref", "deac_standard.e")
-    P = PLAIN["P"]
-    with tempfile.TemporaryDirectory() as d:
-        if os.path.exists(exe):
-            cmd = [exe, "-T", repr(TEMPERATURE), "-N", str(gens + 1), "-P", str(P), "-M", str(G),
-                   "--omega_max", repr(OMEGA_MAX), "--normalize", "--spectra_type", "bfull",
-                   "--stop_minimum_fitness", "0", "--seed", str(PLAIN["seed"]), "--save_directory", d,
-                   os.path.join(ROOT, "tests", "golden", ISF)]
+    isf = os.path.join(ROOT, "tests", "golden", ISF)
+    if os.path.exists(exe):
+        procs = min(BATCH["seeds"], len(os.sched_getaffinity(0)), 16)
+        with tempfile.TemporaryDirectory() as d:
+            cmds = [[exe, "-T", repr(TEMPERATURE), "-N", str(gens + 1), "-P", str(P), "-M", str(G),
+                     "--omega_max", repr(OMEGA_MAX), "--normalize", "--spectra_type", "bfull",
+                     "--stop_minimum_fitness", repr(BATCH["stop"]), "--seed", str(k), "--save_directory", d, isf]
+                    for k in range(1, procs + 1)]
             t0 = time.perf_counter()
-            subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+            running = [subprocess.Popen(c, stdout=subprocess.DEVNULL) for c in cmds]
+            codes = [r.wait() for r in running]
             dt = time.perf_counter() - t0
-            kind = "reference"
-        else:
-            import oracle_binding as ob
-            t, f, s = load_isf()
-            p = ob.make_problem(t, f, s, ob.frequency_grid(G, OMEGA_MAX), TEMPERATURE,
-                                number_of_generations=gens + 1, population_size=P, normalize=True,
-                                stop_minimum_fitness=0.0)
-            t0 = time.perf_counter()
-            ob.run(p, PLAIN["seed"])
-            dt = time.perf_counter() - t0
-            kind = "port"
-    return {"value": P * gens / dt, "unit": "genome-evals/s", "cores": 1, "kind": kind,
-            "sample": f"{gens} generations of the plain workload (G=4096, P=8, T=65, seed 1), one process, "
+        if any(codes):
+            raise RuntimeError(f"reference CPU build failed: exit codes {codes}")
+        return {"value": procs * P * gens / dt, "unit": "genome-evals/s", "cores": procs, "kind": "reference",
+                "sample": f"seeds 1..{procs} of the seed-batch workload (G=4096, P=32, T=65), one reference "
+                          f"process per seed on {procs} cores at once, {gens} generations each, wall time incl. "
+                          f"setup {dt:.2f} s"}
+    import oracle_binding as ob
+    t, f, s = load_isf()
+    p = ob.make_problem(t, f, s, ob.frequency_grid(G, OMEGA_MAX), TEMPERATURE, number_of_generations=gens + 1,
+                        population_size=P, normalize=True, stop_minimum_fitness=BATCH["stop"])
+    t0 = time.perf_counter()
+    ob.run(p, 1)
+    dt = time.perf_counter() - t0
+    return {"value": P * gens / dt, "unit": "genome-evals/s", "cores": 1, "kind": "port",
+            "sample": f"seed 1 of the seed-batch workload (G=4096, P=32, T=65), {gens} generations, one core, "
                       f"wall time incl. setup {dt:.2f} s"}
 
 
@@ -179,7 +189,8 @@ def precision_trajectories(da, gens):
     the reference's trajectory) and in FP32 / BF16x3 (the split loop, chi^2
     and selection in FP64).  Each mixed run reports its final chi^2, that of
     its DSF re-evaluated in FP64, |dchi^2|/chi^2 and the DSF L2 error against
-    the FP64 run, and its time per generation."""
+    the FP64 run, the first generation at which its best-fitness history
+    leaves the FP64 run's, and its time per generation."""
     import numpy as np
     tau, f, s = np.fromfile(os.path.join(ROOT, "tests", "golden", "isf_c3_T257.bin")).reshape(3, -1)
     G3, beta = 16384, 1.0 / TEMPERATURE
@@ -190,7 +201,7 @@ def precision_trajectories(da, gens):
     runs = {}
     with da.Engine(prob) as e:
         for prec in ("fp64", "fp32", "bf16x3"):
-            r = e.run(1, gens + 1, 0.0, track_stats=False, precision=prec)
+            r = e.run(1, gens + 1, 0.0, track_stats=True, precision=prec)
             runs[prec] = r
             out["runs"][prec] = {"chi2": r["minimum_fitness"],
                                  "ms_per_generation": r["profile"]["loop_ms"] / r["loop_generations"],
@@ -208,6 +219,13 @@ def precision_trajectories(da, gens):
             dchi2_rel=abs(r["minimum_fitness"] - ref["minimum_fitness"]) / ref["minimum_fitness"],
             dchi2_rel_fp64_reevaluated=abs(float(chi64[0]) - ref["minimum_fitness"]) / ref["minimum_fitness"],
             dsf_l2_rel=float(np.linalg.norm(dsf - ref["dsf"]) / np.linalg.norm(ref["dsf"])))
+        # first generation whose best fitness departs from the FP64 run's by
+        # more than 1e-4 relative: 100x the contraction's own error, so a
+        # different selection and not the evaluation alone
+        dev = np.abs(r["minimum"] - ref["minimum"]) / ref["minimum"]
+        far = np.nonzero(dev > 1e-4)[0]
+        out["runs"][prec]["first_divergent_generation"] = int(far[0]) if len(far) else None
+        out["runs"][prec]["best_fitness_rel_dev_generation_0"] = float(dev[0])
     return out
 
 
@@ -217,7 +235,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000, help="timed generations of the seed batch")
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=5000, help="timed generations of the plain path")
-    ap.add_argument("--cpu-gens", type=int, default=2000, help="generations of the CPU baseline sample")
+    ap.add_argument("--cpu-gens", type=int, default=150, help="generations per seed of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--uneven-gens", type=int, default=200, help="timed generations of the uneven batch (0: skip)")
     ap.add_argument("--no-precision-sweep", action="store_true")
@@ -277,6 +295,14 @@ def main():
         pout = e.run(PLAIN["seed"], args.plain_steps + 1, 0.0, track_stats=False)
     pprof = pout["profile"]
     plain_value = PLAIN["P"] * pprof["generations"] / (pprof["loop_ms"] / 1e3)
+    # the plain path on the batch's own shape: seeds as sequential
+    # single-instance runs (P=32), timed on seed 1
+    pq = da.make_problem(t, f, s, freq, TEMPERATURE, population_size=Pb, normalize=True, spectra_type="bfull")
+    with da.Engine(pq) as e:
+        e.run(1, 101, 0.0, track_stats=False)
+        qout = e.run(1, args.plain_steps // 4 + 1, 0.0, track_stats=False)
+    qprof = qout["profile"]
+    plain32_value = Pb * qprof["generations"] / (qprof["loop_ms"] / 1e3)
     uneven = uneven_batch(da, args.uneven_gens) if args.uneven_gens > 0 else None
     sweep = None if args.no_precision_sweep else precision_sweep(da)
     trajectories = precision_trajectories(da, args.trajectory_gens) if args.trajectory_gens > 0 else None
@@ -306,6 +332,11 @@ def main():
                                   "workload": "single instance G=4096, T=65, P=8, seed 1",
                                   "ms_per_generation": pprof["loop_ms"] / pprof["generations"]},
                    "batched_over_plain": value / plain_value,
+                   "plain_fp64_batch_shape": {"value": plain32_value, "unit": "genome-evals/s",
+                                              "workload": "the seed-batch instance shape (G=4096, T=65, P=32) "
+                                                          "run seed by seed through the plain path, seed 1",
+                                              "ms_per_generation": qprof["loop_ms"] / qprof["generations"]},
+                   "batched_over_plain_batch_shape": value / plain32_value,
                    "uneven_batch": uneven,
                    "precision_sweep": sweep,
                    "precision_trajectories": trajectories},
