@@ -159,3 +159,27 @@ def test_model_isf_rejects_bad_arguments_before_the_device():
     pz = da.make_problem(tau, np.exp(-tau), 1e-3 * np.exp(-tau), da.frequency_grid(8, 4.0), 0.0,
                          spectra_type="bfull", zero_temperature=True)
     assert da.lib().deac_genome_length(C.byref(pz)) == 8
+
+
+def test_binding_structs_match_the_header_layout(tmp_path):
+    # the ctypes mirrors of the ABI structs have the C compiler's sizes and
+    # field offsets (a C program built against the header prints them)
+    structs = {"deac_problem": da.Problem, "deac_run_options": da.RunOptions, "deac_result": da.Result,
+               "deac_run_profile": da.RunProfile}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "deac_engine.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'    printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HEADER), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, f"{cname}.{field}"
