@@ -32,7 +32,7 @@ struct deac_handle {
     int device = 0;
     hipStream_t stream = nullptr;       // evolution
     hipStream_t draw_stream = nullptr;  // RNG walk, batches ahead of the evolution
-    int MG = 0, TB = 0, stride = 0, nwg = 0, chunk = 0, batch = 0;
+    int MG = 0, TB = 0, stride = 0, nwg = 0, chunk = 0, batch = 0, maxwg = 0;
     bool overlap = false;               // walk beside the evolution (a CU is left free for it)
     std::vector<void *> allocations;
     double *kmat = nullptr, *nterm = nullptr, *fterm = nullptr, *tterm = nullptr, *obs = nullptr, *sig = nullptr;
@@ -273,6 +273,7 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
 
     const int maxwg = deac::evolve_max_workgroups(device);
     if (maxwg <= 0) return fail(DEAC_ERR_DEVICE);
+    h->maxwg = maxwg;
     int nwg = std::max(h->MG * h->TB, 32);
     if (const char *e = std::getenv("DEAC_EVOLVE_WORKGROUPS")) nwg = std::atoi(e);
     h->nwg = std::clamp(nwg, 1, maxwg);
@@ -360,7 +361,10 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         g_message = "ks_precision must be DEAC_PRECISION_FP64, DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3";
         return DEAC_ERR_INVALID_ARGUMENT;
     }
-    const bool split = o->ks_precision != DEAC_PRECISION_FP64 || o->split_loop != 0;
+    // FP64 also takes the split loop (bit-identical) when the fused kernel
+    // would run its chain tasks in more than one round: more (tau block,
+    // member group) tasks than co-resident workgroups.
+    const bool split = o->ks_precision != DEAC_PRECISION_FP64 || o->split_loop != 0 || h->MG * h->TB > h->maxwg;
     const deac::Model &m = h->m;
     const size_t N = o->number_of_generations, P = m.P, Kc = m.Kc;
     const int nsign = m.nsign;
@@ -461,7 +465,8 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         g += static_cast<size_t>(nb);
         len = std::min(2 * len, static_cast<size_t>(h->batch));
     }
-    hipStream_t d = split || !h->overlap ? s : h->draw_stream;
+    // the split loop's kernels leave room on the CUs for the walk
+    hipStream_t d = split || h->overlap ? h->draw_stream : s;
     auto draws = [&](size_t b) -> deac_status {
         const int k = static_cast<int>(b & 1);
         hipEvent_t *e = h->kev[k];
@@ -551,16 +556,17 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         }
         return DEAC_OK;
     };
-    // The split loop: the same generations as separate kernels on one
-    // stream, batch by batch (walk and mask draws, then per generation top,
-    // trial, renormalization, the contraction at the chosen precision,
-    // moments and selection), the status read after each batch.
+    // The split loop: the same generations as separate kernels, batch by
+    // batch on the plan above.  Batch b's draws (walk, then mask) run on the
+    // draw stream while batch b-1 runs, into buffer set b%2; then, per
+    // generation, top, trial, renormalization, the contraction at the chosen
+    // precision, moments and selection run on the evolution stream.  One
+    // status word serves the whole run: once a stop sets it, every kernel
+    // queued behind (the rest of the batch and the next one) returns at once.
     auto split_loop = [&]() -> deac_status {
         const deac_handle::Split &sp = h->split;
         const deac::KsLayout &y = sp.y;
         const int Kb = static_cast<int>(y.Kb);
-        ea.ctl = h->ctl[0];
-        ea.raw = h->raw[0];
         ea.bar = h->sync;
         ea.status = reinterpret_cast<int *>(h->sync + 4);
         const int *halt = ea.status;
@@ -594,22 +600,34 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         DEAC_TRY(evaluate(1));
         DEAC_HIP(deac::launch_split_select(ea, 0, 1, s));
         DEAC_HIP(hipEventRecord(h->ev0, s));
-        hipEvent_t *e = h->kev[0];
-        for (size_t first = 0; first < total;) {
-            const int nb = static_cast<int>(std::min<size_t>(h->batch, total - first));
-            wa.gen_start = h->gen_start[0];
-            wa.ctl = h->ctl[0];
-            wa.ngen = nb;
-            ma.ctl = h->ctl[0];
-            ma.raw = h->raw[0];
-            ma.ngen = nb;
-            ea.gen0 = static_cast<long>(first);
+        DEAC_HIP(hipStreamWaitEvent(d, h->ev0, 0));  // RNG state uploaded
+        auto walk = [&](size_t b) -> deac_status {
+            const int k = static_cast<int>(b & 1);
+            hipEvent_t *e = h->kev[k];
+            if (b >= 2) DEAC_HIP(hipStreamWaitEvent(d, e[5], 0));  // batch b-2 done with the set
+            wa.gen_start = h->gen_start[k];
+            wa.ctl = h->ctl[k];
+            wa.ngen = plan[b].second;
+            ma.ctl = h->ctl[k];
+            ma.raw = h->raw[k];
+            ma.ngen = plan[b].second;
+            DEAC_HIP(hipEventRecord(e[0], d));
+            DEAC_HIP(deac::launch_walk(wa, d));
+            DEAC_HIP(hipEventRecord(e[1], d));
+            DEAC_HIP(deac::launch_mask(ma, d));
+            DEAC_HIP(hipEventRecord(e[2], d));
+            return DEAC_OK;
+        };
+        auto generations = [&](size_t b) -> deac_status {
+            const int k = static_cast<int>(b & 1);
+            hipEvent_t *e = h->kev[k];
+            const int nb = plan[b].second;
+            DEAC_HIP(hipStreamWaitEvent(s, e[2], 0));  // draws ready
+            DEAC_HIP(hipEventRecord(e[3], s));
+            ea.ctl = h->ctl[k];
+            ea.raw = h->raw[k];
+            ea.gen0 = static_cast<long>(plan[b].first);
             ea.ngen = nb;
-            DEAC_HIP(hipEventRecord(e[0], s));
-            DEAC_HIP(deac::launch_walk(wa, s));
-            DEAC_HIP(hipEventRecord(e[1], s));
-            DEAC_HIP(deac::launch_mask(ma, s));
-            DEAC_HIP(hipEventRecord(e[2], s));
             for (int g = 0; g < nb; g++) {
                 DEAC_HIP(deac::launch_split_top(ea, g, s));
                 DEAC_HIP(deac::launch_split_trial(ea, g, s));
@@ -617,30 +635,41 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
                 DEAC_TRY(evaluate(0));
                 DEAC_HIP(deac::launch_split_select(ea, g, 0, s));
             }
-            DEAC_HIP(hipEventRecord(e[3], s));
-            DEAC_HIP(hipMemcpyAsync(h->host_status, h->sync + 4, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-            DEAC_HIP(hipEventRecord(e[4], s));
-            DEAC_HIP(hipEventSynchronize(e[4]));
+            DEAC_HIP(hipEventRecord(e[5], s));
+            DEAC_HIP(hipMemcpyAsync(h->host_status + 4 * k, h->sync + 4, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+            DEAC_HIP(hipEventRecord(e[6], s));
+            return DEAC_OK;
+        };
+        DEAC_TRY(walk(0));
+        if (plan.size() > 1) DEAC_TRY(walk(1));
+        DEAC_TRY(generations(0));
+        for (size_t b = 0; b < plan.size(); b++) {
+            if (b + 1 < plan.size()) DEAC_TRY(generations(b + 1));
+            const int k = static_cast<int>(b & 1);
+            hipEvent_t *e = h->kev[k];
+            DEAC_HIP(hipEventSynchronize(e[6]));
             float t_walk = 0, t_mask = 0, t_evo = 0;
             DEAC_HIP(hipEventElapsedTime(&t_walk, e[0], e[1]));
             DEAC_HIP(hipEventElapsedTime(&t_mask, e[1], e[2]));
-            DEAC_HIP(hipEventElapsedTime(&t_evo, e[2], e[3]));
+            DEAC_HIP(hipEventElapsedTime(&t_evo, e[3], e[5]));
             prof.walk_ms += t_walk;
             prof.mask_ms += t_mask;
             prof.evolve_ms += t_evo;
             prof.evolve_launches += 1;
-            const int *status = h->host_status;
+            // the copy after batch b: a stop there, at generation status[1] of
+            // the batch (later batches have not run, or returned at once)
+            const int *status = h->host_status + 4 * k;
             if (status[0]) {
-                stopped_at = static_cast<long>(first) + status[1];
+                stopped_at = static_cast<long>(plan[b].first) + status[1];
                 uint64_t st4[4];
-                DEAC_HIP(hipMemcpy(st4, h->gen_start[0] + static_cast<size_t>(status[1]) * 4, sizeof st4,
+                DEAC_HIP(hipMemcpy(st4, h->gen_start[k] + static_cast<size_t>(status[1]) * 4, sizeof st4,
                                    hipMemcpyDeviceToHost));
                 std::memcpy(rng.s, st4, sizeof st4);
                 done += static_cast<size_t>(status[1]);
                 return DEAC_OK;
             }
-            done += static_cast<size_t>(nb);
-            first += static_cast<size_t>(nb);
+            done += static_cast<size_t>(plan[b].second);
+            if (b + 2 < plan.size()) DEAC_TRY(walk(b + 2));
         }
         return DEAC_OK;
     };

@@ -17,7 +17,8 @@
 //   split_select_kernel   chi^2 and greedy selection
 //
 // A stop found at the top of generation g sets status[0]; every later kernel
-// of the batch reads it first and returns, leaving the state of the stop.
+// (of this batch and of any batch queued behind it) reads it first and
+// returns, leaving the state of the stop.
 // The file is compiled with -ffp-contract=off: every fused multiply-add is
 // written as fma().
 #include <hip/hip_runtime.h>
