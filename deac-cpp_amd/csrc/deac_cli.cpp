@@ -13,6 +13,10 @@
 // tools/generate_commands.sh:18-32 in one process.  --average_dsf adds the
 // seed-averaged DSF, summed in the order tools/plot_simple_average_deac_results.py:8-28
 // uses (run keys sorted as strings).
+//
+// --ks_precision fp32 | bf16x3 runs the reconstruction with the model ISF's
+// K.S contraction in that precision (the split generation loop); the random
+// draws, chi^2, selection and statistics stay the reference's, in fp64.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -245,6 +249,7 @@ int main(int argc, char **argv)
     prog.add("", "--single_particle_fermionic_spectral_function", "Renormalize every generation when --normalize is set.", "", true);
     prog.add("", "--number_of_seeds", "Run seeds seed, seed+1, .., seed+K-1 in one batched launch; each seed writes the files of its own run.", "1");
     prog.add("", "--average_dsf", "With --number_of_seeds: also write the seed-averaged DSF (<prefix>_seed-average_<first>-<last>.bin).", "", true);
+    prog.add("", "--ks_precision", "Precision of the model ISF K.S [fp64, fp32, bf16x3]; chi^2 and selection stay fp64.", "\"fp64\"");
 
     try {
         prog.parse(argc, argv);
@@ -397,10 +402,19 @@ int main(int argc, char **argv)
                 res[k].stats_squared_mean = s_sq[k].data();
             }
         }
-        const deac_run_options ro{N, stop, track_stats ? 1 : 0};
+        const std::string ksp = prog.get("--ks_precision").present ? prog.get("--ks_precision").value : "fp64";
+        const std::map<std::string, int> ks_ids = {{"fp64", DEAC_PRECISION_FP64}, {"fp32", DEAC_PRECISION_FP32},
+                                                   {"bf16x3", DEAC_PRECISION_BF16X3}};
+        if (!ks_ids.count(ksp)) {
+            std::cout << "Please choose ks_precision from the following options: fp64, fp32, bf16x3" << std::endl;
+            std::exit(1);
+        }
+        const deac_run_options ro{N, stop, track_stats ? 1 : 0, ks_ids.at(ksp), 0};
         const char *msg = nullptr;
         bool done = false;
-        if (nseeds > 1) {
+        if (nseeds > 1 && ro.ks_precision != DEAC_PRECISION_FP64) {
+            std::cout << "seeds run one after another (the batch kernel computes K.S in fp64)" << std::endl;
+        } else if (nseeds > 1) {
             std::vector<deac_problem> probs(nseeds, p);
             deac_batch *b = nullptr;
             deac_status st = deac_batch_create(probs.data(), nseeds, &b, &msg);

@@ -187,13 +187,14 @@ __global__ __launch_bounds__(kThreads) void split_model_kernel(EvoArgs a, const 
                                                                size_t mn, const int *ex)
 {
     if (stopped(a)) return;
+    // consecutive threads take consecutive genomes: coalesced slice reads
     const size_t total = static_cast<size_t>(a.P) * a.T;
     for (size_t idx = grid_thread(); idx < total; idx += grid_threads()) {
-        const int n = static_cast<int>(idx / a.T), t = static_cast<int>(idx % a.T);
+        const int t = static_cast<int>(idx / a.P), n = static_cast<int>(idx % a.P);
         const size_t i = static_cast<size_t>(t) * Npad + n;
         float s = 0.0f;
         for (int k = 0; k < slices; k++) s += part[static_cast<size_t>(k) * mn + i];
-        a.emod[idx] = ldexp(static_cast<double>(s), ex[t]);
+        a.emod[static_cast<size_t>(n) * a.T + t] = ldexp(static_cast<double>(s), ex[t]);
     }
 }
 

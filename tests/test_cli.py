@@ -58,6 +58,14 @@ def test_bad_spectra(tmp_path):
     assert "Please choose spectra_type" in r.stdout
 
 
+def test_bad_ks_precision(tmp_path):
+    fixture = tmp_path / "tiny-isf.bin"
+    write_fixture(fixture)
+    r = run(deac_command(tmp_path, fixture, extra=["--ks_precision", "fp16"]), tmp_path)
+    assert r.returncode == 1
+    assert "Please choose ks_precision from the following options: fp64, fp32, bf16x3" in r.stdout
+
+
 def test_missing_isf_file_argument(tmp_path):
     r = run([EXE, "-T", "1.0"], tmp_path)
     assert r.returncode == 1
@@ -284,3 +292,10 @@ def test_cli_seed_batch_matches_single_runs(tmp_path):
 def test_cli_seed_batch_unbatchable_runs_seed_by_seed(tmp_path):
     # moment constraints run through the single-instance engine
     _seed_batch_vs_single_runs(tmp_path, ["--first_moment", "0.5"], [3, 4], expect_batched=False)
+
+
+@pytest.mark.gpu
+def test_cli_mixed_precision_seeds_run_seed_by_seed(tmp_path):
+    # the batch kernel computes K.S in fp64: fp32 seeds go through deac_run
+    # one after another, each exactly its own single run
+    _seed_batch_vs_single_runs(tmp_path, ["--normalize", "--ks_precision", "fp32"], [5, 6], expect_batched=False)
