@@ -19,6 +19,8 @@
 // written as fma().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "evolve_common.hpp"
 #include "kernels.hpp"
@@ -176,78 +178,117 @@ constexpr int kKBuf = kChainTau * kRowPad;
 constexpr int kSBuf = kChainRows * kRowPad;
 constexpr int kChainLds = kStages * (kKBuf + kSBuf);
 
-typedef __attribute__((address_space(1))) const void global_cvoid;
-typedef __attribute__((address_space(3))) void lds_void;
+// Register-staged copies of one chunk: 20 rows of 1 KB (kChunk doubles), 16
+// kernel-matrix rows then 4 genome rows, dealt round-robin to the helper
+// waves 1-3 (7, 7 and 6 rows); lane l moves bytes [16l, 16l+16) of a row.
+// Plain 16-byte loads into a ring of registers, kAhead chunks in flight per
+// wave, then ds_write_b128 into the chunk's LDS stage: LDS-DMA copies
+// (global_load_lds) stall at issue after a few are in flight per CU.
+constexpr int kCopiesPerChunk = kChainTau + kChainRows;
+constexpr int kHelperCopies = (kCopiesPerChunk + 2) / 3;
+constexpr int kAhead = 4;
+static_assert(kChunk == 128, "one copy per row: 64 lanes x 16 bytes");
+static_assert(kStages >= 2, "the chunk written at step ch must not be the one read at step ch");
 
-// 16-byte LDS-DMA copies of one chunk: one copy of 1 KB per row (kChunk
-// doubles), 16 kernel-matrix rows then 4 genome rows.
-constexpr int kCopiesPerRow = kChunk / 128;
-constexpr int kCopiesPerChunk = kCopiesPerRow * (kChainTau + kChainRows);
-// Helper waves 1..3 share the copies round-robin: 7, 7 and 6 copies; rows
-// past the last timeslice are not copied (their lanes' sums are discarded).
-static_assert(kCopiesPerChunk == 20, "wait_copies assumes 7/7/6 copies per helper wave");
+// Two doubles as a native vector: plain 16-byte loads and stores that stay
+// in registers (HIP's double2 struct is copied through memory).
+typedef double dvec2 __attribute__((ext_vector_type(2)));
 
-// Issues (does not wait for) the copies of chunk `chunk` into LDS stage
-// `stage` owned by helper wave `wid` (1..3).  Lane l moves bytes
-// [16l, 16l+16) of each 1 KB piece straight from global memory into LDS.
-__device__ __forceinline__ void stage_chunk_async(const EvoArgs &a, double *lds, const double *const *srows, int tb,
-                                                  int chunk, int stage, int wid, int lane)
+// A helper wave's copy rows (the same for the whole task, scalar): row j of
+// the wave is copy q = wid-1 + 3j of a chunk -- kernel row q (q < 16), genome
+// row q-16 (q < 20) or, for the third wave's seventh slot, none.  base: the
+// row's start in global memory (rows past the last timeslice, and the empty
+// slot, read the first kernel row; their values are discarded); off: the
+// row's offset in stage 0 of the LDS buffers, step: the stage stride.
+typedef __attribute__((address_space(1))) const dvec2 global_dvec2;
+
+struct HelperRows {
+    const double *b0, *b1, *b2, *b3, *b4, *b5, *b6;
+    int o0, o1, o2, o3, o4, o5, o6;
+    int k0, k1, k2, k3, k4, k5, k6;  // stage stride of the row's buffer
+};
+
+__device__ __forceinline__ void helper_row(const EvoArgs &a, const double *const *srows, int tb, int q,
+                                           const double *&b, int &o, int &k)
 {
-    const size_t col = static_cast<size_t>(chunk) * kChunk;
-    for (int q = wid - 1; q < kCopiesPerChunk; q += 3) {
-        const int row = q / kCopiesPerRow, half = (q % kCopiesPerRow) * 128;
-        const double *src;
-        double *dst;
-        if (row < kChainTau) {
-            if (tb * kChainTau + row >= a.T) {
-                // keep the per-wave copy count fixed: a 16-byte dummy copy
-                // of the first kernel row into this (discarded) LDS row
-                src = a.kmat + col;
-                dst = lds + stage * kKBuf + row * kRowPad + half;
-                __builtin_amdgcn_global_load_lds((global_cvoid *)(src + 2 * lane), (lds_void *)(dst), 16, 0, 0);
-                continue;
-            }
-            src = a.kmat + static_cast<size_t>(tb * kChainTau + row) * a.stride + col + half;
-            dst = lds + stage * kKBuf + row * kRowPad + half;
-        } else {
-            const int r = row - kChainTau;
-            src = srows[r] + col + half;
-            dst = lds + kStages * kKBuf + stage * kSBuf + r * kRowPad + half;
-        }
-        __builtin_amdgcn_global_load_lds((global_cvoid *)(src + 2 * lane),
-                                         (lds_void *)(dst), 16, 0, 0);
+    if (q < kChainTau) {
+        const int t = tb * kChainTau + q;
+        b = a.kmat + (t < a.T ? static_cast<size_t>(t) * a.stride : 0);
+        o = q * kRowPad;
+        k = kKBuf;
+    } else if (q < kCopiesPerChunk) {
+        b = srows[q - kChainTau];
+        o = kStages * kKBuf + (q - kChainTau) * kRowPad;
+        k = kSBuf;
+    } else {
+        b = a.kmat;
+        o = -1;
+        k = 0;
     }
 }
 
+__device__ __forceinline__ HelperRows helper_rows(const EvoArgs &a, const double *const *srows, int tb, int wid)
+{
+    HelperRows h;
+    helper_row(a, srows, tb, wid - 1, h.b0, h.o0, h.k0);
+    helper_row(a, srows, tb, wid + 2, h.b1, h.o1, h.k1);
+    helper_row(a, srows, tb, wid + 5, h.b2, h.o2, h.k2);
+    helper_row(a, srows, tb, wid + 8, h.b3, h.o3, h.k3);
+    helper_row(a, srows, tb, wid + 11, h.b4, h.o4, h.k4);
+    helper_row(a, srows, tb, wid + 14, h.b5, h.o5, h.k5);
+    helper_row(a, srows, tb, wid + 17, h.b6, h.o6, h.k6);
+    return h;
+}
 
-// Workgroup barrier for the chunk pipeline.  Unlike __syncthreads it does not
-// drain outstanding global->LDS copies; every wave retires what the others
-// depend on before arriving (see wait_copies and DEAC_WAIT_LDS).
+// One ring slot: a helper wave's 7 rows of one chunk, in named registers (no
+// array, so nothing is indexed at run time).
+struct RowRing {
+    dvec2 v0, v1, v2, v3, v4, v5, v6;
+};
+static_assert(kHelperCopies == 7, "RowRing holds 7 rows");
+
+__device__ __forceinline__ dvec2 load_row(const double *base, int chunk, int lane)
+{
+    // (a global-address-space pointer: a flat load would also count against
+    // the LDS counter, which the helpers drain before every barrier)
+    return ((global_dvec2 *)(base + static_cast<size_t>(chunk) * kChunk))[lane];
+}
+
+__device__ __forceinline__ void store_row(double *lds, int stage, int off, int step, int lane, const dvec2 &v)
+{
+    if (off >= 0) reinterpret_cast<dvec2 *>(lds + off + stage * step)[lane] = v;
+}
+
+__device__ __forceinline__ void load_chunk(const HelperRows &h, int chunk, int lane, RowRing &r)
+{
+    r.v0 = load_row(h.b0, chunk, lane);
+    r.v1 = load_row(h.b1, chunk, lane);
+    r.v2 = load_row(h.b2, chunk, lane);
+    r.v3 = load_row(h.b3, chunk, lane);
+    r.v4 = load_row(h.b4, chunk, lane);
+    r.v5 = load_row(h.b5, chunk, lane);
+    r.v6 = load_row(h.b6, chunk, lane);
+}
+
+__device__ __forceinline__ void store_chunk(double *lds, int stage, const HelperRows &h, int lane, const RowRing &r)
+{
+    store_row(lds, stage, h.o0, h.k0, lane, r.v0);
+    store_row(lds, stage, h.o1, h.k1, lane, r.v1);
+    store_row(lds, stage, h.o2, h.k2, lane, r.v2);
+    store_row(lds, stage, h.o3, h.k3, lane, r.v3);
+    store_row(lds, stage, h.o4, h.k4, lane, r.v4);
+    store_row(lds, stage, h.o5, h.k5, lane, r.v5);
+    store_row(lds, stage, h.o6, h.k6, lane, r.v6);
+}
+
+// Workgroup barrier for the chunk pipeline.  Every wave retires what the
+// others depend on before arriving: the helpers their LDS writes, the
+// compute wave its LDS reads (DEAC_WAIT_LDS).
 __device__ __forceinline__ void chunk_barrier()
 {
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-}
-
-__device__ __forceinline__ void wait_copies(int wid, int pending)
-{
-    // Copies retire in order: leave the `pending` newest chunks in flight.
-    if (wid == 3) {
-        if (pending == 5) DEAC_WAIT_VM(30);
-        else if (pending == 4) DEAC_WAIT_VM(24);
-        else if (pending == 3) DEAC_WAIT_VM(18);
-        else if (pending == 2) DEAC_WAIT_VM(12);
-        else if (pending == 1) DEAC_WAIT_VM(6);
-        else DEAC_WAIT_VM(0);
-    } else {
-        if (pending == 5) DEAC_WAIT_VM(35);
-        else if (pending == 4) DEAC_WAIT_VM(28);
-        else if (pending == 3) DEAC_WAIT_VM(21);
-        else if (pending == 2) DEAC_WAIT_VM(14);
-        else if (pending == 1) DEAC_WAIT_VM(7);
-        else DEAC_WAIT_VM(0);
-    }
 }
 
 // Model ISF E[i][t] of one (tau block, member group) task; the whole
@@ -257,42 +298,62 @@ __device__ __forceinline__ void wait_copies(int wid, int pending)
 // (matrix_multiply_LxM_by_MxN called twice, deac.cpp:1386-1394).
 __device__ void chain_task(const EvoArgs &a, double *lds, int tb, int mg, bool init)
 {
-    const int wid = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    // the wave index as a scalar: every per-row choice below is then uniform
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     __shared__ const double *srows[kChainRows];
     if (threadIdx.x < kChainRows) srows[threadIdx.x] = chain_row(a, mg * kChainRows + threadIdx.x, init);
     __syncthreads();
     const int nchunk = a.stride / kChunk;
-    // Prologue: chunks 0 .. kStages-2 in flight, chunk 0 complete.
+    // Chunk c travels in ring slot c % kAhead (r0..r3); at step ch the
+    // helpers write chunk ch+1 into LDS stage (ch+1) % kStages and load chunk
+    // ch+1+kAhead into the slot it left.  Loads past the last chunk re-read
+    // it; the steps run to a multiple of kAhead (the extra ones only meet at
+    // the barrier), so every load is unconditional.
+    static_assert(kAhead == 4, "four ring slots r0..r3");
+    RowRing r0, r1, r2, r3;
+    const HelperRows hr = helper_rows(a, srows, tb, wid);
+    const int last = nchunk - 1;
     if (wid > 0) {
-        const int pre = min(kStages - 1, nchunk);
-        for (int q = 0; q < pre; q++) stage_chunk_async(a, lds, srows, tb, q, q, wid, lane);
-        wait_copies(wid, pre - 1);  // chunk 0 landed
+        load_chunk(hr, 0, lane, r0);
+        load_chunk(hr, min(1, last), lane, r1);
+        load_chunk(hr, min(2, last), lane, r2);
+        load_chunk(hr, min(3, last), lane, r3);
+        store_chunk(lds, 0, hr, lane, r0);
+        load_chunk(hr, min(kAhead, last), lane, r0);
+        DEAC_WAIT_LDS();
     }
     chunk_barrier();
     double acc = 0.0;
-    for (int ch = 0; ch < nchunk; ch++) {
+    auto step = [&](int ch, RowRing &rs) __attribute__((always_inline)) {
         if (wid == 0) {
-            // 64 chains (16 tau x 4 members) advanced 4 positions per
-            // v_mfma_f64_4x4x4_4b_f64, which is bit for bit four fused
-            // multiply-adds in ascending position order: block b holds tau
-            // rows 4b..4b+3 of the task, every block the same 4 members.
-            // A: lane l reads tau row l&15 at position j + (l>>4); B: member
-            // l&3 at the same position (a broadcast across the blocks).
-            const int st = ch % kStages;
-            const double *kb = lds + st * kKBuf + (lane & 15) * kRowPad + (lane >> 4);
-            const double *sb = lds + kStages * kKBuf + st * kSBuf + (lane & 3) * kRowPad + (lane >> 4);
+            if (ch < nchunk) {
+                // 64 chains (16 tau x 4 members) advanced 4 positions per
+                // v_mfma_f64_4x4x4_4b_f64, which is bit for bit four fused
+                // multiply-adds in ascending position order: block b holds
+                // tau rows 4b..4b+3 of the task, every block the same 4
+                // members.  A: lane l reads tau row l&15 at position
+                // j + (l>>4); B: member l&3 at the same position (a broadcast
+                // across the blocks).
+                const int st = ch % kStages;
+                const double *kb = lds + st * kKBuf + (lane & 15) * kRowPad + (lane >> 4);
+                const double *sb = lds + kStages * kKBuf + st * kSBuf + (lane & 3) * kRowPad + (lane >> 4);
 #pragma unroll
-            for (int j = 0; j < kChunk; j += 4) acc = __builtin_amdgcn_mfma_f64_4x4x4f64(kb[j], sb[j], acc, 0, 0, 0);
-            DEAC_WAIT_LDS();  // the stage may be refilled after the barrier
+                for (int j = 0; j < kChunk; j += 4)
+                    acc = __builtin_amdgcn_mfma_f64_4x4x4f64(kb[j], sb[j], acc, 0, 0, 0);
+                DEAC_WAIT_LDS();  // the stage may be refilled after the barrier
+            }
         } else {
-            // Keep kStages-1 chunks ahead of the compute wave; before the
-            // barrier, chunk ch+1 must have landed.
-            const int next = ch + kStages - 1;
-            if (next < nchunk) stage_chunk_async(a, lds, srows, tb, next, next % kStages, wid, lane);
-            // chunks ch+2 .. min(next, nchunk-1) may stay in flight
-            wait_copies(wid, max(0, min(next, nchunk - 1) - (ch + 1)));
+            if (ch + 1 < nchunk) store_chunk(lds, (ch + 1) % kStages, hr, lane, rs);
+            load_chunk(hr, min(ch + 1 + kAhead, last), lane, rs);
+            DEAC_WAIT_LDS();  // chunk ch+1 is in LDS before the barrier
         }
         chunk_barrier();
+    };
+    for (int ch0 = 0; ch0 < nchunk; ch0 += kAhead) {
+        step(ch0, r1);
+        step(ch0 + 1, r2);
+        step(ch0 + 2, r3);
+        step(ch0 + 3, r0);
     }
     if (wid == 0) {
         // 4x4x4_4b output: lane l holds row l>>4 of block (l>>2)&3, column l&3
@@ -323,12 +384,14 @@ __device__ unsigned long long deac_phase_clock[4];
     } while (0)
 #endif
 
-__global__ __launch_bounds__(256) void evolve_kernel(EvoArgs a)
+// One workgroup per CU (its LDS fills the CU), so one wave per SIMD: the
+// register file is the wave's, and the helpers' row ring fits beside the rest.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void evolve_kernel(EvoArgs a)
 {
 #ifdef DEAC_PHASE_TIMERS
     unsigned long long phase_t0 = 0;
 #endif
-    extern __shared__ double lds[];
+    extern __shared__ __attribute__((aligned(16))) double lds[];  // 16-byte aligned: ds_write_b128
     __shared__ double dsh[kEvoWaves][kWave];
     __shared__ int stop_flag;
     const int lane = threadIdx.x & (kWave - 1);
@@ -504,16 +567,9 @@ __global__ void zero_u32_kernel(unsigned *p, size_t n)
 
 // ---------------------------------------------------------------- launchers
 
-// LDS the walk kernel holds beyond its own table: with it the walk never
-// shares a CU (160 KiB of LDS) with an evolution workgroup, whose FP64 MFMA
-// chains would otherwise lose issue cycles to the walk's VALU work.
-constexpr size_t kWalkLdsReserve = 32 * 1024;
-static_assert(kChainLds * sizeof(double) + sizeof(uint64_t) * 256 * 4 + kWalkLdsReserve > 160 * 1024,
-              "the walk kernel must not fit on a CU beside an evolution workgroup");
-
 hipError_t launch_walk(const WalkArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(walk_kernel, dim3(1), dim3(64), kWalkLdsReserve, s, a);
+    hipLaunchKernelGGL(walk_kernel, dim3(1), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
@@ -554,7 +610,10 @@ int evolve_max_workgroups(int device)
                                                      kEvoThreads, kChainLds * sizeof(double)) != hipSuccess)
         return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    return per_cu * cus;
+    // One workgroup per CU: the grid barrier needs every workgroup resident,
+    // and a second one per CU is not guaranteed to be even where the
+    // occupancy estimate allows it.
+    return std::min(per_cu, 1) * cus;
 }
 
 }  // namespace deac

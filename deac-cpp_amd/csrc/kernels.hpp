@@ -9,15 +9,15 @@
 
 namespace deac {
 
-// Exact FP64 model-ISF contraction: one compute wavefront holds 4 members
-// (rows of 16 lanes) x 16 timeslices (lanes of a row), one sequential FMA
-// chain per lane; the genome value reaches the row by DPP row broadcast, the
-// kernel matrix comes from LDS.  The genome is streamed through LDS in chunks
-// of kChunk positions, kStages chunks in flight.
+// Exact FP64 model-ISF contraction: one compute wavefront holds 4 members x
+// 16 timeslices, one sequential FMA chain per output on
+// v_mfma_f64_4x4x4_4b_f64; the kernel-matrix and genome chunks of kChunk
+// positions come from LDS, kStages buffers, filled by the helper waves from a
+// ring of registers a few chunks ahead.
 constexpr int kChainRows = 4;
 constexpr int kChainTau = 16;
 constexpr int kChunk = 128;
-constexpr int kStages = 7;
+constexpr int kStages = 2;
 
 // Random choices of one member in one generation, produced by walk_kernel.
 struct MemberCtl {
@@ -74,7 +74,8 @@ hipError_t launch_walk(const WalkArgs &a, hipStream_t s);
 hipError_t launch_mask(const MaskArgs &a, hipStream_t s);
 hipError_t launch_evolve(const EvoArgs &a, int nwg, hipStream_t s);
 hipError_t launch_zero_u32(unsigned *p, size_t n, hipStream_t s);
-// Workgroups of the evolution kernel that fit co-resident on the device.
+// Workgroups of the evolution kernel that are co-resident on the device: one
+// per CU.
 int evolve_max_workgroups(int device);
 
 }  // namespace deac
