@@ -310,22 +310,14 @@ __device__ void chain_task(const EvoArgs &a, double *lds, int tb, int mg, bool i
     // it; the steps run to a multiple of kAhead (the extra ones only meet at
     // the barrier), so every load is unconditional.
     static_assert(kAhead == 4, "four ring slots r0..r3");
-    RowRing r0, r1, r2, r3;
-    const HelperRows hr = helper_rows(a, srows, tb, wid);
-    const int last = nchunk - 1;
-    if (wid > 0) {
-        load_chunk(hr, 0, lane, r0);
-        load_chunk(hr, min(1, last), lane, r1);
-        load_chunk(hr, min(2, last), lane, r2);
-        load_chunk(hr, min(3, last), lane, r3);
-        store_chunk(lds, 0, hr, lane, r0);
-        load_chunk(hr, min(kAhead, last), lane, r0);
-        DEAC_WAIT_LDS();
-    }
-    chunk_barrier();
+    // The compute wave and the helpers run loops of their own, each meeting
+    // the other at one barrier per step (the same count: the steps run to a
+    // multiple of kAhead).
+    const int nsteps = (nchunk + kAhead - 1) / kAhead * kAhead;
     double acc = 0.0;
-    auto step = [&](int ch, RowRing &rs) __attribute__((always_inline)) {
-        if (wid == 0) {
+    if (wid == 0) {
+        chunk_barrier();  // chunk 0 staged
+        for (int ch = 0; ch < nsteps; ch++) {
             if (ch < nchunk) {
                 // 64 chains (16 tau x 4 members) advanced 4 positions per
                 // v_mfma_f64_4x4x4_4b_f64, which is bit for bit four fused
@@ -337,23 +329,44 @@ __device__ void chain_task(const EvoArgs &a, double *lds, int tb, int mg, bool i
                 const int st = ch % kStages;
                 const double *kb = lds + st * kKBuf + (lane & 15) * kRowPad + (lane >> 4);
                 const double *sb = lds + kStages * kKBuf + st * kSBuf + (lane & 3) * kRowPad + (lane >> 4);
+                double ka[kChunk / 4], sa[kChunk / 4];
 #pragma unroll
-                for (int j = 0; j < kChunk; j += 4)
-                    acc = __builtin_amdgcn_mfma_f64_4x4x4f64(kb[j], sb[j], acc, 0, 0, 0);
+                for (int j = 0; j < kChunk / 4; j++) {
+                    ka[j] = kb[4 * j];
+                    sa[j] = sb[4 * j];
+                }
+#pragma unroll
+                for (int j = 0; j < kChunk / 4; j++) acc = __builtin_amdgcn_mfma_f64_4x4x4f64(ka[j], sa[j], acc, 0, 0, 0);
                 DEAC_WAIT_LDS();  // the stage may be refilled after the barrier
             }
-        } else {
+            chunk_barrier();
+        }
+    } else {
+        RowRing r0, r1, r2, r3;
+        const HelperRows hr = helper_rows(a, srows, tb, wid);
+        const int last = nchunk - 1;
+        load_chunk(hr, 0, lane, r0);
+        load_chunk(hr, min(1, last), lane, r1);
+        load_chunk(hr, min(2, last), lane, r2);
+        load_chunk(hr, min(3, last), lane, r3);
+        store_chunk(lds, 0, hr, lane, r0);
+        load_chunk(hr, min(kAhead, last), lane, r0);
+        DEAC_WAIT_LDS();
+        chunk_barrier();
+        // step ch: chunk ch+1 into LDS stage (ch+1) % kStages, then chunk
+        // ch+1+kAhead into the ring slot it left
+        auto step = [&](int ch, RowRing &rs) __attribute__((always_inline)) {
             if (ch + 1 < nchunk) store_chunk(lds, (ch + 1) % kStages, hr, lane, rs);
             load_chunk(hr, min(ch + 1 + kAhead, last), lane, rs);
             DEAC_WAIT_LDS();  // chunk ch+1 is in LDS before the barrier
+            chunk_barrier();
+        };
+        for (int ch0 = 0; ch0 < nsteps; ch0 += kAhead) {
+            step(ch0, r1);
+            step(ch0 + 1, r2);
+            step(ch0 + 2, r3);
+            step(ch0 + 3, r0);
         }
-        chunk_barrier();
-    };
-    for (int ch0 = 0; ch0 < nchunk; ch0 += kAhead) {
-        step(ch0, r1);
-        step(ch0 + 1, r2);
-        step(ch0 + 2, r3);
-        step(ch0 + 3, r0);
     }
     if (wid == 0) {
         // 4x4x4_4b output: lane l holds row l>>4 of block (l>>2)&3, column l&3
