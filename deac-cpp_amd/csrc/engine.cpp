@@ -274,7 +274,11 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
     const int maxwg = deac::evolve_max_workgroups(device);
     if (maxwg <= 0) return fail(DEAC_ERR_DEVICE);
     h->maxwg = maxwg;
-    int nwg = std::max(h->MG * h->TB, 32);
+    // At least one workgroup per chain task; beyond that, enough for the
+    // trial and selection phases (one per 512 trial items, 32 to 128: more
+    // cost more at every grid barrier than they save).
+    const size_t trial_items = static_cast<size_t>(P) * h->stride / 2;
+    int nwg = std::max(h->MG * h->TB, static_cast<int>(std::clamp<size_t>(trial_items / 512, 32, 128)));
     if (const char *e = std::getenv("DEAC_EVOLVE_WORKGROUPS")) nwg = std::atoi(e);
     h->nwg = std::clamp(nwg, 1, maxwg);
     h->overlap = h->nwg < maxwg;
