@@ -279,8 +279,15 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
     // cost more at every grid barrier than they save).
     const size_t trial_items = static_cast<size_t>(P) * h->stride / 2;
     int nwg = std::max(h->MG * h->TB, static_cast<int>(std::clamp<size_t>(trial_items / 512, 32, 128)));
-    if (const char *e = std::getenv("DEAC_EVOLVE_WORKGROUPS")) nwg = std::atoi(e);
+    const char *user_nwg = std::getenv("DEAC_EVOLVE_WORKGROUPS");
+    if (user_nwg) nwg = std::atoi(user_nwg);
     h->nwg = std::clamp(nwg, 1, maxwg);
+    // Leave a CU to the walk when that costs no extra round of chain tasks:
+    // the walk then runs beside the evolution instead of in line.
+    const int tasks = h->MG * h->TB;
+    if (!user_nwg && h->nwg == maxwg && maxwg > 1 &&
+        (tasks + maxwg - 2) / (maxwg - 1) == (tasks + maxwg - 1) / maxwg)
+        h->nwg = maxwg - 1;
     h->overlap = h->nwg < maxwg;
 
     // Device buffers.
@@ -365,10 +372,7 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         g_message = "ks_precision must be DEAC_PRECISION_FP64, DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3";
         return DEAC_ERR_INVALID_ARGUMENT;
     }
-    // FP64 also takes the split loop (bit-identical) when the fused kernel
-    // would run its chain tasks in more than one round: more (tau block,
-    // member group) tasks than co-resident workgroups.
-    const bool split = o->ks_precision != DEAC_PRECISION_FP64 || o->split_loop != 0 || h->MG * h->TB > h->maxwg;
+    const bool split = o->ks_precision != DEAC_PRECISION_FP64 || o->split_loop != 0;
     const deac::Model &m = h->m;
     const size_t N = o->number_of_generations, P = m.P, Kc = m.Kc;
     const int nsign = m.nsign;

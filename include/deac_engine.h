@@ -104,9 +104,7 @@ typedef struct deac_run_options {
      * chi^2, selection, statistics and constraints stay FP64 in the
      * reference's order, and the random draws are the reference's.
      * split_loop nonzero: FP64 runs through the split loop as well, with the
-     * exact contraction (results identical to the fused kernel).  FP64 takes
-     * the split loop by itself when the fused kernel would need more than one
-     * round of its (16 timeslices x 4 members) tasks, e.g. T=257 with P=64. */
+     * exact contraction (results identical to the fused kernel). */
     int ks_precision;
     int split_loop;
 } deac_run_options;
