@@ -59,21 +59,22 @@ __device__ __forceinline__ const double *current_row(const EvoArgs &a, int i)
     return (a.cur[i] ? a.pop1 : a.pop0) + static_cast<size_t>(i) * a.stride;
 }
 
-// Trial value at genome position k of member i (deac.cpp:1261-1304).
+// Trial value at genome position k of member i (deac.cpp:1261-1304).  The
+// parent and donor values are all loaded, whichever the crossover keeps, so
+// the loads issue together instead of behind the comparison.
 __device__ __forceinline__ double trial_value(const EvoArgs &a, const MemberCtl &c, const uint64_t *raw, int i, int k)
 {
     const int sg = k >= a.G ? 1 : 0;
     double cr, f;
     control_of(a, c, i, sg, cr, f);
     const double u = static_cast<double>(raw[k]) * 0x1.0p-53;
-    if (u < cr) {
-        const double x1 = current_row(a, c.mutant[0])[k];
-        const double x2 = current_row(a, c.mutant[1])[k];
-        const double x3 = current_row(a, c.mutant[2])[k];
-        const double v = fma(f, x2 - x3, x1);
-        return a.allow_neg ? v : fabs(v);
-    }
-    return current_row(a, i)[k];
+    const double x1 = current_row(a, c.mutant[0])[k];
+    const double x2 = current_row(a, c.mutant[1])[k];
+    const double x3 = current_row(a, c.mutant[2])[k];
+    const double xp = current_row(a, i)[k];
+    const double v = fma(f, x2 - x3, x1);
+    const double m = a.allow_neg ? v : fabs(v);
+    return u < cr ? m : xp;
 }
 
 }  // namespace

@@ -428,19 +428,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             if (g == a.ngen) break;
             DEAC_PHASE(0);
             // Top of the generation: best fitness, statistics, stop test
-            // (deac.cpp:1145-1166).
-            if (threadIdx.x == 0) {
-                double mn = a.fitness[0];
+            // (deac.cpp:1145-1166).  The fitnesses are staged in LDS with one
+            // coalesced load (the chain buffers are idle here) and scanned
+            // by thread 0 in the reference's order.
+            auto top = [&](const double *fit) __attribute__((always_inline)) {
+                double mn = fit[0];
                 for (int i = 0; i < a.P; i++)
-                    if (a.fitness[i] < mn) mn = a.fitness[i];
+                    if (fit[i] < mn) mn = fit[i];
                 const long ii = a.gen0 + g;
                 if (blockIdx.x == 0 && a.track_stats) {
                     double s = 0.0;
-                    for (int i = 0; i < a.P; i++) s += a.fitness[i];
+                    for (int i = 0; i < a.P; i++) s += fit[i];
                     double q = 0.0;
                     const int p4 = a.P & ~3;
                     for (int i = 0; i < a.P; i++) {
-                        const double v = a.fitness[i];
+                        const double v = fit[i];
                         if (i < p4) q = q + v * v;
                         else q = fma(v, v, q);
                     }
@@ -454,6 +456,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                     a.status[1] = g;
                 }
                 stop_flag = stop ? 1 : 0;
+            };
+            if (a.P <= kChainLds) {
+                for (int i = threadIdx.x; i < a.P; i += kEvoThreads) lds[i] = a.fitness[i];
+                __syncthreads();
+                if (threadIdx.x == 0) top(lds);
+            } else if (threadIdx.x == 0) {
+                top(a.fitness);
             }
             __syncthreads();
             if (stop_flag) break;
