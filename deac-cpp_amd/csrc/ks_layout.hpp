@@ -97,7 +97,9 @@ inline KsLayout ks_layout(const Model &m, int precision, size_t count)
             y.ex[t] = mx > 0.0 ? std::ilogb(mx) : 0;
         }
         // genome slices: enough workgroups (about 2048) to fill the device,
-        // at least 8 k-blocks per slice
+        // at least 8 k-blocks per slice.  (Fewer, longer slices are cheaper
+        // to sum but longer FP32 accumulations: capping them at 32 at
+        // T=257, G=16384, P=64 raised the model-ISF error 100-fold.)
         const size_t tiles = static_cast<size_t>(y.Mt) * ((y.Nt + 3) / 4);
         const size_t want =
             std::max<size_t>(1, std::min<size_t>((2048 + tiles - 1) / tiles, std::max<size_t>(y.Kb / 8, 1)));

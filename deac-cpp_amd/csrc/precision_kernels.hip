@@ -159,6 +159,7 @@ __global__ __launch_bounds__(256) void reduce_slices_kernel(const float *__restr
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= mn) return;
     float s = 0.0f;
+#pragma unroll 8
     for (int k = 0; k < slices; k++) s += part[static_cast<size_t>(k) * mn + i];
     out[i] = s;
 }

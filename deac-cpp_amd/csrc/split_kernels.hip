@@ -193,6 +193,7 @@ __global__ __launch_bounds__(kThreads) void split_model_kernel(EvoArgs a, const 
         const int t = static_cast<int>(idx / a.P), n = static_cast<int>(idx % a.P);
         const size_t i = static_cast<size_t>(t) * Npad + n;
         float s = 0.0f;
+#pragma unroll 8
         for (int k = 0; k < slices; k++) s += part[static_cast<size_t>(k) * mn + i];
         a.emod[static_cast<size_t>(n) * a.T + t] = ldexp(static_cast<double>(s), ex[t]);
     }
