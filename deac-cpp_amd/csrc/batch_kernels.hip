@@ -781,12 +781,23 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
     } else if (wave >= kRngWave0) {
         // ------------------------------------------------ mask waves
         // The next generation's crossover draws, Lsub per lane per pass.
+        // Draws are spread evenly over the steps (a cumulative target, in
+        // groups of 8): these waves share their SIMDs with the chains, whose
+        // MFMAs wait for every VALU instruction issued beside them.
         MaskGen mg;
         const bool draws = mgen >= 0;
-        const int q = draws ? ((d.Lsub + nsteps - 1) / nsteps + 7) / 8 * 8 : 0;
+        const int lsub = d.Lsub;
         if (draws) mask_setup<NM>(mg, d, sh, mgen, kind == kPassEvolve, g, (wave - kRngWave0) * kWave + lane);
+        int drawn = 0;
         for (int c = 0; c < nsteps; c++) {
-            if (draws) mask_draws(mg, q, d.Lw);
+            if (draws) {
+                const int target = ((c + 1) * lsub + nsteps - 1) / nsteps;
+                const int q = (target + 7) / 8 * 8 - drawn;
+                if (q > 0) {
+                    mask_draws(mg, q, d.Lw);
+                    drawn += q;
+                }
+            }
             step_barrier();
         }
         DEAC_WAIT_VM(0);
