@@ -10,6 +10,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -153,6 +154,12 @@ deac_status plan_model(BatchModel &b, std::string &why)
     b.C = deac::kBatchChunkDoubles / b.Ppad;
     b.nchunk = (L + b.C - 1) / b.C;
     b.Lpad = b.nchunk * b.C;
+    // the kernel reads one K tile (Lpad x 16 doubles) through a buffer
+    // resource with 32-bit byte offsets
+    if (static_cast<long>(b.Lpad) * 128 > INT32_MAX) {
+        why = "batched runs take genomes of at most 16 million positions";
+        return DEAC_ERR_UNSUPPORTED;
+    }
     b.Lw = b.Lpad / 32;
     b.ntt = T / 16;
     b.nrem = T % 16;
@@ -165,7 +172,7 @@ deac_status plan_model(BatchModel &b, std::string &why)
     const int per_wave = (b.ntt + 3) / 4;
     b.NT = per_wave <= 1 ? 1 : per_wave <= 2 ? 2 : per_wave <= 4 ? 4 : 0;
     if (b.NT == 0 || T * b.Ppad > deac::kBatchRegionDoubles) {
-        why = "batched runs take at most 271 timeslices";
+        why = "batched runs take at most 260 timeslices";
         return DEAC_ERR_UNSUPPORTED;
     }
     b.S = deac::kBatchMaskLanes / b.Ppad;

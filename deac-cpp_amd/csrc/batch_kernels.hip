@@ -54,6 +54,7 @@ constexpr int kCD = kBatchChunkDoubles;
 constexpr int kOffPop = 0;             // 4 buffers: genomes of chunks in flight, [Ppad][C]
 constexpr int kOffTrial = 4 * kCD;     // 2 buffers: trial genomes in MFMA B order
 constexpr int kMaskWords = 256;        // mask words per chunk: 2 generations x Ppad x 2 candidates x C/32
+constexpr int kMaxT = 4 * 4 * 16 + kBatchMaxRem;  // timeslices: 16 full tiles (4 per chain wave) + remainder rows
 
 enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3 };
 
@@ -81,6 +82,7 @@ struct BatchShared {
     WalkRec rec[4][kBatchMaxP];              // ring over generations
     uint64_t gen_start[4][4];                // state at the top of each generation
     uint64_t walk_state[4];                  // walk position between passes
+    double obs[kMaxT], sig[kMaxT];           // ISF and its errors
     double fitness[kBatchMaxP];
     double cr_old[kBatchMaxP][2], f_old[kBatchMaxP][2];
     double cr_trial[2][kBatchMaxP][2];
@@ -126,16 +128,8 @@ __device__ __forceinline__ T *uniform_ptr(T *p)
     return reinterpret_cast<T *>((static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// Two adjacent doubles (16-byte aligned) from global memory, one dwordx4 load.
-__device__ __forceinline__ double2 load2(const gcdouble *p)
-{
-    double2 v;
-    v.x = p[0];
-    v.y = p[1];
-    return v;
-}
-
-// The same, marked non-temporal: the population streams through once per
+// Two adjacent doubles (16-byte aligned) from global memory, one dwordx4
+// load, marked non-temporal: the population streams through once per
 // generation and should not displace the kernel tiles every instance
 // re-reads from L2.
 __device__ __forceinline__ double2 load2_stream(const gcdouble *p)
@@ -144,6 +138,27 @@ __device__ __forceinline__ double2 load2_stream(const gcdouble *p)
     v.x = __builtin_nontemporal_load(p);
     v.y = __builtin_nontemporal_load(p + 1);
     return v;
+}
+
+// A buffer resource over `bytes` bytes from p (raw buffer, word 3 of gfx9)
+// and a load of two adjacent doubles through it at byte offset vo (per lane)
+// + so (wave-uniform): one buffer_load_dwordx4 whose address needs no vector
+// arithmetic.  Reads past `bytes` return zeros.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ rsrc_t make_rsrc(const double *p, int bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(p), 0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ double2 buffer_load2(rsrc_t r, uint32_t vo, int so)
+{
+    const dv2 v = __builtin_bit_cast(dv2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+    double2 d;
+    d.x = v.x;
+    d.y = v.y;
+    return d;
 }
 
 // u < cr for u = (x >> 11) * 2^-53  <=>  (x >> 11) < ceil(cr * 2^53).
@@ -417,7 +432,6 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
 
     if (wave < kWorker0) {
         // ------------------------------------------------ chain waves
-        const gcdouble *kt = (const gcdouble *)d.ktile;
         const int nkb2 = d.Lpad / 8;
         // Tiles past ntt are computed from a clamped (valid) tile and
         // discarded: straight-line loads and MFMAs keep the prefetch waits
@@ -442,29 +456,31 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
 #pragma unroll
             for (int mt = 0; mt < NM; mt++) acc[i][mt] = d4{0.0, 0.0, 0.0, 0.0};
         double2 kr[KD][NT];
-        // Loads address a wave-uniform block base (scalar registers) plus the
-        // lane's 16-byte offset.
-        const gcdouble *ktw[NT];
+        // K tiles through buffer resources: the lane's byte offset is fixed
+        // and the block offset wave-uniform (a scalar register), so the
+        // refills take no vector address arithmetic.
+        rsrc_t ktr[NT];
 #pragma unroll
-        for (int i = 0; i < NT; i++) ktw[i] = kt + static_cast<size_t>(tload[i]) * nkb2 * (2 * kWave);
-        const int lo = 2 * lane;
+        for (int i = 0; i < NT; i++)
+            ktr[i] = make_rsrc(d.ktile + static_cast<size_t>(tload[i]) * nkb2 * (2 * kWave), nkb2 * (2 * kWave) * 8);
+        const uint32_t lob = 16u * lane;
 #pragma unroll
         for (int p = 0; p < KD; p++)
 #pragma unroll
-            for (int i = 0; i < NT; i++) kr[p][i] = load2(ktw[i] + static_cast<size_t>(min(p, nkb2 - 1)) * (2 * kWave) + lo);
+            for (int i = 0; i < NT; i++) kr[p][i] = buffer_load2(ktr[i], lob, min(p, nkb2 - 1) * (2 * kWave * 8));
         // Rows past the last full tile (REM): one 4x4x4 MFMA per 4 positions
         // for member tile rmt = wave % NM; waves >= NM repeat a tile's work
         // (discarded) to keep the code straight-line.  The remainder rows
         // are stored 16 values (4 rows x 4 positions) per 4-position block,
         // lane l reading row l & 3 at position l >> 4.
-        const gcdouble *krm = (const gcdouble *)d.krem;
+        const rsrc_t krm = make_rsrc(d.krem, nkb2 * 32 * 8);
         const int rmt = wave % NM;
-        const int lro = ((lane & 3) + 4 * (lane >> 4)) * 2;
+        const uint32_t lrob = ((lane & 3) + 4 * (lane >> 4)) * 16u;
         double accr = 0.0;
         double2 krr[KD];
         if (REM) {
 #pragma unroll
-            for (int p = 0; p < KD; p++) krr[p] = load2(krm + static_cast<size_t>(min(p, nkb2 - 1)) * 32 + lro);
+            for (int p = 0; p < KD; p++) krr[p] = buffer_load2(krm, lrob, min(p, nkb2 - 1) * (32 * 8));
         }
         // Steps 0-3 fill the pipeline, steps 4 .. nchunk+3 run the chains of
         // chunk c-4; the hot loop has no branch, so the waits for the K
@@ -515,8 +531,8 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                         for (int mt = 0; mt < NM; mt++) bc[e][mt] = bn[e][mt];
                     const int gn = min(g2 + KD, nkb2 - 1);
 #pragma unroll
-                    for (int i = 0; i < NT; i++) kr[sl][i] = load2(ktw[i] + static_cast<size_t>(gn) * (2 * kWave) + lo);
-                    if (REM) krr[sl] = load2(krm + static_cast<size_t>(gn) * 32 + lro);
+                    for (int i = 0; i < NT; i++) kr[sl][i] = buffer_load2(ktr[i], lob, gn * (2 * kWave * 8));
+                    if (REM) krr[sl] = buffer_load2(krm, lrob, gn * (32 * 8));
                     __builtin_amdgcn_sched_barrier(0);  // issue the refill here, a full step ahead of its use
                 }
                 step_barrier();
@@ -834,6 +850,10 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
     const gcdouble *obs = (const gcdouble *)d.obs, *sig = (const gcdouble *)d.sig;
 
     for (int k = threadIdx.x; k < 256 * 4; k += kBatchThreads) sh.rows[k] = ((const gcu64 *)d.walk_rows)[k];
+    for (int k = threadIdx.x; k < d.T; k += kBatchThreads) {
+        sh.obs[k] = obs[k];
+        sh.sig[k] = sig[k];
+    }
     for (int k = threadIdx.x; k < 2 * kCD; k += kBatchThreads) lds[kOffTrial + k] = 0.0;  // padded members stay 0
     if (threadIdx.x < kBatchMaxP) {
         const int m = threadIdx.x;
@@ -878,14 +898,25 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
 
         if (kind == kPassInit || kind == kPassEvolve) {
             // Fitness (deac.cpp:1539-1555; initial fitness deac.cpp:957-976)
-            // and greedy selection (deac.cpp:1556-1572).
+            // and greedy selection (deac.cpp:1556-1572).  Every thread forms
+            // terms ((o_t - E_tm) / sigma_t)^2 over [t][m] in place of E (the
+            // fused last term keeps its quotient), then one lane per member
+            // adds them in timeslice order.
+            for (int k = threadIdx.x; k < d.T * Ppad; k += kBatchThreads) {
+                const int t = k / Ppad;
+                if (k % Ppad < P) {
+                    const double dv = (sh.obs[t] - lds[k]) / sh.sig[t];
+                    lds[k] = t == tlast ? dv : dv * dv;
+                }
+            }
+            __syncthreads();
             if (wave == kWorker0 && lane < Ppad) {
                 int acc = 0;
                 if (lane < P) {
                     double chi = 0.0;
                     for (int t = 0; t < d.T; t++) {
-                        const double dv = (obs[t] - lds[t * Ppad + lane]) / sig[t];
-                        chi = t == tlast ? fma(dv, dv, chi) : chi + dv * dv;
+                        const double v = lds[t * Ppad + lane];
+                        chi = t == tlast ? fma(v, v, chi) : chi + v;
                     }
                     const double fit = chi / static_cast<double>(d.T);
                     if (kind == kPassInit) {

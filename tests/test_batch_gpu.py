@@ -169,3 +169,31 @@ def test_uneven_batch():
         nout = x["G"] if x["zero_temperature"] else 2 * x["G"] - 1
         assert o["dsf"].shape == (nout,) and np.all(np.isfinite(o["dsf"]))
         assert np.all(np.diff(o["minimum"]) <= 0)
+
+
+def _synthetic_problem(T, G, P):
+    beta = 1.0 / 1.35
+    t = np.linspace(0.0, beta / 2, T)
+    f = 0.6 * np.exp(-2.0 * t) + 0.4 * np.exp(-9.0 * t)
+    return da.make_problem(t, f, 1e-3 * f, da.frequency_grid(G, 64.0), 1.35, population_size=P, normalize=True)
+
+
+def test_batch_shape_limits():
+    # the largest T the batch kernel takes (16 full 16-row tiles on four
+    # chain waves plus 4 rows on 4x4x4 MFMAs) runs bit-exact against
+    # deac_run; one more timeslice, or one more member, is refused
+    print("[gpu] batch shape limits: T=260 vs deac_run, T=261 and P=33 refused", flush=True)
+    p = _synthetic_problem(260, 96, 32)
+    with da.Engine(p) as e:
+        plain = [e.run(sd, 40, 0.0) for sd in (2, 3)]
+    with da.Batch([p] * 2) as b:
+        outs = b.run([2, 3], 40, 0.0, True)
+    for pl, o in zip(plain, outs):
+        assert not _compare(o, pl), _compare(o, pl)
+        assert o["rng"] == pl["rng"]
+    for T, P in ((261, 32), (65, 33)):
+        with pytest.raises(da.DeacError) as err:
+            da.Batch([_synthetic_problem(T, 96, P)])
+        assert err.value.status == 5  # DEAC_ERR_UNSUPPORTED
+    with pytest.raises(da.DeacError, match="260 timeslices"):
+        da.Batch([_synthetic_problem(261, 96, 32)])
