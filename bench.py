@@ -245,8 +245,9 @@ def main():
                     help="only run a device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: instances are independent and nothing here spans devices "
-              "(see DESIGN.md for how the work shards); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: instances are independent (across GPUs they would shard by "
+              "instance, with no communication per generation) and nothing here spans devices; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_bytes:
