@@ -41,21 +41,22 @@ __device__ __forceinline__ bool stopped(const EvoArgs &a) { return a.status[0] !
 __device__ __forceinline__ size_t grid_thread() { return static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ size_t grid_threads() { return static_cast<size_t>(gridDim.x) * blockDim.x; }
 
-// Top of generation g: best fitness, statistics, stop test (deac.cpp:1145-1166).
-__global__ __launch_bounds__(kWave) void split_top_kernel(EvoArgs a, int g)
+// Top of generation g: best fitness, statistics, stop test (deac.cpp:1145-1166),
+// scanned by thread 0 in the reference's order from fit(i).
+template <class Fit>
+__device__ void top_scan(const EvoArgs &a, int g, Fit fit)
 {
-    if (threadIdx.x != 0 || stopped(a)) return;
-    double mn = a.fitness[0];
+    double mn = fit(0);
     for (int i = 0; i < a.P; i++)
-        if (a.fitness[i] < mn) mn = a.fitness[i];
+        if (fit(i) < mn) mn = fit(i);
     const long ii = a.gen0 + g;
     if (a.track_stats) {
         double s = 0.0;
-        for (int i = 0; i < a.P; i++) s += a.fitness[i];
+        for (int i = 0; i < a.P; i++) s += fit(i);
         double q = 0.0;
         const int p4 = a.P & ~3;
         for (int i = 0; i < a.P; i++) {
-            const double v = a.fitness[i];
+            const double v = fit(i);
             if (i < p4) q = q + v * v;
             else q = fma(v, v, q);
         }
@@ -66,6 +67,24 @@ __global__ __launch_bounds__(kWave) void split_top_kernel(EvoArgs a, int g)
     if (mn <= a.stop) {
         a.status[0] = 1;
         a.status[1] = g;
+    }
+}
+
+// The fitnesses are staged in LDS by one coalesced load (up to kTopLds of
+// them), so the sequential scans do not wait on a load each.
+constexpr int kTopLds = 2048;
+constexpr int kSelectLds = 4096;  // residuals staged per member (32 KB)
+
+__global__ __launch_bounds__(kWave) void split_top_kernel(EvoArgs a, int g)
+{
+    if (stopped(a)) return;
+    __shared__ double fs[kTopLds];
+    if (a.P <= kTopLds) {
+        for (int i = threadIdx.x; i < a.P; i += kWave) fs[i] = a.fitness[i];
+        __syncthreads();
+        if (threadIdx.x == 0) top_scan(a, g, [&](int i) { return fs[i]; });
+    } else if (threadIdx.x == 0) {
+        top_scan(a, g, [&](int i) { return a.fitness[i]; });
     }
 }
 
@@ -220,25 +239,27 @@ __global__ __launch_bounds__(kWave) void split_moment_kernel(EvoArgs a, int init
 __global__ __launch_bounds__(kWave) void split_select_kernel(EvoArgs a, int g, int init)
 {
     if (stopped(a)) return;
-    __shared__ double dsh[kWave];
+    extern __shared__ double dsh[];  // T residuals (T <= kSelectLds), else one window of kWave
     const int i = blockIdx.x, lane = threadIdx.x;
     double chi = 0.0;
     const int tlast = (a.T & 1) ? a.T - 1 : -1;
-    for (int t0 = 0; t0 < a.T; t0 += kWave) {
-        const int t = t0 + lane;
-        double d = 0.0;
-        if (t < a.T) d = (a.obs[t] - a.emod[static_cast<size_t>(i) * a.T + t]) / a.sig[t];
-        dsh[lane] = d;
+    auto residual = [&](int t) { return (a.obs[t] - a.emod[static_cast<size_t>(i) * a.T + t]) / a.sig[t]; };
+    auto add = [&](int t, double du) { chi = t == tlast ? fma(du, du, chi) : chi + du * du; };
+    if (a.T <= kSelectLds) {
+        // every residual at once (their loads all in flight), then the sum
+        for (int t = lane; t < a.T; t += kWave) dsh[t] = residual(t);
         __syncthreads();
-        if (lane == 0) {
-            const int n = min(kWave, a.T - t0);
-            for (int u = 0; u < n; u++) {
-                const double du = dsh[u];
-                if (t0 + u == tlast) chi = fma(du, du, chi);
-                else chi = chi + du * du;
-            }
+        if (lane == 0)
+            for (int t = 0; t < a.T; t++) add(t, dsh[t]);
+    } else {
+        for (int t0 = 0; t0 < a.T; t0 += kWave) {
+            const int t = t0 + lane;
+            dsh[lane] = t < a.T ? residual(t) : 0.0;
+            __syncthreads();
+            if (lane == 0)
+                for (int u = 0; u < min(kWave, a.T - t0); u++) add(t0 + u, dsh[u]);
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (lane != 0) return;
     double fit = chi / static_cast<double>(a.T);
@@ -336,7 +357,8 @@ hipError_t launch_split_moments(const EvoArgs &a, int init, hipStream_t s)
 
 hipError_t launch_split_select(const EvoArgs &a, int g, int init, hipStream_t s)
 {
-    hipLaunchKernelGGL(split_select_kernel, dim3(a.P), dim3(kWave), 0, s, a, g, init);
+    const size_t lds = static_cast<size_t>(a.T <= kSelectLds ? a.T : kWave) * sizeof(double);
+    hipLaunchKernelGGL(split_select_kernel, dim3(a.P), dim3(kWave), lds, s, a, g, init);
     return hipGetLastError();
 }
 
