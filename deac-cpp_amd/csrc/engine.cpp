@@ -637,8 +637,7 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
             ea.gen0 = static_cast<long>(plan[b].first);
             ea.ngen = nb;
             for (int g = 0; g < nb; g++) {
-                DEAC_HIP(deac::launch_split_top(ea, g, s));
-                DEAC_HIP(deac::launch_split_trial(ea, g, s));
+                DEAC_HIP(deac::launch_split_trial(ea, g, s));  // with the top of generation g
                 if (ea.renorm) DEAC_HIP(deac::launch_split_renorm(ea, s));
                 DEAC_TRY(evaluate(0));
                 DEAC_HIP(deac::launch_split_select(ea, g, 0, s));

@@ -5,8 +5,8 @@
 // the selection, the statistics and every constraint stay FP64 in the
 // reference's order.  Per generation g of a batch (deac.cpp:1145-1572):
 //
-//   split_top_kernel      best fitness, statistics, stop test
-//   split_trial_kernel    trial vectors into each member's spare buffer
+//   split_trial_kernel    best fitness, statistics, stop test (workgroup 0),
+//                         trial vectors into each member's spare buffer
 //   split_renorm_*        per-generation renormalization, when enabled
 //   split_pack_*          trial rows into the contraction's operand layout
 //   ks_*_kernel           the contraction: FP64 exact chains straight into
@@ -70,17 +70,17 @@ __device__ void top_scan(const EvoArgs &a, int g, Fit fit)
     }
 }
 
-// The fitnesses are staged in LDS by one coalesced load (up to kTopLds of
-// them), so the sequential scans do not wait on a load each.
-constexpr int kTopLds = 2048;
+// Run by every thread of one workgroup: the fitnesses are staged in LDS by
+// one coalesced load (up to kTopLds of them), so the sequential scans do not
+// wait on a load each.
+constexpr int kTopLds = 512;
 constexpr int kSelectLds = 4096;  // residuals staged per member (32 KB)
 
-__global__ __launch_bounds__(kWave) void split_top_kernel(EvoArgs a, int g)
+__device__ void top_of_generation(const EvoArgs &a, int g)
 {
-    if (stopped(a)) return;
     __shared__ double fs[kTopLds];
     if (a.P <= kTopLds) {
-        for (int i = threadIdx.x; i < a.P; i += kWave) fs[i] = a.fitness[i];
+        for (int i = threadIdx.x; i < a.P; i += blockDim.x) fs[i] = a.fitness[i];
         __syncthreads();
         if (threadIdx.x == 0) top_scan(a, g, [&](int i) { return fs[i]; });
     } else if (threadIdx.x == 0) {
@@ -90,9 +90,13 @@ __global__ __launch_bounds__(kWave) void split_top_kernel(EvoArgs a, int g)
 
 // Trial vectors of generation g into each member's spare buffer, two genome
 // positions per work item (deac.cpp:1261-1308); the padding stays zero.
+// Workgroup 0 first runs the top of generation g: it reads only the
+// fitnesses, which no trial writes, and if its stop test fires the trials
+// are never selected (every later kernel of the generation returns).
 __global__ __launch_bounds__(kThreads) void split_trial_kernel(EvoArgs a, int g)
 {
     if (stopped(a)) return;
+    if (blockIdx.x == 0) top_of_generation(a, g);
     const int half = a.stride / 2;
     const MemberCtl *ctlg = a.ctl + static_cast<size_t>(g) * a.P;
     const uint64_t *rawg = a.raw + static_cast<size_t>(g) * a.P * a.Kc;
@@ -293,12 +297,6 @@ unsigned grid_for(size_t work, unsigned threads)
 }
 
 }  // namespace
-
-hipError_t launch_split_top(const EvoArgs &a, int g, hipStream_t s)
-{
-    hipLaunchKernelGGL(split_top_kernel, dim3(1), dim3(kWave), 0, s, a, g);
-    return hipGetLastError();
-}
 
 hipError_t launch_split_trial(const EvoArgs &a, int g, hipStream_t s)
 {

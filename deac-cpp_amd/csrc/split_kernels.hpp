@@ -12,9 +12,8 @@
 
 namespace deac {
 
-// Best fitness, statistics and stop test at the top of generation g of the batch.
-hipError_t launch_split_top(const EvoArgs &a, int g, hipStream_t s);
-// Trial vectors of generation g into the spare buffers.
+// Top of generation g of the batch (best fitness, statistics, stop test) and
+// its trial vectors into the spare buffers, in one launch.
 hipError_t launch_split_trial(const EvoArgs &a, int g, hipStream_t s);
 // Per-generation renormalization of the trials (call only when a.renorm).
 hipError_t launch_split_renorm(const EvoArgs &a, hipStream_t s);
