@@ -135,3 +135,21 @@ def test_batch_rejects_mixed_precision():
         o = (da.RunOptions * 1)(da.RunOptions(10, 0.0, 0, 1, 0))
         st = da.lib().deac_batch_run(b.handle, da.seed_state(1), o, r)
         assert st == 5  # UNSUPPORTED
+
+
+@pytest.mark.parametrize("T,G,P", [(12, 48, 520), (4100, 24, 6)])
+def test_split_loop_fallback_shapes_match_fused(T, G, P):
+    # the split kernels' paths for populations past the LDS-staged top of
+    # generation (P > 512) and for ISFs past the LDS-staged residuals
+    # (T > 4096) against the fused kernel, bit for bit
+    print(f"[gpu] split loop fp64 vs fused, T={T} G={G} P={P}", flush=True)
+    beta = 1.0 / 1.35
+    t = np.linspace(0.0, beta / 2, T)
+    f = 0.6 * np.exp(-2.0 * t) + 0.4 * np.exp(-9.0 * t)
+    p = da.make_problem(t, f, 1e-3 * f, da.frequency_grid(G, 64.0), 1.35, population_size=P, normalize=True)
+    with da.Engine(p) as e:
+        fused = e.run(7, 60, 0.0)
+        split = e.run(7, 60, 0.0, split_loop=True)
+    for k in ("minimum", "mean", "squared_mean", "dsf", "frequency"):
+        assert gc.bit_diff(fused[k], split[k]) is None, k
+    assert fused["rng"] == split["rng"] and fused["generation"] == split["generation"]
