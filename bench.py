@@ -17,7 +17,9 @@ already resident in HBM.  The plain single-instance FP64 path (G=4096, P=8,
 T=65, seed 1) is timed beside it, and its ratio reported.
 
 `roofline` prices the batch kernel against MI355X FP64 matrix peak
-(4*G*T flops per genome evaluation); `cpu_baseline` times the reference CPU
+(4*G*T flops per genome evaluation), with the HBM traffic and the matrix
+pipe's busy fraction from the committed rocprofv3 summaries under profiles/;
+`cpu_baseline` times the reference CPU
 executable built from source (oracle/_ref) on a bounded sample of the seed
 batch, one process per seed on the host cores at once, as the reference is
 run for many seeds.
@@ -116,6 +118,18 @@ def traffic_per_generation():
     except OSError:
         return None
     return float(m.group(1)) if m else None
+
+
+def mfma_busy_profiled():
+    """Fraction of cycles the FP64 matrix pipe was busy in the seed batch's
+    kernel, from the committed rocprofv3 SQ-counter summary."""
+    import re
+    path = os.path.join(ROOT, "profiles", "batch_kernel_seed256_sq_counters.md")
+    try:
+        m = re.search(r"MFMA busy: .*= ([0-9.]+) %", open(path).read())
+    except OSError:
+        return None
+    return float(m.group(1)) / 100.0 if m else None
 
 
 def uneven_batch(da, gens):
@@ -348,7 +362,10 @@ def main():
                                      "algorithmic bytes per launch: " + str(alg_launch),
                      "kernel": "batch_kernel<1,2,1>", "flop_per_genome_eval": flop_per_eval,
                      "kernel_ms_per_generation": prof["evolve_ms"] / args.steps,
-                     "hbm_algorithmic_GBps": evals * 2 * G * 8 / kern_s / 1e9},
+                     "hbm_algorithmic_GBps": evals * 2 * G * 8 / kern_s / 1e9,
+                     "mfma_busy": mfma_busy_profiled(),
+                     "mfma_busy_note": "SQ_VALU_MFMA_BUSY_CYCLES over the dispatch's cycles per SIMD, from "
+                                       "profiles/batch_kernel_seed256_sq_counters.md"},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_gens)
