@@ -455,7 +455,6 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         for (int i = 0; i < NT; i++)
 #pragma unroll
             for (int mt = 0; mt < NM; mt++) acc[i][mt] = d4{0.0, 0.0, 0.0, 0.0};
-        double2 kr[KD][NT];
         // K tiles through buffer resources: the lane's byte offset is fixed
         // and the block offset wave-uniform (a scalar register), so the
         // refills take no vector address arithmetic.
@@ -464,33 +463,35 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         for (int i = 0; i < NT; i++)
             ktr[i] = make_rsrc(d.ktile + static_cast<size_t>(tload[i]) * nkb2 * (2 * kWave), nkb2 * (2 * kWave) * 8);
         const uint32_t lob = 16u * lane;
-#pragma unroll
-        for (int p = 0; p < KD; p++)
-#pragma unroll
-            for (int i = 0; i < NT; i++) kr[p][i] = buffer_load2(ktr[i], lob, min(p, nkb2 - 1) * (2 * kWave * 8));
         // Rows past the last full tile (REM): one 4x4x4 MFMA per 4 positions
-        // for member tile rmt = wave % NM; waves >= NM repeat a tile's work
-        // (discarded) to keep the code straight-line.  The remainder rows
-        // are stored 16 values (4 rows x 4 positions) per 4-position block,
-        // lane l reading row l & 3 at position l >> 4.
+        // for member tile rmt = wave % NM, on waves 0 .. NM-1.  The remainder
+        // rows are stored 16 values (4 rows x 4 positions) per 4-position
+        // block, lane l reading row l & 3 at position l >> 4.
         const rsrc_t krm = make_rsrc(d.krem, nkb2 * 32 * 8);
         const int rmt = wave % NM;
         const uint32_t lrob = ((lane & 3) + 4 * (lane >> 4)) * 16u;
         double accr = 0.0;
-        double2 krr[KD];
-        if (REM) {
-#pragma unroll
-            for (int p = 0; p < KD; p++) krr[p] = buffer_load2(krm, lrob, min(p, nkb2 - 1) * (32 * 8));
-        }
-        // Steps 0-3 fill the pipeline, steps 4 .. nchunk+3 run the chains of
-        // chunk c-4; the hot loop has no branch, so the waits for the K
-        // prefetch are exact across its iterations.
         const int nwork = chains ? nchunk : 0;
-        for (int c = 0; c < min(4, nsteps); c++) step_barrier();
-        // One copy of the loop per remainder member tile RT, so the 4x4x4
-        // MFMA takes its B register directly.
-        auto chain_loop = [&](auto rt) {
+        // One copy of the loop per remainder member tile RT (the 4x4x4 MFMA
+        // takes its B register directly), and one without the remainder for
+        // the other waves.  Each copy issues its own K prefetch before the
+        // four pipeline-filling steps; steps 4 .. nchunk+3 run the chains of
+        // chunk c-4.  The hot loop has no branch, so the waits for the K
+        // prefetch are exact across its iterations.
+        auto chain_loop = [&](auto rt, auto rem) {
             constexpr int RT = decltype(rt)::value;
+            constexpr bool RM = decltype(rem)::value;
+            double2 kr[KD][NT];
+            double2 krr[KD];
+#pragma unroll
+            for (int p = 0; p < KD; p++)
+#pragma unroll
+                for (int i = 0; i < NT; i++) kr[p][i] = buffer_load2(ktr[i], lob, min(p, nkb2 - 1) * (2 * kWave * 8));
+            if (RM) {
+#pragma unroll
+                for (int p = 0; p < KD; p++) krr[p] = buffer_load2(krm, lrob, min(p, nkb2 - 1) * (32 * 8));
+            }
+            for (int c = 0; c < min(4, nsteps); c++) step_barrier();
             for (int cm = 0; cm < nwork; cm++) {
                 const double *tb = lds + kOffTrial + (cm & 1) * kCD;
                 // B operands one block pair ahead of the MFMAs that use them
@@ -520,7 +521,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                             for (int mt = 0; mt < NM; mt++)
                                 acc[i][mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bc[e][mt], acc[i][mt], 0, 0, 0);
                         }
-                        if (REM) {
+                        if (RM) {
                             const double ar = e ? krr[sl].y : krr[sl].x;
                             accr = __builtin_amdgcn_mfma_f64_4x4x4f64(ar, bc[e][RT], accr, 0, 0, 0);
                         }
@@ -532,15 +533,16 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     const int gn = min(g2 + KD, nkb2 - 1);
 #pragma unroll
                     for (int i = 0; i < NT; i++) kr[sl][i] = buffer_load2(ktr[i], lob, gn * (2 * kWave * 8));
-                    if (REM) krr[sl] = buffer_load2(krm, lrob, gn * (32 * 8));
+                    if (RM) krr[sl] = buffer_load2(krm, lrob, gn * (32 * 8));
                     __builtin_amdgcn_sched_barrier(0);  // issue the refill here, a full step ahead of its use
                 }
                 step_barrier();
             }
+            for (int c = min(4, nsteps) + nwork; c < nsteps; c++) step_barrier();
         };
-        if (!REM || rmt == 0) chain_loop(std::integral_constant<int, 0>{});
-        else chain_loop(std::integral_constant<int, NM - 1>{});
-        for (int c = min(4, nsteps) + nwork; c < nsteps; c++) step_barrier();
+        if (!REM || wave >= NM) chain_loop(std::integral_constant<int, 0>{}, std::false_type{});
+        else if (rmt == 0) chain_loop(std::integral_constant<int, 0>{}, std::true_type{});
+        else chain_loop(std::integral_constant<int, NM - 1>{}, std::true_type{});
         __syncthreads();
         // E[t][m] into LDS (the chunk buffers are free now): lane l holds
         // rows 4v + l/16 of the tile, column l%16.
