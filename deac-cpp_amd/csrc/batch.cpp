@@ -195,19 +195,21 @@ deac_status upload_model(deac_batch *h, BatchModel &b)
                     if (tau < T && slot < L) v = m.kernel[static_cast<size_t>(tau) * L + slot];
                     kt[((static_cast<size_t>(t) * nkb2 + kb2) * 64 + l) * 2 + e] = v;
                 }
-    // Rows ntt*16 .. T-1 for v_mfma_f64_4x4x4_4b: per 8 positions, 16 pairs
-    // (row r + 4 * position k of a 4-position block; the pair holds the two
-    // blocks).  Lane l reads entry (l & 3) + 4 (l >> 4): the A operand, the
-    // same for the four 4x4 blocks.
-    std::vector<double> kr(static_cast<size_t>(nkb2) * 16 * 2, -0.0);
+    // Rows ntt*16 .. T-1 for v_mfma_f64_4x4x4_4b: per 8 positions, 4 nrem
+    // pairs (row r + nrem * position k of a 4-position block; the pair holds
+    // the two blocks).  Lane l with l & 3 < nrem reads entry
+    // (l & 3) + nrem (l >> 4): the A operand, the same for the four 4x4
+    // blocks; the rows past nrem read as zeros (their chains are discarded).
+    const int nr = std::max(b.nrem, 1);
+    std::vector<double> kr(static_cast<size_t>(nkb2) * 4 * nr * 2, -0.0);
     for (int kb2 = 0; kb2 < nkb2; kb2++)
-        for (int idx = 0; idx < 16; idx++)
-            for (int e = 0; e < 2; e++) {
-                const int r = idx & 3, slot = (2 * kb2 + e) * 4 + (idx >> 2);
-                double v = -0.0;
-                if (slot < L) v = r < b.nrem ? m.kernel[static_cast<size_t>(b.ntt * 16 + r) * L + slot] : 0.0;
-                kr[(static_cast<size_t>(kb2) * 16 + idx) * 2 + e] = v;
-            }
+        for (int q = 0; q < 4; q++)
+            for (int r = 0; r < b.nrem; r++)
+                for (int e = 0; e < 2; e++) {
+                    const int slot = (2 * kb2 + e) * 4 + q;
+                    const double v = slot < L ? m.kernel[static_cast<size_t>(b.ntt * 16 + r) * L + slot] : -0.0;
+                    kr[(static_cast<size_t>(kb2) * 4 * nr + r + nr * q) * 2 + e] = v;
+                }
     // Jump tables: T^L by rows for the walk, T^(k*Lsub) by nibbles for the
     // mask sub-streams k = 1..S-1.
     std::vector<uint64_t> rows(256 * 4), tabs(static_cast<size_t>(std::max(b.S - 1, 1)) * 64 * 16 * 4);

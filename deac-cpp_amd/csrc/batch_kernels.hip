@@ -464,12 +464,15 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             ktr[i] = make_rsrc(d.ktile + static_cast<size_t>(tload[i]) * nkb2 * (2 * kWave), nkb2 * (2 * kWave) * 8);
         const uint32_t lob = 16u * lane;
         // Rows past the last full tile (REM): one 4x4x4 MFMA per 4 positions
-        // for member tile rmt = wave % NM, on waves 0 .. NM-1.  The remainder
-        // rows are stored 16 values (4 rows x 4 positions) per 4-position
-        // block, lane l reading row l & 3 at position l >> 4.
-        const rsrc_t krm = make_rsrc(d.krem, nkb2 * 32 * 8);
+        // for member tile rmt = wave % NM, on waves 0 .. NM-1.  The nrem
+        // remainder rows are stored nrem x 4 values per 4-position block,
+        // lane l reading row l & 3 at position l >> 4; lanes of rows past
+        // nrem read past the buffer's end, i.e. zeros, into chains whose
+        // results are discarded.
+        const int nr = max(d.nrem, 1);
+        const rsrc_t krm = make_rsrc(d.krem, nkb2 * nr * 64);
         const int rmt = wave % NM;
-        const uint32_t lrob = ((lane & 3) + 4 * (lane >> 4)) * 16u;
+        const uint32_t lrob = (lane & 3) < nr ? ((lane & 3) + nr * (lane >> 4)) * 16u : 0x40000000u;
         double accr = 0.0;
         const int nwork = chains ? nchunk : 0;
         // One copy of the loop per remainder member tile RT (the 4x4x4 MFMA
@@ -489,7 +492,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                 for (int i = 0; i < NT; i++) kr[p][i] = buffer_load2(ktr[i], lob, min(p, nkb2 - 1) * (2 * kWave * 8));
             if (RM) {
 #pragma unroll
-                for (int p = 0; p < KD; p++) krr[p] = buffer_load2(krm, lrob, min(p, nkb2 - 1) * (32 * 8));
+                for (int p = 0; p < KD; p++) krr[p] = buffer_load2(krm, lrob, min(p, nkb2 - 1) * (nr * 64));
             }
             for (int c = 0; c < min(4, nsteps); c++) step_barrier();
             for (int cm = 0; cm < nwork; cm++) {
@@ -533,7 +536,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     const int gn = min(g2 + KD, nkb2 - 1);
 #pragma unroll
                     for (int i = 0; i < NT; i++) kr[sl][i] = buffer_load2(ktr[i], lob, gn * (2 * kWave * 8));
-                    if (RM) krr[sl] = buffer_load2(krm, lrob, gn * (32 * 8));
+                    if (RM) krr[sl] = buffer_load2(krm, lrob, gn * (nr * 64));
                     __builtin_amdgcn_sched_barrier(0);  // issue the refill here, a full step ahead of its use
                 }
                 step_barrier();

@@ -23,8 +23,8 @@ constexpr int kBatchMaxRem = 4;             // kernel rows past the last full ti
 struct BatchDesc {
     // model, laid out for the kernel by the host
     const double *ktile;       // [ntt][Lpad/8][64 lanes][2]: K tiles in MFMA A order
-    const double *krem;        // [Lpad/8][16][2]: rows past the last full 16-row tile, in
-                               // v_mfma_f64_4x4x4_4b A order (row l&3, position 4k+(l>>4))
+    const double *krem;        // [Lpad/8][4 nrem][2]: rows past the last full 16-row tile, in
+                               // v_mfma_f64_4x4x4_4b A order (row r + nrem * position k)
     const double *obs, *sig;   // [T] ISF and its error
     const uint64_t *walk_rows; // [256][4] rows of T^L, T the xoshiro transition
     const uint64_t *sub_tab;   // [S-1][64][16][4] nibble tables of T^(k*Lsub), k = 1..S-1
