@@ -197,3 +197,19 @@ def test_batch_shape_limits():
         assert err.value.status == 5  # DEAC_ERR_UNSUPPORTED
     with pytest.raises(da.DeacError, match="260 timeslices"):
         da.Batch([_synthetic_problem(261, 96, 32)])
+
+
+@pytest.mark.parametrize("T,P", [(18, 8), (35, 32), (51, 20)])
+def test_batch_remainder_rows_match_plain_engine(T, P):
+    # every count of rows past the last full 16-row tile that runs on the
+    # 4x4x4 remainder chains (T = 16k + 2, 3; 51 = 48 + 3 with 20 members,
+    # two member tiles) against deac_run, bit for bit
+    print(f"[gpu] batch remainder rows: T={T} ({T % 16} rows) P={P} vs deac_run", flush=True)
+    p = _synthetic_problem(T, 80, P)
+    with da.Engine(p) as e:
+        plain = [e.run(sd, 50, 0.0) for sd in (4, 9)]
+    with da.Batch([p] * 2) as b:
+        outs = b.run([4, 9], 50, 0.0, True)
+    for pl, o in zip(plain, outs):
+        assert not _compare(o, pl), _compare(o, pl)
+        assert o["rng"] == pl["rng"]
