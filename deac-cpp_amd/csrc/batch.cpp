@@ -88,7 +88,7 @@ struct deac_batch {
     double *pop = nullptr, *fitness = nullptr, *cr_old = nullptr, *f_old = nullptr;
     uint32_t *masks = nullptr;
     uint64_t *rng = nullptr;
-    int *status = nullptr;
+    long *status = nullptr;                    // per instance: stopped, generation reached
     deac::BatchDesc *descs = nullptr;          // device copy of the launch descriptors
     deac_run_profile last{};
 
@@ -377,6 +377,11 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
                               "dsf and frequency outputs are required";
             return DEAC_ERR_INVALID_ARGUMENT;
         }
+        if ((rng_states[4 * i] | rng_states[4 * i + 1] | rng_states[4 * i + 2] | rng_states[4 * i + 3]) == 0) {
+            // a fixed point of xoshiro256+: the donor draws would never end
+            g_batch_message = "instance " + std::to_string(i) + ": the rng state must not be all zero";
+            return DEAC_ERR_INVALID_ARGUMENT;
+        }
         if (options[i].ks_precision != DEAC_PRECISION_FP64 || options[i].split_loop) {
             g_batch_message = "instance " + std::to_string(i) + ": the batch kernel evaluates K.S in exact FP64 "
                               "(ks_precision 0, no split loop); run mixed precision through deac_run";
@@ -524,8 +529,8 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
             }
         }
         BATCH_HIP(hipEventRecord(h->kev1, s));
-        std::vector<int> st(2 * n);
-        BATCH_HIP(hipMemcpyAsync(st.data(), h->status, st.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        std::vector<long> st(2 * n);
+        BATCH_HIP(hipMemcpyAsync(st.data(), h->status, st.size() * sizeof(long), hipMemcpyDeviceToHost, s));
         BATCH_HIP(hipStreamSynchronize(s));
         float ms = 0.0f;
         BATCH_HIP(hipEventElapsedTime(&ms, h->kev0, h->kev1));

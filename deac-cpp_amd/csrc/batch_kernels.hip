@@ -1000,10 +1000,10 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
     }
     if (threadIdx.x == 0) {
         __attribute__((address_space(1))) uint64_t *rng = (__attribute__((address_space(1))) uint64_t *)d.rng;
-        __attribute__((address_space(1))) int *status = (__attribute__((address_space(1))) int *)d.status;
+        __attribute__((address_space(1))) long *status = (__attribute__((address_space(1))) long *)d.status;
         for (int w = 0; w < 4; w++) rng[w] = sh.gen_start[g & 3][w];
         status[0] = sh.flag == 1 ? 1 : 0;
-        status[1] = static_cast<int>(g);
+        status[1] = g;
     }
 }
 

@@ -35,7 +35,7 @@ struct BatchDesc {
     uint32_t *masks;           // [3][Ppad][2][Lw] crossover bits, two candidates per generation
     uint64_t *rng;             // [4] in: state at the top of gen0; out: at the first generation not evolved
     double *stats;             // [3][stats_len] mean, minimum, squared mean per generation, or null
-    int *status;               // [0] stopped, [1] generation reached
+    long *status;              // [0] stopped, [1] generation reached
     long stats_len;
     long gen0, gen_end;        // evolve generations gen0 .. gen_end-1 unless the stop test fires
     double stop, tau_cr, tau_f;

@@ -372,6 +372,12 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         g_message = "ks_precision must be DEAC_PRECISION_FP64, DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3";
         return DEAC_ERR_INVALID_ARGUMENT;
     }
+    if ((rng_state[0] | rng_state[1] | rng_state[2] | rng_state[3]) == 0) {
+        // the all-zero state is a fixed point of xoshiro256+: every draw is 0
+        // and the donor draws (deac.cpp:200-216) would never end
+        g_message = "rng_state must not be all zero (a fixed point of xoshiro256+)";
+        return DEAC_ERR_INVALID_ARGUMENT;
+    }
     const bool split = o->ks_precision != DEAC_PRECISION_FP64 || o->split_loop != 0;
     const deac::Model &m = h->m;
     const size_t N = o->number_of_generations, P = m.P, Kc = m.Kc;

@@ -6,8 +6,11 @@ frequency grid, the generation count, the best fitness and the per-generation
 statistics that --track_stats writes.  The library is required: importing
 this module without a built libdeac_amd.so raises, there is no fallback.
 """
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -129,6 +132,33 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
+# Engines and batches still open when the interpreter exits are closed by an
+# atexit hook, while the HIP runtime is certainly alive: a handle destroyed
+# later, from a finalizer during interpreter teardown, would reach
+# hipStreamSynchronize/hipFree after the runtime (or a profiler attached to
+# it) has shut down.  Finalizers that run while the interpreter is
+# finalizing leave the handle to the process's exit instead.
+_open = weakref.WeakSet()
+
+
+@atexit.register
+def _close_open_handles():
+    for obj in list(_open):
+        try:
+            obj.close()
+        except Exception:
+            pass
+
+
+def _finalize(obj):
+    if sys.is_finalizing():
+        return
+    try:
+        obj.close()
+    except Exception:
+        pass
+
+
 def seed_state(seed):
     s = (C.c_uint64 * 4)()
     lib().deac_seed_state(C.c_uint64(int(seed)), s)
@@ -171,15 +201,18 @@ class Engine:
         self.problem = problem
         h = C.c_void_p()
         msg = C.c_char_p()
+        self.handle = None
         st = lib().deac_create(C.byref(problem), C.byref(h), C.byref(msg))
         if st:
             raise DeacError(st, msg.value.decode() if msg.value else None)
         self.handle = h
+        _open.add(self)
 
     def close(self):
         if self.handle:
             lib().deac_destroy(self.handle)
             self.handle = None
+        _open.discard(self)
 
     def __enter__(self):
         return self
@@ -188,10 +221,7 @@ class Engine:
         self.close()
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
     def run(self, seed, number_of_generations, stop_minimum_fitness=1.0, track_stats=True, precision="fp64",
             split_loop=False):
@@ -244,15 +274,18 @@ class Batch:
         self._arr = arr
         h = C.c_void_p()
         msg = C.c_char_p()
+        self.handle = None
         st = lib().deac_batch_create(arr, n, C.byref(h), C.byref(msg))
         if st:
             raise DeacError(st, msg.value.decode() if msg.value else None)
         self.handle = h
+        _open.add(self)
 
     def close(self):
         if self.handle:
             lib().deac_batch_destroy(self.handle)
             self.handle = None
+        _open.discard(self)
 
     def __enter__(self):
         return self
@@ -261,10 +294,7 @@ class Batch:
         self.close()
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
     def __len__(self):
         return len(self.problems)
@@ -272,6 +302,8 @@ class Batch:
     def run(self, seeds, number_of_generations, stop_minimum_fitness=1.0, track_stats=True):
         """seeds: list of --seed ints or 4-word states; N, stop, track_stats: scalar or per instance."""
         n = len(self.problems)
+        if len(seeds) != n:
+            raise ValueError(f"one seed or state per problem: {len(seeds)} given for {n} problems")
         Ns = _per_instance(number_of_generations, n)
         stops = _per_instance(stop_minimum_fitness, n)
         tracks = _per_instance(track_stats, n)

@@ -133,7 +133,9 @@ typedef struct deac_handle deac_handle;
 deac_status deac_create(const deac_problem *problem, deac_handle **out, const char **message);
 
 /* Runs one reconstruction: initial population from rng_state (in/out, the
- * 4-word xoshiro256+ state of rng.hpp:21-23), optional normalization, then at
+ * 4-word xoshiro256+ state of rng.hpp:21-23; the all-zero state, a fixed
+ * point of the generator, is DEAC_ERR_INVALID_ARGUMENT), optional
+ * normalization, then at
  * most N-1 generations of the fused GPU pipeline.  Plain FP64: reproduces the
  * reference CPU build's trajectory.  Replaces deac.cpp:430-1685. */
 deac_status deac_run(deac_handle *handle, uint64_t rng_state[4],
@@ -186,8 +188,9 @@ typedef struct deac_batch deac_batch;
 
 deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_batch **out, const char **message);
 
-/* rng_states: 4*count words, in/out as deac_run's rng_state; options and
- * results: count entries each, as deac_run's. */
+/* rng_states: 4*count words, in/out as deac_run's rng_state (no instance's
+ * state may be all zero); options and results: count entries each, as
+ * deac_run's. */
 deac_status deac_batch_run(deac_batch *batch, uint64_t *rng_states, const deac_run_options *options,
                            deac_result *results);
 

@@ -81,6 +81,13 @@ CASES["c2_prefix2000"] = ("standard", "isf_c2_T65.bin", dict(T=1.35, N=2000, P=8
                                                            stop=0.0, normalize=True, spectra="bfull"))
 CASES["c4_seed7_prefix300"] = ("standard", "isf_c2_T65.bin", dict(T=1.35, N=300, P=32, M=4096, omega_max=512.0,
                                                                 seed=7, stop=1.0, normalize=True, spectra="bfull"))
+# long horizons at the headline shapes: config 2 in full, and two config-4
+# seeds (P = 32, stop at chi^2 <= 1) for 2000 generations
+CASES["c2_full"] = ("standard", "isf_c2_T65.bin", dict(T=1.35, N=200000, P=8, M=4096, omega_max=512.0, seed=1,
+                                                     stop=0.0, normalize=True, spectra="bfull"))
+for _seed in (1, 200):
+    CASES[f"c4_seed{_seed}_prefix2000"] = ("standard", "isf_c2_T65.bin", dict(
+        T=1.35, N=2000, P=32, M=4096, omega_max=512.0, seed=_seed, stop=1.0, normalize=True, spectra="bfull"))
 CASES["c3_hyperbolic_prefix4"] = ("hyperbolic", "isf_c3_T257.bin", dict(T=1.35, N=4, P=64, M=16384, omega_max=512.0,
                                                                       seed=1, stop=0.0, spectra="bfull"))
 
@@ -118,7 +125,7 @@ def run_case(name, variant, data, args, workdir):
     arrays = dict(tau=np.asarray(data["tau"]), isf=np.asarray(data["isf"]), err=np.asarray(data["err"]),
                   mean=res["mean"], minimum=res["minimum"], squared_mean=res["squared-mean"], dsf=res["dsf"],
                   frequency_out=res["frequency_out"], meta=np.array(json.dumps(meta)))
-    if name == "c1_full":
+    if name in ("c1_full", "c2_full"):
         # keep the fixture small: the full best-fitness history, 5000
         # generations of the other two histories
         arrays["mean"] = arrays["mean"][:5000]

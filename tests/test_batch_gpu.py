@@ -20,8 +20,11 @@ pytestmark = pytest.mark.gpu
 def batch_compatible(case):
     m = case["meta"]
     a = m["args"]
+    # the complete 200000-generation config-2 run is the plain engine's case
+    # (one instance is a single workgroup here); config 4's long seeds run
+    # inside a 256-seed batch below
     return (a["P"] <= 32 and a.get("first_moment", -1.0) < 0 and a.get("third_moment", -1.0) < 0
-            and m["variant"] != "standard_sp")
+            and m["variant"] != "standard_sp" and a["N"] <= 20000)
 
 
 def _engine_problem(case):
@@ -142,6 +145,31 @@ def test_seed_batch_full_size():
     assert np.all(np.isfinite(avg)) and avg.shape == (2 * 4096 - 1,)
 
 
+def test_seed_batch_config4_long_goldens():
+    # the 256-seed config-4 batch for 2000 generations; seeds 1 and 200
+    # against the reference executable's own 2000-generation runs (stop at
+    # chi^2 <= 1), bit for bit: histories, DSF, frequency, generation
+    print("[gpu] 256-seed batch, 2000 generations vs reference goldens (seeds 1, 200)", flush=True)
+    cases = {sd: gc.load(f"c4_seed{sd}_prefix2000") for sd in (1, 200)}
+    c = cases[1]
+    a = c["meta"]["args"]
+    prob = _engine_problem(c)
+    with da.Batch([prob] * 256) as b:
+        outs = b.run(list(range(1, 257)), a["N"], a["stop"], True)
+    for sd, case in cases.items():
+        errors = gc.compare_outputs(outs[sd - 1], case)
+        assert not errors, (sd, errors)
+    for o in outs:
+        assert np.all(np.diff(o["minimum"]) <= 0)
+
+
+def _oracle_of(x, N, mu):
+    return ob.run(ob.make_problem(x["tau"], x["isf"], x["err"], ob.frequency_grid(x["G"], x["omega_max"]),
+                                  mu.TEMPERATURE, model=x["model"], zero_temperature=x["zero_temperature"],
+                                  population_size=x["P"], normalize=True, spectra_type="bfull",
+                                  number_of_generations=N, stop_minimum_fitness=0.0), x["seed"])
+
+
 def test_uneven_batch():
     # the uneven batch (tests/golden/make_uneven.py): 64 wavevectors x 16
     # seeds of mixed T, G, omega_max, model and zero-temperature kernels in
@@ -165,17 +193,42 @@ def test_uneven_batch():
             ref = e.run(x["seed"], N, 0.0)
         assert not _compare(outs[i], ref), (i, x["k"], x["seed"], _compare(outs[i], ref))
         assert outs[i]["rng"] == ref["rng"]
+    # and against the oracle: one instance of every (T, zero-temperature)
+    # class, the largest genome of each, bit for bit
+    pick = {}
+    for i, x in enumerate(insts):
+        key = (x["T"], x["zero_temperature"], x["model"])
+        if key not in pick or x["G"] > insts[pick[key]]["G"]:
+            pick[key] = i
+    for key, i in sorted(pick.items()):
+        x = insts[i]
+        print(f"[gpu] uneven instance {i} T={x['T']} G={x['G']} {x['model']} zeroT={x['zero_temperature']} "
+              "vs oracle", flush=True)
+        r = _oracle_of(x, N, mu)
+        assert not _compare(outs[i], r), (i, key, _compare(outs[i], r))
+        assert outs[i]["rng"] == r["rng"]
     for x, o in zip(insts, outs):
         nout = x["G"] if x["zero_temperature"] else 2 * x["G"] - 1
         assert o["dsf"].shape == (nout,) and np.all(np.isfinite(o["dsf"]))
         assert np.all(np.diff(o["minimum"]) <= 0)
 
 
-def _synthetic_problem(T, G, P):
+def _synthetic_arrays(T):
     beta = 1.0 / 1.35
     t = np.linspace(0.0, beta / 2, T)
     f = 0.6 * np.exp(-2.0 * t) + 0.4 * np.exp(-9.0 * t)
-    return da.make_problem(t, f, 1e-3 * f, da.frequency_grid(G, 64.0), 1.35, population_size=P, normalize=True)
+    return t, f, 1e-3 * f
+
+
+def _synthetic_problem(T, G, P):
+    t, f, s = _synthetic_arrays(T)
+    return da.make_problem(t, f, s, da.frequency_grid(G, 64.0), 1.35, population_size=P, normalize=True)
+
+
+def _synthetic_oracle(T, G, P, seed, N):
+    t, f, s = _synthetic_arrays(T)
+    return ob.run(ob.make_problem(t, f, s, ob.frequency_grid(G, 64.0), 1.35, population_size=P, normalize=True,
+                                  number_of_generations=N, stop_minimum_fitness=0.0), seed)
 
 
 def test_batch_shape_limits():
@@ -188,9 +241,12 @@ def test_batch_shape_limits():
         plain = [e.run(sd, 40, 0.0) for sd in (2, 3)]
     with da.Batch([p] * 2) as b:
         outs = b.run([2, 3], 40, 0.0, True)
-    for pl, o in zip(plain, outs):
+    for sd, pl, o in zip((2, 3), plain, outs):
         assert not _compare(o, pl), _compare(o, pl)
         assert o["rng"] == pl["rng"]
+        r = _synthetic_oracle(260, 96, 32, sd, 40)
+        assert not _compare(o, r), _compare(o, r)
+        assert o["rng"] == r["rng"]
     for T, P in ((261, 32), (65, 33)):
         with pytest.raises(da.DeacError) as err:
             da.Batch([_synthetic_problem(T, 96, P)])
@@ -210,6 +266,29 @@ def test_batch_remainder_rows_match_plain_engine(T, P):
         plain = [e.run(sd, 50, 0.0) for sd in (4, 9)]
     with da.Batch([p] * 2) as b:
         outs = b.run([4, 9], 50, 0.0, True)
-    for pl, o in zip(plain, outs):
+    for sd, pl, o in zip((4, 9), plain, outs):
         assert not _compare(o, pl), _compare(o, pl)
         assert o["rng"] == pl["rng"]
+        r = _synthetic_oracle(T, 80, P, sd, 50)
+        assert not _compare(o, r), _compare(o, r)
+        assert o["rng"] == r["rng"]
+
+
+def test_all_zero_rng_state_is_refused():
+    # the all-zero xoshiro256+ state is a fixed point (every draw 0, donor
+    # draws without end): refused before anything is launched
+    print("[gpu] all-zero rng state refused", flush=True)
+    p = _synthetic_problem(18, 40, 8)
+    with da.Engine(p) as e:
+        with pytest.raises(da.DeacError) as err:
+            e.run([0, 0, 0, 0], 10, 0.0)
+        assert err.value.status == 1
+        ok = e.run(4, 10, 0.0)                   # the handle still works
+    with da.Batch([p] * 2) as b:
+        with pytest.raises(da.DeacError) as err:
+            b.run([4, [0, 0, 0, 0]], 10, 0.0)
+        assert err.value.status == 1
+        with pytest.raises(ValueError):
+            b.run([4], 10, 0.0)                  # one seed per problem
+        outs = b.run([4, 9], 10, 0.0)
+    assert not _compare(outs[0], ok)

@@ -9,9 +9,10 @@ import pytest
 import golden_cases as gc
 import oracle_binding as ob
 
-# the full 100000-generation history is checked on a prefix here (the prefix
-# of a longer run is itself a complete run) and in full by the GPU tests
-CPU_PREFIX = {"c1_full": 20000}
+# the long histories (config 1 in full, config 2 in full, two 2000-generation
+# config-4 seeds) are checked on a prefix here (the prefix of a longer run is
+# itself a complete run) and in full by the GPU tests
+CPU_PREFIX = {"c1_full": 20000, "c2_full": 400, "c4_seed1_prefix2000": 120, "c4_seed200_prefix2000": 120}
 
 
 @pytest.mark.parametrize("name", gc.names())

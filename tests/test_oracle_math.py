@@ -67,6 +67,64 @@ def test_jump_commutes_with_step():
     assert list(a) == list(b)
 
 
+def _step_matrix():
+    """The xoshiro256+ transition as a 256x256 matrix over GF(2): state bit
+    64w+b is bit b of word w; column j is the step applied to unit vector j
+    (the step is linear over GF(2): xors, a shift and a rotation)."""
+    M = np.zeros((256, 256), dtype=np.uint8)
+    for j in range(256):
+        s = (C.c_uint64 * 4)(0, 0, 0, 0)
+        s[j // 64] = 1 << (j % 64)
+        L.dor_xoshiro256p_next(s)
+        for w in range(4):
+            M[64 * w:64 * w + 64, j] = [(s[w] >> b) & 1 for b in range(64)]
+    return M
+
+
+def _gf2_mul(A, B):
+    # entries of the integer product are at most 256: exact in float64
+    return (A.astype(np.float64) @ B.astype(np.float64)).astype(np.int64).astype(np.uint8) & 1
+
+
+def _apply(M, state):
+    v = np.array([(state[w] >> b) & 1 for w in range(4) for b in range(64)], dtype=np.float64)
+    o = (M.astype(np.float64) @ v).astype(np.int64) & 1
+    return [int(sum(int(o[64 * w + b]) << b for b in range(64))) for w in range(4)]
+
+
+def test_step_matrix_matches_generator():
+    # linearity: the matrix reproduces the generator on arbitrary states
+    M = _step_matrix()
+    for seed in (1, 7, 256):
+        s = (C.c_uint64 * 4)(*ob.seed_state(seed))
+        want = _apply(M, list(s))
+        L.dor_xoshiro256p_next(s)
+        assert list(s) == want
+
+
+def test_jump_is_two_to_the_128_steps():
+    # xoshiro256p_jump (rng.hpp:64-87) advances the state by 2^128 steps:
+    # T^(2^128) from 128 squarings of the one-step matrix.  A wrong jump
+    # polynomial, an xor taken after the step or a dropped final copy all
+    # give a different state.
+    J = _step_matrix()
+    for _ in range(128):
+        J = _gf2_mul(J, J)
+    for seed in (1, 3, 99):
+        s = (C.c_uint64 * 4)(*ob.seed_state(seed))
+        want = _apply(J, list(s))
+        L.dor_xoshiro256p_jump(s)
+        assert list(s) == want
+    # and not a smaller power: 2^127 steps give another state
+    H = _step_matrix()
+    for _ in range(127):
+        H = _gf2_mul(H, H)
+    s = (C.c_uint64 * 4)(*ob.seed_state(1))
+    half = _apply(H, list(s))
+    L.dor_xoshiro256p_jump(s)
+    assert list(s) != half
+
+
 def test_mutant_indices_distinct_and_rejection_order():
     s = ob.seed_state(9)
     ref = [int(v) for v in s]
@@ -231,6 +289,11 @@ def _run(N=60, P=8, G=16, T=6, seed=3, **kw):
     return ob.run(p, seed, want_population=True), p, tau, isf, f
 
 
+def _uniforms(seed, n):
+    s = ob.seed_state(seed)
+    return np.array([(L.dor_xoshiro256p_next(s) >> 11) * 2.0 ** -53 for _ in range(n)])
+
+
 def test_best_fitness_is_monotone_and_min_of_population():
     out, *_ = _run(N=400)
     m = out["minimum"]
@@ -283,3 +346,118 @@ def test_trials_stay_non_negative():
     assert np.all(out["population"] >= 0)
     out, *_ = _run(N=200, allow_negative=True)
     assert np.any(out["population"] < 0)
+
+
+# ---------------------------------------------------------------- zero temperature
+# The reference's ZeroT build does not compile (deac.cpp:345, 397); its
+# intent is a single-sign kernel K = df e^{-tau w} over the positive
+# frequencies with a G-length output (deac.cpp:392-399, 531-535, 1669-1673).
+# These checks pin the oracle's zero-temperature branch of dor_run.
+
+def _zero_t_kernel(tau, f):
+    return _weights(f) * np.exp(-np.outer(tau, f))
+
+
+def test_zero_temperature_initial_population_is_one_draw_per_gene():
+    # CR = 0 and tau_CR = 0 freeze the population: it is the initial one, one
+    # uniform per (member, gene) in member-major order -- not two per gene
+    # as with two signs, and without the zero-frequency copy.
+    P, G = 5, 7
+    out, *_ = _run(N=4, P=P, G=G, zero_temperature=True, crossover_probability=0.0,
+                   self_adapting_crossover_probability=0.0)
+    pop = out["population"]
+    assert pop.shape == (P, 1, G)
+    np.testing.assert_array_equal(pop[:, 0, :].ravel(), _uniforms(3, P * G))
+
+
+def test_zero_temperature_best_fitness_monotone():
+    out, *_ = _run(N=300, zero_temperature=True)
+    m = out["minimum"]
+    assert np.all(np.diff(m) <= 0)
+    assert m[-1] < m[0]
+    assert m[-1] == out["fitness"].min() == out["minimum_fitness"]
+
+
+def test_zero_temperature_fitness_is_chi_square_of_reported_dsf():
+    out, p, tau, isf, f = _run(N=120, zero_temperature=True)
+    G = len(f)
+    dsf, freq = out["dsf"], out["frequency"]
+    assert len(dsf) == G and len(freq) == G
+    np.testing.assert_array_equal(freq, f)                    # no mirror, no -0.0 centre
+    best = out["population"][out["best_index"], 0]
+    np.testing.assert_array_equal(dsf, best)                  # the DSF is the genome
+    E = _zero_t_kernel(tau, f) @ dsf
+    chi = np.sum(((isf - E) / 0.02) ** 2) / len(tau)
+    assert out["minimum_fitness"] == pytest.approx(chi, rel=1e-12)
+    # a two-sign evaluation of the same genome (the standard negative kernel
+    # added) gives a different fitness: the branch really is single-sign
+    beta = 1 / 1.3
+    E2 = E + (_weights(f) * np.exp(np.outer(tau - beta, f))) @ dsf
+    assert abs(np.sum(((isf - E2) / 0.02) ** 2) / len(tau) - chi) > 1e-3 * chi
+
+
+def test_zero_temperature_normalize_uses_single_sign_weights():
+    out, p, tau, isf, f = _run(N=2, zero_temperature=True, normalize=True, crossover_probability=0.0,
+                               self_adapting_crossover_probability=0.0)
+    pop = out["population"][:, 0, :]
+    np.testing.assert_allclose(pop @ _weights(f), isf[0], rtol=1e-13)
+    # the normalization is a rescaling of the one-draw-per-gene population
+    raw = _uniforms(3, pop.size).reshape(pop.shape)
+    ratio = pop / raw
+    np.testing.assert_allclose(ratio, ratio[:, :1] * np.ones_like(ratio), rtol=1e-14)
+
+
+def test_zero_temperature_moment_terms():
+    tau = np.linspace(0, 0.4, 3)
+    f = ob.frequency_grid(9, 6.0)
+    df = _weights(f)
+    p = ob.make_problem(tau, np.ones(3), np.ones(3), f, 1.3, zero_temperature=True)
+    want = {0: df, 1: df * f, 3: df * f ** 4}
+    for which, w in want.items():
+        tp = np.zeros(9)
+        L.dor_moment_terms(C.byref(p), which, ob.ptr(tp), None)
+        np.testing.assert_allclose(tp, w, rtol=1e-15)
+
+
+def test_zero_temperature_moment_constraints_enter_fitness():
+    # first and third moments of the best genome under the single-sign terms
+    # reproduce the reported fitness (deac.cpp:1550-1555, CPU formula)
+    m1, m3, e3 = 0.8, 2.5, 0.4
+    out, p, tau, isf, f = _run(N=80, zero_temperature=True, first_moment=m1, third_moment=m3,
+                               third_moment_error=e3)
+    best = out["population"][out["best_index"], 0]
+    df = _weights(f)
+    E = _zero_t_kernel(tau, f) @ best
+    fit = np.sum(((isf - E) / 0.02) ** 2) / len(tau)
+    fit += (best @ (df * f) - m1) ** 2 / m1
+    fit += ((m3 - best @ (df * f ** 4)) / e3) ** 2
+    assert out["minimum_fitness"] == pytest.approx(fit, rel=1e-12)
+
+
+@pytest.mark.parametrize("tau_c", [0.0, 1.0])
+def test_zero_temperature_draws_per_generation(tau_c):
+    # One generation consumes the jDE draws of ONE sign for every member (a
+    # check for CR and one for F, each followed by a redraw when it passes:
+    # deac.cpp:1180-1213), then per member the donor draws by rejection
+    # (deac.cpp:200-216) and G crossover draws (deac.cpp:1231-1240); the
+    # initial population took P*G.  The returned state must sit exactly that
+    # far along the stream: two-sign jDE draws or 2G crossover draws land
+    # elsewhere.
+    P, G, seed = 6, 11, 5
+    out, *_ = _run(N=2, P=P, G=G, seed=seed, zero_temperature=True, self_adapting_crossover_probability=tau_c,
+                   self_adapting_differential_weight_probability=tau_c)
+    s = ob.seed_state(seed)
+    nxt = lambda: L.dor_xoshiro256p_next(s)
+    for _ in range(P * G):
+        nxt()
+    for _ in range(P * (4 if tau_c == 1.0 else 2)):
+        nxt()
+    for i in range(P):
+        got = []
+        while len(got) < 3:
+            x = nxt() % P
+            if x != i and x not in got:
+                got.append(x)
+        for _ in range(G):
+            nxt()
+    assert out["rng"] == [int(v) for v in s]

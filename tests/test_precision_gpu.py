@@ -99,3 +99,37 @@ def test_mixed_precision_model_isf_error(prec, tol):
     rel = np.max(np.abs(E - E64) / np.abs(E64))
     print(f"[gpu] {prec}: max relative E error {rel:.3e} (L = {2 * G})", flush=True)
     assert rel < tol
+
+
+@pytest.mark.parametrize("prec,tol_isf,tol_chi", [("fp32", 5e-6, 5e-3), ("bf16x3", 1e-5, 2e-2)])
+def test_mixed_precision_at_config3_shape(prec, tol_isf, tol_chi):
+    # K.S at the precision sweep's own shape: T = 257, G = 16384 per sign,
+    # hyperbolic model, bfull.  64 genomes near the solution (the true
+    # spectrum on the genome grid as 2 S(w) e^{beta w / 2} on both signs,
+    # 0.1 % multiplicative noise): every model-ISF element within tol_isf
+    # (relative) of FP64, and |dchi^2|/chi^2 within tol_chi (the residuals
+    # are (O - E) / sigma with sigma = 1e-4 F, so a relative E error d moves
+    # each by about d 1e4)
+    import os
+    import sys
+    sys.path.insert(0, gc.GOLDEN_DIR)
+    import make_inputs as mi
+    tau, f, s = np.fromfile(os.path.join(gc.GOLDEN_DIR, "isf_c3_T257.bin")).reshape(3, -1)
+    G, temp, wmax = 16384, 1.35, 512.0
+    beta = 1.0 / temp
+    w = da.frequency_grid(G, wmax)
+    grid = np.linspace(0.0, wmax, 65536)
+    norm = np.trapezoid(sum(wt * mi.gaussian(grid, mu, sd) for mu, sd, wt in mi.CONFIG2_PEAKS), grid)
+    S = sum(wt * mi.gaussian(w, mu, sd) for mu, sd, wt in mi.CONFIG2_PEAKS) / norm
+    g0 = np.concatenate([2.0 * S * np.exp(0.5 * beta * w)] * 2)
+    pop = g0 * (1.0 + 1e-3 * np.random.default_rng(3).standard_normal((64, 2 * G)))
+    dp = da.make_problem(tau, f, s, w, temp, model="hyperbolic", spectra_type="bfull", population_size=64,
+                         normalize=True)
+    E64, c64, _ = da.model_isf(dp, pop, "fp64")
+    E, c, _ = da.model_isf(dp, pop, prec)
+    rel_isf = np.max(np.abs(E - E64) / np.abs(E64))
+    rel_chi = np.abs(c - c64) / c64
+    print(f"[gpu] T=257 G=16384 {prec}: max rel E error {rel_isf:.3e}, |dchi2|/chi2 median "
+          f"{np.median(rel_chi):.3e} max {np.max(rel_chi):.3e} (chi2 ~ {np.median(c64):.3g})", flush=True)
+    assert rel_isf < tol_isf
+    assert np.max(rel_chi) < tol_chi
