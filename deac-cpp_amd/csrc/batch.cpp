@@ -23,10 +23,10 @@
 #include "batch_kernels.hpp"
 #include "deac_engine.h"
 #include "host_model.hpp"
+#include "ks_layout.hpp"
 
 namespace {
 
-thread_local std::string g_batch_message;
 
 #define BATCH_HIP(expr)                        \
     do {                                       \
@@ -40,8 +40,11 @@ thread_local std::string g_batch_message;
 struct BatchModel {
     deac::Model m;
     int Ppad = 0, NM = 0, NT = 0, C = 0, nchunk = 0, Lpad = 0, Lw = 0, ntt = 0, nrem = 0, S = 0, Lsub = 0;
+    int ntm = 0, NQ = 0;                   // mixed K.S: kernel tiles (rows past T zero), pairs per chain wave
     double *ktile = nullptr, *krem = nullptr, *obs = nullptr, *sig = nullptr;
     uint64_t *walk_rows = nullptr, *sub_tab = nullptr;
+    void *kmix[3] = {nullptr, nullptr, nullptr};  // mixed K tiles by precision, built on first use
+    double *kscale = nullptr;                     // 2^ex[t] of the mixed K rows
 };
 
 // Byte image of everything a problem's results depend on.
@@ -84,8 +87,9 @@ struct deac_batch {
     std::vector<int> model_of;                 // instance -> model
     std::vector<void *> allocations;
     // per-instance state, packed
-    std::vector<size_t> pop_off, mask_off;     // offsets into the pools
+    std::vector<size_t> pop_off, mask_off, epar_off;  // offsets into the pools
     double *pop = nullptr, *fitness = nullptr, *cr_old = nullptr, *f_old = nullptr;
+    double *epar = nullptr;                    // per instance [T][Ppad]: the members' model ISF (mixed K.S)
     uint32_t *masks = nullptr;
     uint64_t *rng = nullptr;
     long *status = nullptr;                    // per instance: stopped, generation reached
@@ -177,6 +181,67 @@ deac_status plan_model(BatchModel &b, std::string &why)
     }
     b.S = deac::kBatchMaskLanes / b.Ppad;
     b.Lsub = (L + 32 * b.S - 1) / (32 * b.S) * 32;
+    b.ntm = (T + 15) / 16;
+    b.NQ = deac::batch_mixed_nq(b.ntm, b.NM);
+    return DEAC_OK;
+}
+
+// K in the mixed kernel's operand order (batch_kernels.hpp, BatchDesc::kmix),
+// each row scaled by 2^-ex[t] with ex[t] the exponent of its largest
+// magnitude (the split loop's scaling, ks_layout.hpp), and 2^ex[t] itself.
+deac_status upload_mixed(deac_batch *h, BatchModel &b, int precision)
+{
+    if (b.kmix[precision]) return DEAC_OK;
+    const deac::Model &m = b.m;
+    const size_t T = m.T, L = m.Kc;
+    std::vector<int> ex(T, 0);
+    for (size_t t = 0; t < T; t++) {
+        double mx = 0.0;
+        for (size_t k = 0; k < L; k++) mx = std::fmax(mx, std::fabs(m.kernel[t * L + k]));
+        ex[t] = mx > 0.0 ? std::ilogb(mx) : 0;
+    }
+    auto scaled = [&](size_t row, size_t pos) {
+        return row < T && pos < L ? std::ldexp(m.kernel[row * L + pos], -ex[row]) : 0.0;
+    };
+    const size_t tile = static_cast<size_t>(b.Lpad) * 16;  // values per 16-row tile
+    std::vector<uint32_t> img(static_cast<size_t>(b.ntm) * tile, 0);  // 4 bytes per value either way
+    if (precision == DEAC_PRECISION_FP32) {
+        // [tile][unit of 16 positions][lane][4]: row 16 tile + l % 16, position 16 u + 4 e + l / 16
+        float *f = reinterpret_cast<float *>(img.data());
+        for (int tt = 0; tt < b.ntm; tt++)
+            for (int u = 0; u < b.Lpad / 16; u++)
+                for (int l = 0; l < 64; l++)
+                    for (int e = 0; e < 4; e++)
+                        f[((static_cast<size_t>(tt) * (b.Lpad / 16) + u) * 64 + l) * 4 + e] = static_cast<float>(
+                            scaled(16 * tt + (l & 15), 16 * static_cast<size_t>(u) + 4 * e + (l >> 4)));
+    } else {
+        // [tile][32-position block][lane][8 hi, 8 lo]: row 16 tile + l % 16, position 32 u + 8 (l / 16) + e
+        uint16_t *q = reinterpret_cast<uint16_t *>(img.data());
+        for (int tt = 0; tt < b.ntm; tt++)
+            for (int u = 0; u < b.Lpad / 32; u++)
+                for (int l = 0; l < 64; l++)
+                    for (int e = 0; e < 8; e++) {
+                        uint16_t hi, lo;
+                        deac::bf16_split(scaled(16 * tt + (l & 15), 32 * static_cast<size_t>(u) + 8 * (l >> 4) + e),
+                                         hi, lo);
+                        const size_t o = ((static_cast<size_t>(tt) * (b.Lpad / 32) + u) * 64 + l) * 16 + e;
+                        q[o] = hi;
+                        q[o + 8] = lo;
+                    }
+    }
+    uint32_t *dk = nullptr;
+    deac_status st = h->alloc(dk, img.size());
+    if (st != DEAC_OK) return st;
+    if (hipMemcpy(dk, img.data(), img.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+        return DEAC_ERR_DEVICE;
+    if (!b.kscale) {
+        std::vector<double> sc(T);
+        for (size_t t = 0; t < T; t++) sc[t] = std::ldexp(1.0, ex[t]);
+        if ((st = h->alloc(b.kscale, T)) != DEAC_OK) return st;
+        if (hipMemcpy(b.kscale, sc.data(), T * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return DEAC_ERR_DEVICE;
+    }
+    b.kmix[precision] = dk;
     return DEAC_OK;
 }
 
@@ -270,16 +335,16 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
     if (!out) return DEAC_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     if (!problems || count == 0 || count > (size_t(1) << 20)) {
-        g_batch_message = "a batch holds 1 to 1048576 problems";
-        if (message) *message = g_batch_message.c_str();
+        deac::last_message() = "a batch holds 1 to 1048576 problems";
+        if (message) *message = deac::last_message().c_str();
         return DEAC_ERR_INVALID_ARGUMENT;
     }
     for (size_t i = 0; i < count; i++) {
         std::string why;
         const deac_status v = deac::validate(problems[i], why);
         if (v != DEAC_OK) {
-            g_batch_message = "problem " + std::to_string(i) + ": " + why;
-            if (message) *message = g_batch_message.c_str();
+            deac::last_message() = "problem " + std::to_string(i) + ": " + why;
+            if (message) *message = deac::last_message().c_str();
             return v;
         }
     }
@@ -299,8 +364,8 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
         std::string why;
         const deac_status st = plan_model(*b, why);
         if (st != DEAC_OK) {
-            g_batch_message = "problem " + std::to_string(i) + ": " + why;
-            if (message) *message = g_batch_message.c_str();
+            deac::last_message() = "problem " + std::to_string(i) + ": " + why;
+            if (message) *message = deac::last_message().c_str();
             return st;
         }
         seen.emplace(key, static_cast<int>(h->models.size()));
@@ -329,15 +394,18 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
     }
 
     // Per-instance state pools.
-    size_t npop = 0, nmask = 0;
+    size_t npop = 0, nmask = 0, nepar = 0;
     h->pop_off.resize(count);
     h->mask_off.resize(count);
+    h->epar_off.resize(count);
     for (size_t i = 0; i < count; i++) {
         const BatchModel &b = *h->models[h->model_of[i]];
         h->pop_off[i] = npop;
         h->mask_off[i] = nmask;
+        h->epar_off[i] = nepar;
         npop += b.m.P * static_cast<size_t>(b.Lpad);
         nmask += static_cast<size_t>(3) * b.Ppad * 2 * b.Lw;
+        nepar += b.m.T * static_cast<size_t>(b.Ppad);
     }
     deac_status st = DEAC_OK;
     auto A = [&](auto *&p, size_t n) {
@@ -350,7 +418,8 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
     A(h->f_old, count * deac::kBatchMaxP * 2);
     A(h->rng, count * 4);
     A(h->status, count * 2);
-    A(h->descs, count);
+    A(h->epar, nepar);
+    A(h->descs, 2 * count);  // evolution launches, and the evaluation launches of mixed K.S
     if (st != DEAC_OK) return st;
     *out = h.release();
     return DEAC_OK;
@@ -373,23 +442,38 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
     const size_t n = h->model_of.size();
     for (size_t i = 0; i < n; i++) {
         if (options[i].number_of_generations < 2 || !results[i].dsf || !results[i].frequency) {
-            g_batch_message = "instance " + std::to_string(i) + ": number_of_generations must be at least 2 and the "
+            deac::last_message() = "instance " + std::to_string(i) + ": number_of_generations must be at least 2 and the "
                               "dsf and frequency outputs are required";
             return DEAC_ERR_INVALID_ARGUMENT;
         }
         if ((rng_states[4 * i] | rng_states[4 * i + 1] | rng_states[4 * i + 2] | rng_states[4 * i + 3]) == 0) {
             // a fixed point of xoshiro256+: the donor draws would never end
-            g_batch_message = "instance " + std::to_string(i) + ": the rng state must not be all zero";
+            deac::last_message() = "instance " + std::to_string(i) + ": the rng state must not be all zero";
             return DEAC_ERR_INVALID_ARGUMENT;
         }
-        if (options[i].ks_precision != DEAC_PRECISION_FP64 || options[i].split_loop) {
-            g_batch_message = "instance " + std::to_string(i) + ": the batch kernel evaluates K.S in exact FP64 "
-                              "(ks_precision 0, no split loop); run mixed precision through deac_run";
+        const int prec = options[i].ks_precision;
+        if (prec != DEAC_PRECISION_FP64 && prec != DEAC_PRECISION_FP32 && prec != DEAC_PRECISION_BF16X3) {
+            deac::last_message() = "instance " + std::to_string(i) + ": ks_precision must be DEAC_PRECISION_FP64, "
+                              "DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3";
+            return DEAC_ERR_INVALID_ARGUMENT;
+        }
+        if (options[i].split_loop) {
+            deac::last_message() = "instance " + std::to_string(i) + ": the batch kernel has no split loop (run it "
+                              "through deac_run)";
+            return DEAC_ERR_UNSUPPORTED;
+        }
+        if (prec != DEAC_PRECISION_FP64 && h->models[h->model_of[i]]->NQ == 0) {
+            deac::last_message() = "instance " + std::to_string(i) + ": no mixed K.S kernel for this shape";
             return DEAC_ERR_UNSUPPORTED;
         }
     }
     BATCH_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
+    for (size_t i = 0; i < n; i++)
+        if (options[i].ks_precision != DEAC_PRECISION_FP64) {
+            const deac_status su = upload_mixed(h, *h->models[h->model_of[i]], options[i].ks_precision);
+            if (su != DEAC_OK) return su;
+        }
 
     // Initial populations on the host (deac.cpp:430-603), each from its own
     // stream; the state after them is where the device walk starts.
@@ -434,22 +518,37 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
 
     // Launch slices: every active instance evolves up to `slice` generations
     // per launch; stopped or finished instances leave the next launch.
-    long slice = 4096;
+    // Mixed K.S instances run slices of at most `refresh` generations, each
+    // followed by an exact FP64 evaluation of the population, which resets
+    // the parent model ISF the delta form builds on (and makes the final
+    // fitness exact); one more such evaluation precedes their first slice.
+    long slice = 4096, refresh = 1024;
     if (const char *e = std::getenv("DEAC_BATCH_SLICE")) slice = std::max(1L, std::atol(e));
+    if (const char *e = std::getenv("DEAC_BATCH_REFRESH")) refresh = std::max(1L, std::atol(e));
     std::vector<long> next_gen(n, 0), stopped_at(n, -1);
     std::vector<char> active(n, 1);
-    std::vector<deac::BatchDesc> descs(n);
+    std::vector<deac::BatchDesc> descs(2 * n);
     deac_run_profile prof{};
     BATCH_HIP(hipEventRecord(h->ev0, s));
     bool first = true;
+    // Launch key: (precision, NT or NQ, NM, remainder rows).
+    typedef std::tuple<int, int, int, int> Key;
+    auto key_of = [&](size_t i, bool eval) -> Key {
+        const BatchModel &b = *h->models[h->model_of[i]];
+        const int prec = options[i].ks_precision;
+        if (prec == DEAC_PRECISION_FP64 || eval) return Key{DEAC_PRECISION_FP64, b.NT, b.NM, b.nrem > 0 ? 1 : 0};
+        return Key{prec, b.NQ, b.NM, 0};
+    };
+    auto launch = [&](const Key &k, const deac::BatchDesc *dd, int cnt, hipStream_t st) {
+        if (std::get<0>(k) == DEAC_PRECISION_FP64)
+            return deac::launch_batch(dd, cnt, std::get<1>(k), std::get<2>(k), std::get<3>(k), st);
+        return deac::launch_batch_mixed(dd, cnt, std::get<0>(k), std::get<1>(k), std::get<2>(k), st);
+    };
     for (;;) {
         // Group the active instances by kernel instantiation.
-        std::map<std::tuple<int, int, int>, std::vector<size_t>> groups;
+        std::map<Key, std::vector<size_t>> groups;
         for (size_t i = 0; i < n; i++)
-            if (active[i]) {
-                const BatchModel &b = *h->models[h->model_of[i]];
-                groups[{b.NT, b.NM, b.nrem > 0 ? 1 : 0}].push_back(i);
-            }
+            if (active[i]) groups[key_of(i, false)].push_back(i);
         if (groups.empty()) break;
         // Longest instances first within a group (workgroups are dispatched
         // in index order): per-generation cost ~ chunks x tiles per wave.
@@ -460,17 +559,21 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
                        static_cast<long>(mb.nchunk) * (mb.NT * 4 + mb.nrem);
             });
         size_t slot = 0;
-        std::vector<std::pair<std::tuple<int, int, int>, std::pair<size_t, size_t>>> launches;
+        std::vector<std::pair<Key, std::pair<size_t, size_t>>> launches, evals_before, evals_after;
         std::vector<size_t> order;
+        std::vector<size_t> mixed_slots;  // evolution slots of mixed instances
         for (auto &[key, members] : groups) {
             const size_t begin = slot;
             for (size_t i : members) {
                 const BatchModel &b = *h->models[h->model_of[i]];
                 const deac::Model &m = b.m;
-                deac::BatchDesc &d = descs[slot++];
+                const int prec = options[i].ks_precision;
+                deac::BatchDesc &d = descs[slot];
                 d = deac::BatchDesc{};
                 d.ktile = b.ktile;
                 d.krem = b.krem;
+                d.kmix = prec == DEAC_PRECISION_FP64 ? nullptr : b.kmix[prec];
+                d.kscale = b.kscale;
                 d.obs = b.obs;
                 d.sig = b.sig;
                 d.walk_rows = b.walk_rows;
@@ -481,16 +584,19 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.f_old = h->f_old + i * deac::kBatchMaxP * 2;
                 d.masks = h->masks + h->mask_off[i];
                 d.rng = h->rng + 4 * i;
+                d.epar = prec == DEAC_PRECISION_FP64 ? nullptr : h->epar + h->epar_off[i];
                 d.stats = stats[i];
                 d.status = h->status + 2 * i;
                 d.stats_len = static_cast<long>(options[i].number_of_generations);
                 d.gen0 = next_gen[i];
-                d.gen_end = std::min<long>(next_gen[i] + slice, static_cast<long>(options[i].number_of_generations) - 1);
+                const long len = prec == DEAC_PRECISION_FP64 ? slice : std::min(slice, refresh);
+                d.gen_end = std::min<long>(next_gen[i] + len, static_cast<long>(options[i].number_of_generations) - 1);
                 d.stop = options[i].stop_minimum_fitness;
                 d.tau_cr = m.tau_cr;
                 d.tau_f = m.tau_f;
                 d.mod_magic = ~UINT64_C(0) / m.P;
-                d.init = first ? 1 : 0;
+                d.init = (first && prec == DEAC_PRECISION_FP64) ? 1 : 0;
+                d.mode = 0;
                 d.T = static_cast<int>(m.T);
                 d.G = static_cast<int>(m.G);
                 d.nsign = m.nsign;
@@ -500,34 +606,60 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.nchunk = b.nchunk;
                 d.ntt = b.ntt;
                 d.nrem = b.nrem;
+                d.ntm = b.ntm;
                 d.P = static_cast<int>(m.P);
                 d.S = b.S;
                 d.Lsub = b.Lsub;
                 d.allow_neg = m.allow_negative ? 1 : 0;
+                if (prec != DEAC_PRECISION_FP64) mixed_slots.push_back(slot);
                 order.push_back(i);
+                slot++;
             }
             launches.push_back({key, {begin, slot - begin}});
         }
+        // Evaluation launches (FP64 kernel, mode 1) of the mixed instances,
+        // grouped by the FP64 kernel shape: before the first slice (initial
+        // fitness and epar) and after every slice.
+        {
+            std::map<Key, std::vector<size_t>> eg;
+            for (size_t k : mixed_slots) eg[key_of(order[k], true)].push_back(k);
+            const size_t nevo = slot;
+            for (auto &[key, ks] : eg) {
+                const size_t begin = slot;
+                for (size_t k : ks) {
+                    deac::BatchDesc &e = descs[slot++];
+                    e = descs[k];
+                    e.mode = 1;
+                    e.init = 1;
+                }
+                evals_after.push_back({key, {begin, slot - begin}});
+                if (first) evals_before.push_back({key, {begin, slot - begin}});
+            }
+            (void)nevo;
+        }
         BATCH_HIP(hipMemcpyAsync(h->descs, descs.data(), slot * sizeof(deac::BatchDesc), hipMemcpyHostToDevice, s));
         BATCH_HIP(hipEventRecord(h->kev0, s));
+        for (auto &[key, range] : evals_before)
+            BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), s));
         if (launches.size() == 1) {
             auto &[key, range] = launches[0];
-            BATCH_HIP(deac::launch_batch(h->descs + range.first, static_cast<int>(range.second), std::get<0>(key),
-                                         std::get<1>(key), std::get<2>(key), s));
+            BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), s));
         } else {
-            // at most 2 x 2 x 3 shapes; groups beyond the streams share them
-            for (int g = 0; g < deac_batch::kGroupStreams; g++) BATCH_HIP(hipStreamWaitEvent(h->gstream[g], h->kev0, 0));
+            // shapes side by side, on the group streams
+            BATCH_HIP(hipEventRecord(h->kev1, s));
+            for (int g = 0; g < deac_batch::kGroupStreams; g++) BATCH_HIP(hipStreamWaitEvent(h->gstream[g], h->kev1, 0));
             for (size_t k = 0; k < launches.size(); k++) {
                 auto &[key, range] = launches[k];
-                BATCH_HIP(deac::launch_batch(h->descs + range.first, static_cast<int>(range.second), std::get<0>(key),
-                                             std::get<1>(key), std::get<2>(key),
-                                             h->gstream[k % deac_batch::kGroupStreams]));
+                BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second),
+                                 h->gstream[k % deac_batch::kGroupStreams]));
             }
             for (int g = 0; g < deac_batch::kGroupStreams; g++) {
                 BATCH_HIP(hipEventRecord(h->gdone[g], h->gstream[g]));
                 BATCH_HIP(hipStreamWaitEvent(s, h->gdone[g], 0));
             }
         }
+        for (auto &[key, range] : evals_after)
+            BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), s));
         BATCH_HIP(hipEventRecord(h->kev1, s));
         std::vector<long> st(2 * n);
         BATCH_HIP(hipMemcpyAsync(st.data(), h->status, st.size() * sizeof(long), hipMemcpyDeviceToHost, s));
@@ -535,8 +667,8 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
         float ms = 0.0f;
         BATCH_HIP(hipEventElapsedTime(&ms, h->kev0, h->kev1));
         prof.evolve_ms += ms;
-        prof.evolve_launches += launches.size();
-        for (size_t k = 0; k < slot; k++) {
+        prof.evolve_launches += launches.size() + evals_before.size() + evals_after.size();
+        for (size_t k = 0; k < order.size(); k++) {
             const size_t i = order[k];
             const long reached = st[2 * i + 1];
             // the kernel ends an instance inside its slice: anything else is

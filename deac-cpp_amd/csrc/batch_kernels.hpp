@@ -1,6 +1,8 @@
 // Batched DEAC kernel: many independent reconstructions in one launch, one
-// persistent workgroup per reconstruction, plain FP64 arithmetic in the
-// reference's order (bit-identical to deac_run for every instance).
+// persistent workgroup per reconstruction.  Exact FP64: the reference's
+// arithmetic and order (bit-identical to deac_run for every instance); mixed
+// K.S (FP32, BF16X3): the delta form on an FP64 parent model ISF, chi^2 and
+// selection in FP64.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -25,6 +27,9 @@ struct BatchDesc {
     const double *ktile;       // [ntt][Lpad/8][64 lanes][2]: K tiles in MFMA A order
     const double *krem;        // [Lpad/8][4 nrem][2]: rows past the last full 16-row tile, in
                                // v_mfma_f64_4x4x4_4b A order (row r + nrem * position k)
+    const void *kmix;          // mixed K.S: [ntm][Lpad/16][64][4] floats (FP32) or [ntm][Lpad/32][64][8 hi, 8 lo]
+                               // bfloat16 (BF16X3), rows scaled by 2^-ex[t], zero past T and L
+    const double *kscale;      // mixed K.S: [T] 2^ex[t]
     const double *obs, *sig;   // [T] ISF and its error
     const uint64_t *walk_rows; // [256][4] rows of T^L, T the xoshiro transition
     const uint64_t *sub_tab;   // [S-1][64][16][4] nibble tables of T^(k*Lsub), k = 1..S-1
@@ -34,18 +39,28 @@ struct BatchDesc {
     double *cr_old, *f_old;    // [P][2] self-adapted CR and F per sign
     uint32_t *masks;           // [3][Ppad][2][Lw] crossover bits, two candidates per generation
     uint64_t *rng;             // [4] in: state at the top of gen0; out: at the first generation not evolved
+    double *epar;              // [T][Ppad] model ISF of each member's genome, FP64 (mixed K.S; written by
+                               // evaluation launches), or null
     double *stats;             // [3][stats_len] mean, minimum, squared mean per generation, or null
     long *status;              // [0] stopped, [1] generation reached
     long stats_len;
     long gen0, gen_end;        // evolve generations gen0 .. gen_end-1 unless the stop test fires
     double stop, tau_cr, tau_f;
     uint64_t mod_magic;        // floor((2^64-1)/P)
-    int init;                  // evaluate the initial population first (gen0 == 0)
-    int T, G, nsign, L, Lpad, Lw, nchunk, ntt, nrem, P, S, Lsub, allow_neg;
+    int init;                  // evaluate the initial population first (gen0 == 0; FP64 kernel only)
+    int mode;                  // 0: evolve; 1: evaluation only (FP64 kernel, init set): fitness and epar of the
+                               // stored population, nothing else written
+    int T, G, nsign, L, Lpad, Lw, nchunk, ntt, nrem, ntm, P, S, Lsub, allow_neg;
 };
 
-// NT: 16-row kernel tiles per chain wavefront (1, 2 or 4); NM: 16-member tiles (1 or 2);
-// rem: 1 when the kernel has 1..4 rows past its last full tile.
+// Exact FP64 kernel.  NT: 16-row kernel tiles per chain wavefront (1, 2 or 4); NM: 16-member
+// tiles (1 or 2); rem: 1 when the kernel has 1..4 rows past its last full tile.
 hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, int rem, hipStream_t s);
+
+// Mixed K.S kernel (precision DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3).  nq: (kernel
+// tile, member tile) pairs per chain wavefront, one of kBatchMixedNq; nm as above.
+constexpr int kBatchMixedNq[] = {1, 2, 3, 5, 9};
+int batch_mixed_nq(int ntm, int nm);  // smallest instantiated nq for ntm kernel tiles (0: none)
+hipError_t launch_batch_mixed(const BatchDesc *descs, int count, int precision, int nq, int nm, hipStream_t s);
 
 }  // namespace deac

@@ -1,0 +1,60 @@
+// Mixed-precision K.S instantiations of the batched DEAC kernel
+// (batch_device.hpp): FP32 and BF16X3 delta-form contractions.
+#include "batch_device.hpp"
+
+namespace deac {
+
+namespace {
+
+template <int PREC, int NQ, int NM>
+hipError_t launch_mixed_one(const BatchDesc *descs, int count, hipStream_t s)
+{
+    constexpr size_t lds_bytes = static_cast<size_t>(kBatchRegionDoubles) * sizeof(double);
+    static bool configured = false;
+    if (!configured) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<PREC, NQ, NM, 0>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes));
+        if (e != hipSuccess) return e;
+        configured = true;
+    }
+    hipLaunchKernelGGL((batch_kernel<PREC, NQ, NM, 0>), dim3(count), dim3(kBatchThreads), lds_bytes, s, descs);
+    return hipGetLastError();
+}
+
+template <int PREC, int NM>
+hipError_t launch_mixed_nq(const BatchDesc *descs, int count, int nq, hipStream_t s)
+{
+    switch (nq) {
+    case 1: return launch_mixed_one<PREC, 1, NM>(descs, count, s);
+    case 2: return launch_mixed_one<PREC, 2, NM>(descs, count, s);
+    case 3: return launch_mixed_one<PREC, 3, NM>(descs, count, s);
+    case 5: return launch_mixed_one<PREC, 5, NM>(descs, count, s);
+    case 9: if constexpr (NM == 2) return launch_mixed_one<PREC, 9, NM>(descs, count, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+int batch_mixed_nq(int ntm, int nm)
+{
+    // pairs per chain wave: ceil(ntm / 2) with two member tiles (a wave keeps
+    // one member tile), ceil(ntm / 4) with one
+    const int need = nm == 2 ? (ntm + 1) / 2 : (ntm + 3) / 4;
+    for (int nq : kBatchMixedNq)
+        if (nq >= need && (nm == 2 || nq <= 5)) return nq;
+    return 0;
+}
+
+hipError_t launch_batch_mixed(const BatchDesc *descs, int count, int precision, int nq, int nm, hipStream_t s)
+{
+    if (precision == kPrecFp32)
+        return nm == 1 ? launch_mixed_nq<kPrecFp32, 1>(descs, count, nq, s)
+                       : launch_mixed_nq<kPrecFp32, 2>(descs, count, nq, s);
+    if (precision == kPrecBf16)
+        return nm == 1 ? launch_mixed_nq<kPrecBf16, 1>(descs, count, nq, s)
+                       : launch_mixed_nq<kPrecBf16, 2>(descs, count, nq, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace deac

@@ -18,7 +18,6 @@
 
 namespace {
 
-thread_local std::string g_message;
 
 struct DeviceBuffer {
     void *ptr = nullptr;
@@ -196,6 +195,8 @@ const char *deac_status_string(deac_status s)
     return "unknown status";
 }
 
+const char *deac_last_error_message(void) { return deac::last_message().c_str(); }
+
 size_t deac_output_length(const deac_problem *p)
 {
     if (!p) return 0;
@@ -225,9 +226,9 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
     if (message) *message = nullptr;
     if (!p || !out) return DEAC_ERR_INVALID_ARGUMENT;
     *out = nullptr;
-    const deac_status v = deac::validate(*p, g_message);
+    const deac_status v = deac::validate(*p, deac::last_message());
     if (v != DEAC_OK) {
-        if (message) *message = g_message.c_str();
+        if (message) *message = deac::last_message().c_str();
         return v;
     }
 
@@ -364,18 +365,18 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
 {
     if (!h || !rng_state || !o || !r || !r->dsf || !r->frequency) return DEAC_ERR_INVALID_ARGUMENT;
     if (o->number_of_generations < 2) {
-        g_message = "number_of_generations must be at least 2";
+        deac::last_message() = "number_of_generations must be at least 2";
         return DEAC_ERR_INVALID_ARGUMENT;
     }
     if (o->ks_precision != DEAC_PRECISION_FP64 && o->ks_precision != DEAC_PRECISION_FP32 &&
         o->ks_precision != DEAC_PRECISION_BF16X3) {
-        g_message = "ks_precision must be DEAC_PRECISION_FP64, DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3";
+        deac::last_message() = "ks_precision must be DEAC_PRECISION_FP64, DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3";
         return DEAC_ERR_INVALID_ARGUMENT;
     }
     if ((rng_state[0] | rng_state[1] | rng_state[2] | rng_state[3]) == 0) {
         // the all-zero state is a fixed point of xoshiro256+: every draw is 0
         // and the donor draws (deac.cpp:200-216) would never end
-        g_message = "rng_state must not be all zero (a fixed point of xoshiro256+)";
+        deac::last_message() = "rng_state must not be all zero (a fixed point of xoshiro256+)";
         return DEAC_ERR_INVALID_ARGUMENT;
     }
     const bool split = o->ks_precision != DEAC_PRECISION_FP64 || o->split_loop != 0;

@@ -10,6 +10,12 @@
 
 namespace deac {
 
+std::string &last_message()
+{
+    static thread_local std::string message;
+    return message;
+}
+
 // ---------------------------------------------------------------- RNG
 
 uint64_t Xoshiro::next()  // rng.hpp:25-39

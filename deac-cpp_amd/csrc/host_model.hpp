@@ -15,6 +15,10 @@
 
 namespace deac {
 
+// Text of the last error on this thread (validation messages, unsupported
+// combinations); deac_last_error_message returns it.
+std::string &last_message();
+
 // ---------------------------------------------------------------- RNG
 // xoshiro256+ (rng.hpp:21-44) and its seeding (rng.hpp:46-59).
 struct Xoshiro {

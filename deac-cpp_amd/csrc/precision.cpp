@@ -21,7 +21,6 @@
 
 namespace {
 
-thread_local std::string g_precision_message;
 
 // Device buffers of one call, freed on every exit path.
 struct Buffers {
@@ -79,18 +78,18 @@ extern "C" deac_status deac_model_isf(const deac_problem *p, const double *genom
     if (message) *message = nullptr;
     if (!p || !genomes || count == 0) return DEAC_ERR_INVALID_ARGUMENT;
     if (precision != DEAC_PRECISION_FP64 && precision != DEAC_PRECISION_FP32 && precision != DEAC_PRECISION_BF16X3) {
-        g_precision_message = "precision must be DEAC_PRECISION_FP64, DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3";
-        if (message) *message = g_precision_message.c_str();
+        deac::last_message() = "precision must be DEAC_PRECISION_FP64, DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3";
+        if (message) *message = deac::last_message().c_str();
         return DEAC_ERR_INVALID_ARGUMENT;
     }
     if (count > (1u << 24)) {
-        g_precision_message = "at most 2^24 genomes per call";
-        if (message) *message = g_precision_message.c_str();
+        deac::last_message() = "at most 2^24 genomes per call";
+        if (message) *message = deac::last_message().c_str();
         return DEAC_ERR_UNSUPPORTED;
     }
-    const deac_status v = deac::validate(*p, g_precision_message);
+    const deac_status v = deac::validate(*p, deac::last_message());
     if (v != DEAC_OK) {
-        if (message) *message = g_precision_message.c_str();
+        if (message) *message = deac::last_message().c_str();
         return v;
     }
     int devices = 0, device = 0;

@@ -23,7 +23,7 @@ STATUS = {0: "success", 1: "invalid argument", 2: "no usable gfx950 device", 3: 
           4: "device error", 5: "unsupported"}
 
 EXPORTED = ["deac_create", "deac_run", "deac_destroy", "deac_output_length", "deac_seed_state",
-            "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_run_profile",
+            "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_error_message", "deac_last_run_profile",
             "deac_batch_create", "deac_batch_run", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile", "deac_genome_length", "deac_model_isf"]
 
 _dp = C.POINTER(C.c_double)
@@ -108,6 +108,8 @@ def lib():
         L.deac_frequency_grid.restype = C.c_int
         L.deac_status_string.argtypes = [C.c_int]
         L.deac_status_string.restype = C.c_char_p
+        L.deac_last_error_message.argtypes = []
+        L.deac_last_error_message.restype = C.c_char_p
         L.deac_last_run_profile.argtypes = [C.c_void_p, C.POINTER(RunProfile)]
         L.deac_last_run_profile.restype = C.c_int
         L.deac_batch_create.argtypes = [C.POINTER(Problem), C.c_size_t, C.POINTER(C.c_void_p), C.POINTER(C.c_char_p)]
@@ -130,6 +132,11 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def _last_message():
+    m = lib().deac_last_error_message()
+    return m.decode() if m else None
 
 
 # Engines and batches still open when the interpreter exits are closed by an
@@ -244,7 +251,7 @@ class Engine:
         o = RunOptions(N, stop_minimum_fitness, int(track_stats), PRECISIONS[precision], int(split_loop))
         st = lib().deac_run(self.handle, s, C.byref(o), C.byref(r))
         if st:
-            raise DeacError(st)
+            raise DeacError(st, _last_message())
         n = r.generation + 1
         for k in ("mean", "minimum", "squared_mean"):
             out[k] = out[k][:n]
@@ -299,20 +306,23 @@ class Batch:
     def __len__(self):
         return len(self.problems)
 
-    def run(self, seeds, number_of_generations, stop_minimum_fitness=1.0, track_stats=True):
-        """seeds: list of --seed ints or 4-word states; N, stop, track_stats: scalar or per instance."""
+    def run(self, seeds, number_of_generations, stop_minimum_fitness=1.0, track_stats=True, precision="fp64"):
+        """seeds: list of --seed ints or 4-word states; N, stop, track_stats, precision ("fp64" exact, "fp32"
+        or "bf16x3": mixed K.S in the delta form, FP64 everything else): scalar or per instance."""
         n = len(self.problems)
         if len(seeds) != n:
             raise ValueError(f"one seed or state per problem: {len(seeds)} given for {n} problems")
         Ns = _per_instance(number_of_generations, n)
         stops = _per_instance(stop_minimum_fitness, n)
         tracks = _per_instance(track_stats, n)
+        precs = _per_instance(precision, n)
         states = (C.c_uint64 * (4 * n))()
         for i, sd in enumerate(seeds):
             w = seed_state(sd) if isinstance(sd, (int, np.integer)) else sd
             for k in range(4):
                 states[4 * i + k] = int(w[k])
-        opts = (RunOptions * n)(*[RunOptions(Ns[i], stops[i], int(tracks[i])) for i in range(n)])
+        opts = (RunOptions * n)(*[RunOptions(Ns[i], stops[i], int(tracks[i]), PRECISIONS[precs[i]], 0)
+                                  for i in range(n)])
         outs, res = [], (Result * n)()
         for i, p in enumerate(self.problems):
             nout = lib().deac_output_length(C.byref(p))
@@ -323,7 +333,7 @@ class Batch:
             outs.append(o)
         st = lib().deac_batch_run(self.handle, states, opts, res)
         if st:
-            raise DeacError(st)
+            raise DeacError(st, _last_message())
         prof = RunProfile()
         lib().deac_batch_last_profile(self.handle, C.byref(prof))
         self.profile = {k: getattr(prof, k) for k, _ in RunProfile._fields_}

@@ -97,14 +97,20 @@ typedef struct deac_run_options {
     size_t number_of_generations;         /* N >= 2; at most N-1 evolution steps */
     double stop_minimum_fitness;          /* stop when the best fitness <= this */
     int track_stats;                      /* fill the three stats arrays */
-    /* deac_run only (deac_batch_run requires FP64 and no split loop):
-     * ks_precision, a deac_precision.  FP64 (0) runs the fused exact kernel.
-     * FP32 and BF16X3 run the split generation loop, one kernel per step,
-     * with the model ISF from the mixed-precision contraction; trial vectors,
-     * chi^2, selection, statistics and constraints stay FP64 in the
-     * reference's order, and the random draws are the reference's.
-     * split_loop nonzero: FP64 runs through the split loop as well, with the
-     * exact contraction (results identical to the fused kernel). */
+    /* ks_precision, a deac_precision.  FP64 (0) is exact everywhere.
+     * deac_run: FP32 and BF16X3 run the split generation loop, one kernel
+     * per step, with the model ISF from the mixed-precision contraction.
+     * deac_batch_run: FP32 and BF16X3 run the batch kernel in the delta form:
+     * each member's model ISF is kept in FP64 and a trial's is that plus
+     * K.(trial - parent) at the chosen precision; the population is
+     * re-evaluated exactly in FP64 before the run, every 1024 generations
+     * and at the end, so the reported fitness is the exact FP64 chi^2 of the
+     * reported genome.  Either way, trial vectors, chi^2, selection,
+     * statistics and constraints stay FP64 in the reference's order, and the
+     * random draws are the reference's.
+     * split_loop nonzero (deac_run only): FP64 runs through the split loop
+     * as well, with the exact contraction (results identical to the fused
+     * kernel). */
     int ks_precision;
     int split_loop;
 } deac_run_options;
@@ -161,6 +167,11 @@ deac_status deac_frequency_grid(size_t genome_size, double omega_max, double *fr
 
 const char *deac_status_string(deac_status status);
 
+/* Text of the last error reported on the calling thread by any entry point
+ * (the reference prints such messages before exit(), deac.cpp:2075-2160);
+ * empty if none.  Valid until the next call on this thread. */
+const char *deac_last_error_message(void);
+
 /* Device timing of the last deac_run on this handle, from HIP events on the
  * handle's stream (no counterpart in the reference, which has no timers). */
 typedef struct deac_run_profile {
@@ -177,12 +188,14 @@ deac_status deac_last_run_profile(const deac_handle *handle, deac_run_profile *p
  * Many independent reconstructions in one persistent launch: the reference's
  * multi-seed workflow (tools/generate_commands.sh:18-32, one deac.e process
  * per seed, averaged by tools/plot_simple_average_deac_results.py:8-28) as
- * one call.  Each instance i computes exactly what deac_run would compute for
- * problems[i] from rng_states[4i..4i+3] (plain FP64, the reference's order and
- * random stream), with its own generation count and stop test; an instance
- * that stops leaves the launch.  Identical problems share one device copy of
- * the kernel matrix.  Population size at most 32; moment constraints and
- * per-generation renormalization run only through deac_run
+ * one call.  At ks_precision FP64, each instance i computes exactly what
+ * deac_run would compute for problems[i] from rng_states[4i..4i+3] (the
+ * reference's order and random stream); at FP32 or BF16X3 it runs the same
+ * random stream with the delta-form mixed contraction (deac_run_options).
+ * Each instance has its own generation count, stop test and precision; an
+ * instance that stops leaves the launch.  Identical problems share one
+ * device copy of the kernel matrix.  Population size at most 32; moment
+ * constraints and per-generation renormalization run only through deac_run
  * (DEAC_ERR_UNSUPPORTED here). */
 typedef struct deac_batch deac_batch;
 
