@@ -468,6 +468,66 @@ __device__ __forceinline__ void mask_draws(MaskGen &mg, int q, int Lw)
     }
 }
 
+// Population staging by the mixed kernel's chain waves.  Chunk c's rows wave, wave + 4,
+// + 8, + 12 of the [Ppad][C] chunk buffer (two positions per lane; padded
+// members read the last member) and its crossover words (this generation's
+// and the previous one's, two candidates each: 256 words) are loaded at the
+// bottom of step c - 1, after chunk c - 1 has landed, and stored into LDS at
+// the bottom of step c.  The loads fly across a barrier and a whole step,
+// beside the chain waves' own kernel-tile stream, and the workers' steps do
+// not wait on HBM.
+template <int NM>
+struct Stager {
+    static constexpr int C = 128 / NM, Ppad = 16 * NM, kWpc = C / 32;
+    const char *popb, *maskb;
+    uint32_t poff[4], moff;
+    double2 pv[4];
+    uint32_t mw;
+    int nload;
+
+    __device__ __forceinline__ void init(const BatchDesc &d, int kind, long g, int nchunk, int wave, int lane)
+    {
+        const int nP = d.P, nLpad = d.Lpad, nLw = d.Lw;
+        popb = reinterpret_cast<const char *>(uniform_ptr(d.pop));
+        maskb = reinterpret_cast<const char *>(uniform_ptr(d.masks));
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int own = (wave + 4 * r) * 128 + 2 * lane;
+            poff[r] = static_cast<uint32_t>(min(own / C, nP - 1) * nLpad + own % C) * 8u;
+        }
+        const long pg = kind == kPassEvolve ? g - 1 : g;
+        const int gslot = static_cast<int>((g + 3) % 3), pslot = static_cast<int>((pg + 3) % 3);
+        const int e = wave * kWave + lane;
+        const int w = e % kWpc, cand = (e / kWpc) & 1, m = (e / (2 * kWpc)) % Ppad;
+        const int slot = e / (2 * kWpc * Ppad) == 0 ? gslot : pslot;
+        moff = static_cast<uint32_t>(((slot * Ppad + m) * 2 + cand) * nLw + w) * 4u;
+        nload = kind != kPassPre ? nchunk : 0;
+        mw = 0;
+    }
+    __device__ __forceinline__ void issue(int c)
+    {
+        const gchar *pb = (const gchar *)popb + static_cast<size_t>(c) * C * 8;
+#pragma unroll
+        for (int r = 0; r < 4; r++) pv[r] = load2_stream((const gcdouble *)(pb + poff[r]));
+        mw = *(const guint *)((const gchar *)maskb + static_cast<size_t>(c) * kWpc * 4 + moff);
+    }
+    __device__ __forceinline__ void start()
+    {
+        if (0 < nload) issue(0);
+    }
+    // at the bottom of step c, before its barrier
+    __device__ __forceinline__ void step(int c, BatchShared &sh, double *lds, int wave, int lane)
+    {
+        if (c < nload) {
+            double *buf = lds + kOffPop + (c % 4) * kCD + wave * 128 + 2 * lane;
+#pragma unroll
+            for (int r = 0; r < 4; r++) *reinterpret_cast<double2 *>(buf + r * 512) = pv[r];
+            sh.mring[c % 4][wave * kWave + lane] = mw;
+        }
+        if (c + 1 < nload) issue(c + 1);
+    }
+};
+
 // Chain waves of the mixed kernel: dE = K.dS for this wave's pairs (kernel
 // tile, member tile), NQ of them.  With two member tiles, waves 0 and 1 take
 // the even kernel tiles 0, 2, 4, .. and waves 2 and 3 the odd ones, each wave
@@ -523,45 +583,6 @@ __device__ __forceinline__ void mixed_chains(const BatchDesc &d, BatchShared &sh
     }
     const int bfoff = (mt * 16 + (lane & 15)) * 64 + ((((lane >> 4) + ((lane & 15) >> 2)) & 3) * 16);
     const int nwork = chains ? nchunk : 0;
-    // The chain waves also stage the population: chunk c's rows (wave, wave
-    // + 4, + 8, + 12 of the [Ppad][C] chunk buffer, two positions per lane,
-    // padded members reading the last member) and its crossover words are
-    // loaded at the top of step c and stored into LDS at its bottom, so the
-    // workers' steps do not wait on HBM.
-    constexpr int kWpc = C / 32;
-    const int nP = d.P, nLpad = d.Lpad, nLw = d.Lw;
-    const char *const popb = reinterpret_cast<const char *>(uniform_ptr(d.pop));
-    const char *const maskb = reinterpret_cast<const char *>(uniform_ptr(d.masks));
-    uint32_t poff[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int own = (wave + 4 * r) * 128 + 2 * lane;
-        poff[r] = static_cast<uint32_t>(min(own / C, nP - 1) * nLpad + own % C) * 8u;
-    }
-    uint32_t moff;
-    {
-        const long pg = kind == kPassEvolve ? g - 1 : g;
-        const int gslot = static_cast<int>((g + 3) % 3), pslot = static_cast<int>((pg + 3) % 3);
-        const int e = wave * kWave + lane;
-        const int w = e % kWpc, cand = (e / kWpc) & 1, m = (e / (2 * kWpc)) % Ppad;
-        const int slot = e / (2 * kWpc * Ppad) == 0 ? gslot : pslot;
-        moff = static_cast<uint32_t>(((slot * Ppad + m) * 2 + cand) * nLw + w) * 4u;
-    }
-    const int nload = kind != kPassPre ? nchunk : 0;
-    double2 pv[4];
-    uint32_t mw = 0;
-    auto issue_loads = [&](int c) {
-        const gchar *pb = (const gchar *)popb + static_cast<size_t>(c) * C * 8;
-#pragma unroll
-        for (int r = 0; r < 4; r++) pv[r] = load2_stream((const gcdouble *)(pb + poff[r]));
-        mw = *(const guint *)((const gchar *)maskb + static_cast<size_t>(c) * kWpc * 4 + moff);
-    };
-    auto land_loads = [&](int c) {
-        double *buf = lds + kOffPop + (c % 4) * kCD + wave * 128 + 2 * lane;
-#pragma unroll
-        for (int r = 0; r < 4; r++) *reinterpret_cast<double2 *>(buf + r * 512) = pv[r];
-        sh.mring[c % 4][wave * kWave + lane] = mw;
-    };
     u4 ka[KD][NQ], kl[KD][NQ];
 #pragma unroll
     for (int p = 0; p < KD; p++)
@@ -571,14 +592,15 @@ __device__ __forceinline__ void mixed_chains(const BatchDesc &d, BatchShared &sh
             ka[p][q] = buffer_load4(kr[q], lo_, so);
             if (BF) kl[p][q] = buffer_load4(kr[q], lo_ + 16u, so);
         }
+    Stager<NM> st;
+    st.init(d, kind, g, nchunk, wave, lane);
+    st.start();
     for (int c = 0; c < min(4, nsteps); c++) {
-        if (c < nload) issue_loads(c);
-        if (c < nload) land_loads(c);
+        st.step(c, sh, lds, wave, lane);
         step_barrier();
     }
     for (int cm = 0; cm < nwork; cm++) {
         const int c = cm + 4;
-        if (c < nload) issue_loads(c);
         const char *tb = reinterpret_cast<const char *>(lds + kOffTrial + (cm & 1) * kCD);
 #pragma unroll
         for (int jj = 0; jj < UPC; jj++) {
@@ -618,12 +640,11 @@ __device__ __forceinline__ void mixed_chains(const BatchDesc &d, BatchShared &sh
             }
             __builtin_amdgcn_sched_barrier(0);  // issue the refill here, a ring ahead of its use
         }
-        if (c < nload) land_loads(c);
+        st.step(c, sh, lds, wave, lane);
         step_barrier();
     }
     for (int c = min(4, nsteps) + nwork; c < nsteps; c++) {
-        if (c < nload) issue_loads(c);
-        if (c < nload) land_loads(c);
+        st.step(c, sh, lds, wave, lane);
         step_barrier();
     }
     __syncthreads();
@@ -983,7 +1004,8 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
 #pragma unroll
         for (int u = 0; u < kT; u++) bfo[u] = bf_trial_offset<NM>(mrow(u), p);
         for (int c = 0; c < nsteps; c++) {
-            // (the mixed kernel's chain waves stage the chunks instead)
+            // (the mixed kernel's chain waves stage the chunks instead: Stager;
+            // in the FP64 kernel their registers hold the kernel-tile ring)
             const bool ld = PREC == kPrecFp64 && kind != kPassPre && c < nchunk;
             if (ld) {
                 const gchar *pb = (const gchar *)popp + static_cast<size_t>(c) * C * 8;
