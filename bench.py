@@ -5,42 +5,42 @@ Headline workload (one GPU): the seed batch -- 256 independent
 reconstructions (seeds 1..256) of the synthetic three-peak ISF with 65
 timeslices (relative error 1e-4), genome 4096 per sign, population 32,
 omega_max 512, standard model, bfull, --normalize, per-instance stop at
-chi^2 <= 1.0, all in one persistent launch (deac_batch_run).  The batch
-computes in plain FP64 in the reference's order: every instance is
-bit-identical to the reference CPU build for its seed.
+chi^2 <= 1.0, all in one persistent launch (deac_batch_run), with the
+model ISF's K.S on the FP32 matrix cores in the delta form (each member's
+model ISF kept in FP64, a trial's = that + K.(trial - parent) in FP32;
+chi^2, selection and everything else FP64; an exact FP64 re-evaluation of
+the population every 1024 generations and at the end, so every reported
+chi^2 is exact for its genome).
 
 A step is one generation of every instance (256 x 32 genome evaluations: trial
 vector, model ISF over 65 timeslices and 8192 frequencies, chi^2, selection).
-`value` is genome-evals/s over the timed generations, from HIP events on the
-stream the batch kernel runs on, with the problems and initial populations
-already resident in HBM.  The plain single-instance FP64 path (G=4096, P=8,
-T=65, seed 1) is timed beside it, and its ratio reported.
+The warm-up runs W generations; the K timed steps continue those same runs
+(deac_batch_continue), so they are generations W+1..W+K of each run, the
+segment's closing FP64 re-evaluation included.  `value` is genome-evals/s
+over the timed generations, from HIP events on the stream the batch kernel
+runs on, with everything resident in HBM.  Beside it: the same batch at
+BF16x3 and at exact FP64 (bit-identical to the reference CPU build per seed),
+a 400-generation steady-state figure for each, the plain single-instance
+FP64 path (P=8, and P=32 run seed by seed) and the ratios that split the
+speed-up into batching and precision.
 
-`roofline` prices the batch kernel against MI355X FP64 matrix peak
-(4*G*T flops per genome evaluation), with the HBM traffic and the matrix
-pipe's busy fraction from the committed rocprofv3 summaries under profiles/;
-`cpu_baseline` times the reference CPU
-executable built from source (oracle/_ref) on a bounded sample of the seed
-batch, one process per seed on the host cores at once, as the reference is
-run for many seeds.
+`roofline` prices the FP32 batch kernel against HBM (its 16.25 flop/B lies
+below the FP32-MFMA ridge): algorithmic bytes per launch (65536 per genome
+evaluation) over the kernel's average launch duration from HIP events, with
+the measured HBM traffic from the committed rocprofv3 summary;
+`roofline_fp64_batch` prices the exact FP64 kernel against FP64 MFMA.
+`cpu_baseline` times the reference CPU executable built from source
+(oracle/_ref) on a bounded sample of the seed batch, one process per seed
+on the host cores at once, as the reference is run for many seeds.
 
-`config.uneven_batch` times the uneven batch (tests/golden/make_uneven.py:
-64 wavevectors x 16 seeds with mixed T, G, omega_max, model and
-zero-temperature kernels, population 16) for a bounded number of generations
-in one deac_batch_run, as instance-generations/s and sustained FP64 TFLOP/s.
-
-`config.precision_trajectories` runs the T=257, G=16384, P=64 hyperbolic
-workload for a bounded number of generations (--trajectory-gens) with K.S in exact
-FP64 (the fused kernel) and in FP32 and split-BF16x3 (the split loop, chi^2
-and selection in FP64), and reports each mixed run's final |dchi^2|/chi^2 and
-DSF L2 error against the FP64 run.
-
-`config.precision_sweep` evaluates K.S with deac_model_isf at FP64, FP32 and
-split-BF16x3 (chi^2 in FP64): |dchi^2|/chi^2 against FP64 for 64 genomes
-near the solution of the T=257, G=16384 hyperbolic workload (the true
-spectrum on the genome grid, 0.1 % multiplicative noise), and the
-contraction's throughput for a stack of 8192 genomes (256 seeds x 32
-members) at G=4096, T=65.
+`config.batch_precision_accuracy` runs the seed batch at FP64, FP32 and
+BF16x3 with statistics and reports each mixed precision's first divergent
+generation against FP64, its final chi^2 and the exactness of the reported
+chi^2.  `config.uneven_batch` times the uneven batch (tests/golden/
+make_uneven.py: 64 wavevectors x 16 seeds with mixed T, G, omega_max, model
+and zero-temperature kernels, population 16).  `config.precision_trajectories`
+and `config.precision_sweep` cover the single-instance T=257, G=16384
+hyperbolic workload (split loop, deac_model_isf).
 """
 import argparse
 import json
@@ -57,7 +57,11 @@ sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
 
 METRIC = "genome-evals/sec at G=4096,T=65: batched mixed-precision vs plain FP64"
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 matrix (and vector), dense
+FP32_PEAK_TFLOPS = 157.3         # MI355X FP32 matrix, dense
 HBM_PEAK_GBS = 8000.0
+HEADLINE = "fp32"                # the seed batch's K.S precision (FP32-MFMA)
+HEADLINE_KERNEL = "batch_kernel<1,3,2,0> (FP32 K.S, 3 tile pairs per chain wave, 2 member tiles)"
+HEADLINE_PROFILE = "profiles/batch_kernel_fp32_seed256_hbm_counters.md"
 
 ISF = "isf_c2_T65.bin"
 TEMPERATURE, G, OMEGA_MAX, T = 1.35, 4096, 512.0, 65
@@ -93,6 +97,7 @@ def cpu_baseline(gens):
         if any(codes):
             raise RuntimeError(f"reference CPU build failed: exit codes {codes}")
         return {"value": procs * P * gens / dt, "unit": "genome-evals/s", "cores": procs, "kind": "reference",
+                "host": host_cpu(),
                 "sample": f"seeds 1..{procs} of the seed-batch workload (G=4096, P=32, T=65), one reference "
                           f"process per seed on {procs} cores at once, {gens} generations each, wall time incl. "
                           f"setup {dt:.2f} s"}
@@ -108,28 +113,68 @@ def cpu_baseline(gens):
                       f"wall time incl. setup {dt:.2f} s"}
 
 
-def traffic_per_generation():
-    """HBM bytes per generation of the seed batch, from the committed rocprofv3
+def host_cpu():
+    """CPU model and core counts of the host (lscpu's model name, nproc)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count()}
+
+
+def traffic_per_generation(path):
+    """HBM bytes per generation of the seed batch, from a committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (calibrated against a plain copy)."""
     import re
-    path = os.path.join(ROOT, "profiles", "batch_kernel_seed256_hbm_counters.md")
     try:
-        m = re.search(r"traffic_bytes_per_generation:\s*([0-9.]+)", open(path).read())
+        m = re.search(r"traffic_bytes_per_generation:\s*([0-9.]+)", open(os.path.join(ROOT, path)).read())
     except OSError:
         return None
     return float(m.group(1)) if m else None
 
 
-def mfma_busy_profiled():
-    """Fraction of cycles the FP64 matrix pipe was busy in the seed batch's
-    kernel, from the committed rocprofv3 SQ-counter summary."""
-    import re
-    path = os.path.join(ROOT, "profiles", "batch_kernel_seed256_sq_counters.md")
-    try:
-        m = re.search(r"MFMA busy: .*= ([0-9.]+) %", open(path).read())
-    except OSError:
-        return None
-    return float(m.group(1)) / 100.0 if m else None
+def batch_precision_accuracy(da, prob, gens):
+    """The seed batch at FP64, FP32 and BF16x3 for `gens` generations with
+    statistics: per mixed precision, the first generation at which each
+    seed's best-fitness history departs from the FP64 batch's by more than
+    1e-4 relative (100x the contraction's error: a different selection, not
+    the evaluation), the final chi^2 against FP64's, and the reported chi^2
+    against an exact FP64 evaluation of the reported genome (deac_model_isf)."""
+    import numpy as np
+    seeds = list(range(1, BATCH["seeds"] + 1))
+    out = {"workload": f"the seed batch, {gens} generations, stop at chi^2 <= {BATCH['stop']}"}
+    with da.Batch([prob] * len(seeds)) as b:
+        runs = {prec: b.run(seeds, gens + 1, BATCH["stop"], True, precision=prec)
+                for prec in ("fp64", "fp32", "bf16x3")}
+    ref = runs["fp64"]
+    for prec in ("fp32", "bf16x3"):
+        o = runs[prec]
+        first = []
+        for a, r in zip(o, ref):
+            n = min(len(a["minimum"]), len(r["minimum"]))
+            far = np.nonzero(np.abs(a["minimum"][:n] - r["minimum"][:n]) / r["minimum"][:n] > 1e-4)[0]
+            first.append(int(far[0]) if len(far) else None)
+        div = [x for x in first if x is not None]
+        fin = np.array([a["minimum_fitness"] for a in o])
+        finr = np.array([r["minimum_fitness"] for r in ref])
+        exact = []
+        for k in (0, 63, 127, 255):
+            dsf = o[k]["dsf"]
+            g = np.concatenate([dsf[G - 1:], dsf[G - 1::-1]])
+            _, chi, _ = da.model_isf(prob, g, "fp64")
+            exact.append(abs(chi[0] - o[k]["minimum_fitness"]) / chi[0])
+        out[prec] = {"first_divergent_generation_median": float(np.median(div)) if div else None,
+                     "seeds_not_diverged": sum(1 for x in first if x is None),
+                     "final_chi2_median": float(np.median(fin)), "final_chi2_median_fp64": float(np.median(finr)),
+                     "final_chi2_rel_diff_median": float(np.median(np.abs(fin - finr) / finr)),
+                     "reported_vs_fp64_reevaluated_rel_max": float(max(exact)),
+                     "generation1_best_rel_diff_max": float(max(abs(a["minimum"][1] - r["minimum"][1]) /
+                                                                r["minimum"][1] for a, r in zip(o, ref)))}
+    return out
 
 
 def uneven_batch(da, gens):
@@ -248,11 +293,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000, help="timed generations of the seed batch")
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--plain-steps", type=int, default=5000, help="timed generations of the plain path")
+    ap.add_argument("--plain-steps", type=int, default=5000, help="timed generations of the plain path (0: skip)")
+    ap.add_argument("--batch-precisions", default="fp32,bf16x3,fp64",
+                    help="K.S precisions of the seed batch to time, the headline (fp32) first")
     ap.add_argument("--cpu-gens", type=int, default=150, help="generations per seed of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--uneven-gens", type=int, default=200, help="timed generations of the uneven batch (0: skip)")
     ap.add_argument("--no-precision-sweep", action="store_true")
+    ap.add_argument("--accuracy-gens", type=int, default=2000,
+                    help="generations of the seed batch's FP64/FP32/BF16x3 accuracy comparison (0: skip)")
     ap.add_argument("--trajectory-gens", type=int, default=2000,
                     help="generations of the T=257, G=16384 precision trajectories (0: skip)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -280,44 +329,65 @@ def main():
     t, f, s = load_isf()
     freq = da.frequency_grid(G, OMEGA_MAX)
     flop_per_eval = 4 * G * T
+    bytes_per_eval = 2 * G * 8
 
-    # ---- seed batch (headline)
+    # ---- seed batch (headline): W warm-up generations, then K timed
+    # generations of the same runs (deac_batch_continue)
     nb, Pb = BATCH["seeds"], BATCH["P"]
     prob = da.make_problem(t, f, s, freq, TEMPERATURE, population_size=Pb, normalize=True, spectra_type="bfull")
     seeds = list(range(1, nb + 1))
+    batch = {}
     with da.Batch([prob] * nb) as b:
-        if args.warmup > 0:
-            b.run(seeds, args.warmup + 1, BATCH["stop"], track_stats=False)
-        t0 = time.perf_counter()
-        outs = b.run(seeds, args.steps + 1, BATCH["stop"], track_stats=False)
-        wall = time.perf_counter() - t0
-        prof = b.profile
-    loop_s = prof["loop_ms"] / 1e3
-    evals = Pb * prof["generations"]            # genome evaluations actually performed
-    value = evals / loop_s
-    kern_s = prof["evolve_ms"] / 1e3
-    achieved = evals * flop_per_eval / kern_s / 1e12
-    stopped = sum(1 for o in outs if o["generation"] < args.steps)
-    per_gen = traffic_per_generation()
-    gens_per_launch = min(args.steps, 4096)
+        for prec in args.batch_precisions.split(","):
+            b.run(seeds, max(args.warmup, 1) + 1, BATCH["stop"], track_stats=False, precision=prec)
+            t0 = time.perf_counter()
+            outs = b.continue_run(args.steps + 1, BATCH["stop"], track_stats=False, precision=prec)
+            wall = time.perf_counter() - t0
+            prof = dict(b.profile)
+            # steady state: 400 more generations of the same runs
+            b.continue_run(401, BATCH["stop"], track_stats=False, precision=prec)
+            steady = dict(b.profile)
+            evals = Pb * prof["generations"]
+            batch[prec] = {
+                "value": evals / (prof["loop_ms"] / 1e3), "unit": "genome-evals/s",
+                "ms_per_step": prof["loop_ms"] / args.steps,
+                "kernel_ms_per_launch": prof["evolve_ms"] / max(prof["evolve_launches"], 1),
+                "evolve_launches": prof["evolve_launches"], "eval_ms": prof["eval_ms"],
+                "eval_launches": prof["eval_launches"], "evals": evals,
+                "steady_ms_per_generation": steady["loop_ms"] / (steady["generations"] / nb),
+                "instances_stopped": sum(1 for o in outs if o["generation"] < args.steps),
+                "wall_s_including_host": wall}
+    head = batch[HEADLINE]
+    value = head["value"]
+    evals_per_launch = head["evals"] / max(head["evolve_launches"], 1)
+    kern_s = head["kernel_ms_per_launch"] / 1e3
+    per_gen = traffic_per_generation(HEADLINE_PROFILE)
+    gens_per_launch = args.steps
     traffic = per_gen * gens_per_launch if per_gen else None
-    alg_launch = nb * Pb * 2 * G * 8 * gens_per_launch
+    fp64b = batch.get("fp64", head)
+    fp64_kern_s = fp64b["kernel_ms_per_launch"] / 1e3
+    fp64_evals = fp64b["evals"] / max(fp64b["evolve_launches"], 1)
 
     # ---- plain FP64 single instance
-    pp = da.make_problem(t, f, s, freq, TEMPERATURE, population_size=PLAIN["P"], normalize=True, spectra_type="bfull")
-    with da.Engine(pp) as e:
-        e.run(PLAIN["seed"], 201, 0.0, track_stats=False)
-        pout = e.run(PLAIN["seed"], args.plain_steps + 1, 0.0, track_stats=False)
-    pprof = pout["profile"]
-    plain_value = PLAIN["P"] * pprof["generations"] / (pprof["loop_ms"] / 1e3)
-    # the plain path on the batch's own shape: seeds as sequential
-    # single-instance runs (P=32), timed on seed 1
-    pq = da.make_problem(t, f, s, freq, TEMPERATURE, population_size=Pb, normalize=True, spectra_type="bfull")
-    with da.Engine(pq) as e:
-        e.run(1, 101, 0.0, track_stats=False)
-        qout = e.run(1, args.plain_steps // 4 + 1, 0.0, track_stats=False)
-    qprof = qout["profile"]
-    plain32_value = Pb * qprof["generations"] / (qprof["loop_ms"] / 1e3)
+    plain_value = plain32_value = float("nan")
+    pprof = qprof = {"loop_ms": float("nan"), "generations": 1}
+    if args.plain_steps > 0:
+        pp = da.make_problem(t, f, s, freq, TEMPERATURE, population_size=PLAIN["P"], normalize=True,
+                             spectra_type="bfull")
+        with da.Engine(pp) as e:
+            e.run(PLAIN["seed"], 201, 0.0, track_stats=False)
+            pout = e.run(PLAIN["seed"], args.plain_steps + 1, 0.0, track_stats=False)
+        pprof = pout["profile"]
+        plain_value = PLAIN["P"] * pprof["generations"] / (pprof["loop_ms"] / 1e3)
+        # the plain path on the batch's own shape: seeds as sequential
+        # single-instance runs (P=32), timed on seed 1
+        pq = da.make_problem(t, f, s, freq, TEMPERATURE, population_size=Pb, normalize=True, spectra_type="bfull")
+        with da.Engine(pq) as e:
+            e.run(1, 101, 0.0, track_stats=False)
+            qout = e.run(1, args.plain_steps // 4 + 1, 0.0, track_stats=False)
+        qprof = qout["profile"]
+        plain32_value = Pb * qprof["generations"] / (qprof["loop_ms"] / 1e3)
+    accuracy = batch_precision_accuracy(da, prob, args.accuracy_gens) if args.accuracy_gens > 0 else None
     uneven = uneven_batch(da, args.uneven_gens) if args.uneven_gens > 0 else None
     sweep = None if args.no_precision_sweep else precision_sweep(da)
     trajectories = precision_trajectories(da, args.trajectory_gens) if args.trajectory_gens > 0 else None
@@ -329,20 +399,24 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": prof["loop_ms"] / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 K.S (delta form), f64 chi2/selection",
         "data": "synthetic",
         "config": {"workload": "seed batch: 256 seeds x (G=4096 per sign, T=65, P=32, standard model, bfull, "
                                "--normalize, stop at chi^2 <= 1), one persistent launch, synthetic three-peak "
-                               "ISF with rel. error 1e-4",
-                   "precision": "fp64 in the reference's order on FP64 MFMA: bit-identical per seed to the "
-                                "reference CPU build",
-                   "generations_per_s": prof["generations"] / nb / loop_s,
-                   "instances_stopped": stopped,
-                   "wall_s_including_host_setup": wall,
+                               "ISF with rel. error 1e-4; FP32-MFMA K.S in the delta form (E_parent FP64 + "
+                               "K.(trial - parent)), chi^2 and selection FP64, exact FP64 re-evaluation every "
+                               "1024 generations and at the end of a run",
+                   "timed_region": f"generations {max(args.warmup, 1) + 1}..{max(args.warmup, 1) + args.steps} of "
+                                   "the runs the warm-up started (deac_batch_continue), including the exact FP64 "
+                                   "re-evaluation that ends the segment",
+                   "generations_per_s": 1e3 / head["ms_per_step"],
+                   "steady_state_ms_per_generation": head["steady_ms_per_generation"],
+                   "steady_state_genome_evals_per_s": nb * Pb / (head["steady_ms_per_generation"] / 1e3),
+                   "seed_batch": batch,
                    "plain_fp64": {"value": plain_value, "unit": "genome-evals/s",
                                   "workload": "single instance G=4096, T=65, P=8, seed 1",
                                   "ms_per_generation": pprof["loop_ms"] / pprof["generations"]},
@@ -352,20 +426,31 @@ def main():
                                                           "run seed by seed through the plain path, seed 1",
                                               "ms_per_generation": qprof["loop_ms"] / qprof["generations"]},
                    "batched_over_plain_batch_shape": value / plain32_value,
+                   "batching_vs_precision": {"fp64_batch_over_plain_batch_shape": fp64b["value"] / plain32_value,
+                                             **{f"{p}_over_fp64_batch": batch[p]["value"] / fp64b["value"]
+                                                for p in batch if p != "fp64"}},
+                   "batch_precision_accuracy": accuracy,
                    "uneven_batch": uneven,
                    "precision_sweep": sweep,
                    "precision_trajectories": trajectories},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "traffic_note": "HBM bytes per launch (one launch = the timed generations, up to the "
-                                     "4096-generation slice) from profiles/batch_kernel_seed256_hbm_counters.md; "
-                                     "algorithmic bytes per launch: " + str(alg_launch),
-                     "kernel": "batch_kernel<1,2,1>", "flop_per_genome_eval": flop_per_eval,
-                     "kernel_ms_per_generation": prof["evolve_ms"] / args.steps,
-                     "hbm_algorithmic_GBps": evals * 2 * G * 8 / kern_s / 1e9,
-                     "mfma_busy": mfma_busy_profiled(),
-                     "mfma_busy_note": "SQ_VALU_MFMA_BUSY_CYCLES over the dispatch's cycles per SIMD, from "
-                                       "profiles/batch_kernel_seed256_sq_counters.md"},
+        "roofline": {"bound": "hbm", "achieved": evals_per_launch * bytes_per_eval / kern_s / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": evals_per_launch * bytes_per_eval / kern_s / 1e9 / HBM_PEAK_GBS,
+                     "traffic": traffic,
+                     "kernel": HEADLINE_KERNEL,
+                     "bound_note": "FP32 K.S: 16.25 flop/B is below the FP32-MFMA ridge (157.3 TF / 8 TB/s = "
+                                   "19.7 flop/B), so HBM bounds it; algorithmic bytes per genome evaluation "
+                                   "2 G 8 = 65536 (one FP64 read of the genome pair)",
+                     "algorithmic_bytes_per_launch": evals_per_launch * bytes_per_eval,
+                     "kernel_ms_per_launch": head["kernel_ms_per_launch"],
+                     "fp32_mfma_tflops": evals_per_launch * flop_per_eval / kern_s / 1e12,
+                     "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (calibrated "
+                                     f"against a plain copy) in {HEADLINE_PROFILE}, per generation x the launch's "
+                                     "generations"},
+        "roofline_fp64_batch": {"bound": "mfma", "kernel": "batch_kernel<0,1,2,1>",
+                                "achieved": fp64_evals * flop_per_eval / fp64_kern_s / 1e12,
+                                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac": fp64_evals * flop_per_eval / fp64_kern_s / 1e12 / FP64_PEAK_TFLOPS},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_gens)

@@ -77,7 +77,8 @@ std::string problem_key(const deac_problem &p)
 struct deac_batch {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr, kev2 = nullptr, kev3 = nullptr;
+    hipEvent_t fork = nullptr;
     // Instances of different kernel shapes run side by side: one stream per
     // shape group (default priority), joined to `stream` by events.
     static constexpr int kGroupStreams = 3;
@@ -94,6 +95,8 @@ struct deac_batch {
     uint64_t *rng = nullptr;
     long *status = nullptr;                    // per instance: stopped, generation reached
     deac::BatchDesc *descs = nullptr;          // device copy of the launch descriptors
+    std::vector<long> reached;                 // per instance: generation reached by the last run (absolute)
+    std::vector<char> epar_valid;              // per instance: epar holds the current population's model ISF
     deac_run_profile last{};
 
     ~deac_batch()
@@ -102,7 +105,7 @@ struct deac_batch {
         for (hipStream_t g : gstream)
             if (g) (void)hipStreamSynchronize(g);
         for (void *p : allocations) (void)hipFree(p);
-        for (hipEvent_t e : {ev0, ev1, kev0, kev1})
+        for (hipEvent_t e : {ev0, ev1, kev0, kev1, kev2, kev3, fork})
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : gdone)
             if (e) (void)hipEventDestroy(e);
@@ -383,7 +386,8 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
     BATCH_HIP(hipGetDeviceProperties(&prop, h->device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DEAC_ERR_NO_DEVICE;
     BATCH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    for (hipEvent_t *e : {&h->ev0, &h->ev1, &h->kev0, &h->kev1}) BATCH_HIP(hipEventCreate(e));
+    for (hipEvent_t *e : {&h->ev0, &h->ev1, &h->kev0, &h->kev1, &h->kev2, &h->kev3}) BATCH_HIP(hipEventCreate(e));
+    BATCH_HIP(hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
     for (int g = 0; g < deac_batch::kGroupStreams; g++) {
         BATCH_HIP(hipStreamCreateWithFlags(&h->gstream[g], hipStreamNonBlocking));
         BATCH_HIP(hipEventCreateWithFlags(&h->gdone[g], hipEventDisableTiming));
@@ -419,7 +423,7 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
     A(h->rng, count * 4);
     A(h->status, count * 2);
     A(h->epar, nepar);
-    A(h->descs, 2 * count);  // evolution launches, and the evaluation launches of mixed K.S
+    A(h->descs, 3 * count);  // evolution launches, and the evaluation launches of mixed K.S before and after
     if (st != DEAC_OK) return st;
     *out = h.release();
     return DEAC_OK;
@@ -436,17 +440,29 @@ deac_status deac_batch_last_profile(const deac_batch *h, deac_run_profile *profi
     return DEAC_OK;
 }
 
-deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_options *options, deac_result *results)
+}  // extern "C"
+
+namespace {
+
+// deac_batch_run (resume false) and deac_batch_continue (resume true).
+// stats_pool: device memory for the statistics histories (3 N doubles per
+// tracking instance, packed), or null to allocate them here.
+deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_options *options,
+                           deac_result *results, bool resume, double *stats_pool)
 {
     if (!h || !rng_states || !options || !results) return DEAC_ERR_INVALID_ARGUMENT;
     const size_t n = h->model_of.size();
+    if (resume && h->reached.size() != n) {
+        deac::last_message() = "no completed deac_batch_run to continue";
+        return DEAC_ERR_INVALID_ARGUMENT;
+    }
     for (size_t i = 0; i < n; i++) {
         if (options[i].number_of_generations < 2 || !results[i].dsf || !results[i].frequency) {
             deac::last_message() = "instance " + std::to_string(i) + ": number_of_generations must be at least 2 and the "
                               "dsf and frequency outputs are required";
             return DEAC_ERR_INVALID_ARGUMENT;
         }
-        if ((rng_states[4 * i] | rng_states[4 * i + 1] | rng_states[4 * i + 2] | rng_states[4 * i + 3]) == 0) {
+        if (!resume && (rng_states[4 * i] | rng_states[4 * i + 1] | rng_states[4 * i + 2] | rng_states[4 * i + 3]) == 0) {
             // a fixed point of xoshiro256+: the donor draws would never end
             deac::last_message() = "instance " + std::to_string(i) + ": the rng state must not be all zero";
             return DEAC_ERR_INVALID_ARGUMENT;
@@ -475,46 +491,69 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
             if (su != DEAC_OK) return su;
         }
 
-    // Initial populations on the host (deac.cpp:430-603), each from its own
-    // stream; the state after them is where the device walk starts.
-    std::vector<double> pop_host(h->pop_off.back() + h->models[h->model_of[n - 1]]->m.P *
-                                                         static_cast<size_t>(h->models[h->model_of[n - 1]]->Lpad),
-                                 0.0);
+    // Generation base of each instance: 0 for a new run; where the last run
+    // stopped for a continuation (its population, fitness, self-adapted CR
+    // and F and random state are still on the device).
+    std::vector<long> base(n, 0);
+    if (resume) base = h->reached;
+    std::vector<char> epar_ok = h->epar_valid;
+    epar_ok.resize(n, 0);
+    h->reached.clear();  // set again when this call completes
     std::vector<uint64_t> rng_host(4 * n);
-    std::vector<double> cr_host(n * deac::kBatchMaxP * 2, 0.0), f_host(n * deac::kBatchMaxP * 2, 0.0);
-    parallel_for(n, [&](size_t i) {
-        const BatchModel &b = *h->models[h->model_of[i]];
-        deac::Xoshiro rng{{rng_states[4 * i], rng_states[4 * i + 1], rng_states[4 * i + 2], rng_states[4 * i + 3]}};
-        std::vector<double> pop;
-        deac::initial_population(b.m, rng, pop);
-        for (size_t m = 0; m < b.m.P; m++)
-            std::memcpy(&pop_host[h->pop_off[i] + m * b.Lpad], &pop[m * b.m.Kc], b.m.Kc * sizeof(double));
-        std::memcpy(&rng_host[4 * i], rng.s, sizeof rng.s);
-        for (size_t m = 0; m < b.m.P; m++)
-            for (int sg = 0; sg < 2; sg++) {
-                cr_host[(i * deac::kBatchMaxP + m) * 2 + sg] = b.m.cr0;  // deac.cpp:990-1032
-                f_host[(i * deac::kBatchMaxP + m) * 2 + sg] = b.m.f0;
-            }
-    });
-    BATCH_HIP(hipMemcpyAsync(h->pop, pop_host.data(), pop_host.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    BATCH_HIP(hipMemcpyAsync(h->rng, rng_host.data(), rng_host.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-    BATCH_HIP(hipMemcpyAsync(h->cr_old, cr_host.data(), cr_host.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    BATCH_HIP(hipMemcpyAsync(h->f_old, f_host.data(), f_host.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    if (!resume) {
+        // Initial populations on the host (deac.cpp:430-603), each from its own
+        // stream; the state after them is where the device walk starts.
+        std::vector<double> pop_host(h->pop_off.back() + h->models[h->model_of[n - 1]]->m.P *
+                                                             static_cast<size_t>(h->models[h->model_of[n - 1]]->Lpad),
+                                     0.0);
+        std::vector<double> cr_host(n * deac::kBatchMaxP * 2, 0.0), f_host(n * deac::kBatchMaxP * 2, 0.0);
+        parallel_for(n, [&](size_t i) {
+            const BatchModel &b = *h->models[h->model_of[i]];
+            deac::Xoshiro rng{{rng_states[4 * i], rng_states[4 * i + 1], rng_states[4 * i + 2], rng_states[4 * i + 3]}};
+            std::vector<double> pop;
+            deac::initial_population(b.m, rng, pop);
+            for (size_t m = 0; m < b.m.P; m++)
+                std::memcpy(&pop_host[h->pop_off[i] + m * b.Lpad], &pop[m * b.m.Kc], b.m.Kc * sizeof(double));
+            std::memcpy(&rng_host[4 * i], rng.s, sizeof rng.s);
+            for (size_t m = 0; m < b.m.P; m++)
+                for (int sg = 0; sg < 2; sg++) {
+                    cr_host[(i * deac::kBatchMaxP + m) * 2 + sg] = b.m.cr0;  // deac.cpp:990-1032
+                    f_host[(i * deac::kBatchMaxP + m) * 2 + sg] = b.m.f0;
+                }
+        });
+        BATCH_HIP(hipMemcpyAsync(h->pop, pop_host.data(), pop_host.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        BATCH_HIP(hipMemcpyAsync(h->rng, rng_host.data(), rng_host.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+        BATCH_HIP(hipMemcpyAsync(h->cr_old, cr_host.data(), cr_host.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        BATCH_HIP(hipMemcpyAsync(h->f_old, f_host.data(), f_host.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        // the uploads read host memory that goes out of scope here
+        BATCH_HIP(hipStreamSynchronize(s));
+    }
 
     // Statistics buffers for the instances that track them.
     std::vector<double *> stats(n, nullptr);
     struct Freer {
         deac_batch *h;
         std::vector<double *> &v;
+        bool own;
         ~Freer()
         {
-            for (double *p : v)
-                if (p) h->release(p);
+            if (own)
+                for (double *p : v)
+                    if (p) h->release(p);
         }
-    } freer{h, stats};
-    for (size_t i = 0; i < n; i++)
-        if (options[i].track_stats && h->alloc(stats[i], 3 * options[i].number_of_generations) != DEAC_OK)
-            return DEAC_ERR_OUT_OF_MEMORY;
+    } freer{h, stats, stats_pool == nullptr};
+    if (stats_pool) {
+        size_t off = 0;
+        for (size_t i = 0; i < n; i++)
+            if (options[i].track_stats) {
+                stats[i] = stats_pool + off;
+                off += 3 * options[i].number_of_generations;
+            }
+    } else {
+        for (size_t i = 0; i < n; i++)
+            if (options[i].track_stats && h->alloc(stats[i], 3 * options[i].number_of_generations) != DEAC_OK)
+                return DEAC_ERR_OUT_OF_MEMORY;
+    }
 
     // Launch slices: every active instance evolves up to `slice` generations
     // per launch; stopped or finished instances leave the next launch.
@@ -525,9 +564,10 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
     long slice = 4096, refresh = 1024;
     if (const char *e = std::getenv("DEAC_BATCH_SLICE")) slice = std::max(1L, std::atol(e));
     if (const char *e = std::getenv("DEAC_BATCH_REFRESH")) refresh = std::max(1L, std::atol(e));
-    std::vector<long> next_gen(n, 0), stopped_at(n, -1);
+    std::vector<long> next_gen(base), stopped_at(n, -1), n_abs(n);
+    for (size_t i = 0; i < n; i++) n_abs[i] = base[i] + static_cast<long>(options[i].number_of_generations);
     std::vector<char> active(n, 1);
-    std::vector<deac::BatchDesc> descs(2 * n);
+    std::vector<deac::BatchDesc> descs(3 * n);
     deac_run_profile prof{};
     BATCH_HIP(hipEventRecord(h->ev0, s));
     bool first = true;
@@ -588,14 +628,15 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.stats = stats[i];
                 d.status = h->status + 2 * i;
                 d.stats_len = static_cast<long>(options[i].number_of_generations);
+                d.stats_base = base[i];
                 d.gen0 = next_gen[i];
                 const long len = prec == DEAC_PRECISION_FP64 ? slice : std::min(slice, refresh);
-                d.gen_end = std::min<long>(next_gen[i] + len, static_cast<long>(options[i].number_of_generations) - 1);
+                d.gen_end = std::min<long>(next_gen[i] + len, n_abs[i] - 1);
                 d.stop = options[i].stop_minimum_fitness;
                 d.tau_cr = m.tau_cr;
                 d.tau_f = m.tau_f;
                 d.mod_magic = ~UINT64_C(0) / m.P;
-                d.init = (first && prec == DEAC_PRECISION_FP64) ? 1 : 0;
+                d.init = (first && !resume && prec == DEAC_PRECISION_FP64) ? 1 : 0;
                 d.mode = 0;
                 d.T = static_cast<int>(m.T);
                 d.G = static_cast<int>(m.G);
@@ -619,35 +660,40 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
         }
         // Evaluation launches (FP64 kernel, mode 1) of the mixed instances,
         // grouped by the FP64 kernel shape: before the first slice (initial
-        // fitness and epar) and after every slice.
+        // fitness and epar; skipped when a continuation finds epar current)
+        // and after every slice.
         {
-            std::map<Key, std::vector<size_t>> eg;
-            for (size_t k : mixed_slots) eg[key_of(order[k], true)].push_back(k);
-            const size_t nevo = slot;
-            for (auto &[key, ks] : eg) {
-                const size_t begin = slot;
-                for (size_t k : ks) {
-                    deac::BatchDesc &e = descs[slot++];
-                    e = descs[k];
-                    e.mode = 1;
-                    e.init = 1;
-                }
-                evals_after.push_back({key, {begin, slot - begin}});
-                if (first) evals_before.push_back({key, {begin, slot - begin}});
+            std::map<Key, std::vector<size_t>> eb, ea;
+            for (size_t k : mixed_slots) {
+                const size_t i = order[k];
+                ea[key_of(i, true)].push_back(k);
+                if (first && !(resume && epar_ok[i])) eb[key_of(i, true)].push_back(k);
             }
-            (void)nevo;
+            for (auto *grp : {&eb, &ea})
+                for (auto &[key, ks] : *grp) {
+                    const size_t begin = slot;
+                    for (size_t k : ks) {
+                        deac::BatchDesc &e = descs[slot++];
+                        e = descs[k];
+                        e.mode = 1;
+                        e.init = 1;
+                    }
+                    (grp == &eb ? evals_before : evals_after).push_back({key, {begin, slot - begin}});
+                }
         }
         BATCH_HIP(hipMemcpyAsync(h->descs, descs.data(), slot * sizeof(deac::BatchDesc), hipMemcpyHostToDevice, s));
+        // events: kev0 | evaluations before | kev1 | evolution | kev2 | evaluations after | kev3
         BATCH_HIP(hipEventRecord(h->kev0, s));
         for (auto &[key, range] : evals_before)
             BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), s));
+        BATCH_HIP(hipEventRecord(h->kev1, s));
         if (launches.size() == 1) {
             auto &[key, range] = launches[0];
             BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), s));
         } else {
             // shapes side by side, on the group streams
-            BATCH_HIP(hipEventRecord(h->kev1, s));
-            for (int g = 0; g < deac_batch::kGroupStreams; g++) BATCH_HIP(hipStreamWaitEvent(h->gstream[g], h->kev1, 0));
+            BATCH_HIP(hipEventRecord(h->fork, s));
+            for (int g = 0; g < deac_batch::kGroupStreams; g++) BATCH_HIP(hipStreamWaitEvent(h->gstream[g], h->fork, 0));
             for (size_t k = 0; k < launches.size(); k++) {
                 auto &[key, range] = launches[k];
                 BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second),
@@ -658,16 +704,21 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 BATCH_HIP(hipStreamWaitEvent(s, h->gdone[g], 0));
             }
         }
+        BATCH_HIP(hipEventRecord(h->kev2, s));
         for (auto &[key, range] : evals_after)
             BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), s));
-        BATCH_HIP(hipEventRecord(h->kev1, s));
+        BATCH_HIP(hipEventRecord(h->kev3, s));
         std::vector<long> st(2 * n);
         BATCH_HIP(hipMemcpyAsync(st.data(), h->status, st.size() * sizeof(long), hipMemcpyDeviceToHost, s));
         BATCH_HIP(hipStreamSynchronize(s));
-        float ms = 0.0f;
-        BATCH_HIP(hipEventElapsedTime(&ms, h->kev0, h->kev1));
-        prof.evolve_ms += ms;
-        prof.evolve_launches += launches.size() + evals_before.size() + evals_after.size();
+        float ms_before = 0.0f, ms_evolve = 0.0f, ms_after = 0.0f;
+        BATCH_HIP(hipEventElapsedTime(&ms_before, h->kev0, h->kev1));
+        BATCH_HIP(hipEventElapsedTime(&ms_evolve, h->kev1, h->kev2));
+        BATCH_HIP(hipEventElapsedTime(&ms_after, h->kev2, h->kev3));
+        prof.evolve_ms += ms_evolve;
+        prof.evolve_launches += launches.size();
+        prof.eval_ms += ms_before + ms_after;
+        prof.eval_launches += evals_before.size() + evals_after.size();
         for (size_t k = 0; k < order.size(); k++) {
             const size_t i = order[k];
             const long reached = st[2 * i + 1];
@@ -680,7 +731,7 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
             if (st[2 * i]) {
                 stopped_at[i] = reached;
                 active[i] = 0;
-            } else if (reached >= static_cast<long>(options[i].number_of_generations) - 1) {
+            } else if (reached >= n_abs[i] - 1) {
                 active[i] = 0;
             }
         }
@@ -715,7 +766,7 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
         BATCH_HIP(hipMemcpy(row.data(), h->pop + h->pop_off[i] + best * b.Lpad, m.Kc * sizeof(double),
                             hipMemcpyDeviceToHost));
         deac::output_transform(m, row.data(), r->dsf, r->frequency);
-        size_t generation = stopped_at[i] >= 0 ? static_cast<size_t>(stopped_at[i]) : N - 2;
+        size_t generation = stopped_at[i] >= 0 ? static_cast<size_t>(stopped_at[i] - base[i]) : N - 2;
         if (stats[i]) {
             std::vector<double> hs(3 * N, 0.0);
             const size_t k = generation + 1;
@@ -738,7 +789,26 @@ deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_o
         if (r->final_fitness) std::memcpy(r->final_fitness, fit, P * sizeof(double));
         std::memcpy(&rng_states[4 * i], &rng_host[4 * i], 4 * sizeof(uint64_t));
     }
+    h->reached = next_gen;
+    h->epar_valid.assign(n, 0);
+    for (size_t i = 0; i < n; i++) h->epar_valid[i] = options[i].ks_precision != DEAC_PRECISION_FP64;
     return DEAC_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+deac_status deac_batch_run(deac_batch *h, uint64_t *rng_states, const deac_run_options *options, deac_result *results)
+{
+    return batch_run_impl(h, rng_states, options, results, false, nullptr);
+}
+
+deac_status deac_batch_continue(deac_batch *h, uint64_t *rng_states, const deac_run_options *options,
+                                deac_result *results)
+{
+    return batch_run_impl(h, rng_states, options, results, true, nullptr);
+}
+
 
 }  // extern "C"

@@ -1295,9 +1295,10 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
                             sum += v;
                             sq = i < p4 ? sq + v * v : fma(v, v, sq);
                         }
-                        st[g] = sum / static_cast<double>(P);
-                        st[d.stats_len + g] = mn;
-                        st[2 * d.stats_len + g] = sq / static_cast<double>(P);
+                        const long gi = g - d.stats_base;
+                        st[gi] = sum / static_cast<double>(P);
+                        st[d.stats_len + gi] = mn;
+                        st[2 * d.stats_len + gi] = sq / static_cast<double>(P);
                     }
                     if (mn <= d.stop) flag = 1;
                 }

@@ -43,7 +43,7 @@ struct BatchDesc {
                                // evaluation launches), or null
     double *stats;             // [3][stats_len] mean, minimum, squared mean per generation, or null
     long *status;              // [0] stopped, [1] generation reached
-    long stats_len;
+    long stats_len, stats_base;  // entry for generation g: stats[g - stats_base] (and + stats_len, + 2 stats_len)
     long gen0, gen_end;        // evolve generations gen0 .. gen_end-1 unless the stop test fires
     double stop, tau_cr, tau_f;
     uint64_t mod_magic;        // floor((2^64-1)/P)

@@ -24,7 +24,7 @@ STATUS = {0: "success", 1: "invalid argument", 2: "no usable gfx950 device", 3: 
 
 EXPORTED = ["deac_create", "deac_run", "deac_destroy", "deac_output_length", "deac_seed_state",
             "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_error_message", "deac_last_run_profile",
-            "deac_batch_create", "deac_batch_run", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile", "deac_genome_length", "deac_model_isf"]
+            "deac_batch_create", "deac_batch_run", "deac_batch_continue", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile", "deac_genome_length", "deac_model_isf"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -76,7 +76,8 @@ class Result(C.Structure):
 
 class RunProfile(C.Structure):
     _fields_ = [("loop_ms", C.c_double), ("walk_ms", C.c_double), ("mask_ms", C.c_double),
-                ("evolve_ms", C.c_double), ("generations", C.c_size_t), ("evolve_launches", C.c_size_t)]
+                ("evolve_ms", C.c_double), ("generations", C.c_size_t), ("evolve_launches", C.c_size_t),
+                ("eval_ms", C.c_double), ("eval_launches", C.c_size_t)]
 
 
 class DeacError(RuntimeError):
@@ -116,6 +117,8 @@ def lib():
         L.deac_batch_create.restype = C.c_int
         L.deac_batch_run.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOptions), C.POINTER(Result)]
         L.deac_batch_run.restype = C.c_int
+        L.deac_batch_continue.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOptions), C.POINTER(Result)]
+        L.deac_batch_continue.restype = C.c_int
         L.deac_batch_destroy.argtypes = [C.c_void_p]
         L.deac_batch_size.argtypes = [C.c_void_p]
         L.deac_batch_size.restype = C.c_size_t
@@ -312,12 +315,21 @@ class Batch:
         n = len(self.problems)
         if len(seeds) != n:
             raise ValueError(f"one seed or state per problem: {len(seeds)} given for {n} problems")
+        return self._run(seeds, number_of_generations, stop_minimum_fitness, track_stats, precision, False)
+
+    def continue_run(self, number_of_generations, stop_minimum_fitness=1.0, track_stats=True, precision="fp64"):
+        """deac_batch_continue: N more generations (N - 1 evolution steps) of every instance from where the
+        last run left it."""
+        return self._run(None, number_of_generations, stop_minimum_fitness, track_stats, precision, True)
+
+    def _run(self, seeds, number_of_generations, stop_minimum_fitness, track_stats, precision, resume):
+        n = len(self.problems)
         Ns = _per_instance(number_of_generations, n)
         stops = _per_instance(stop_minimum_fitness, n)
         tracks = _per_instance(track_stats, n)
         precs = _per_instance(precision, n)
         states = (C.c_uint64 * (4 * n))()
-        for i, sd in enumerate(seeds):
+        for i, sd in enumerate(seeds or []):
             w = seed_state(sd) if isinstance(sd, (int, np.integer)) else sd
             for k in range(4):
                 states[4 * i + k] = int(w[k])
@@ -331,7 +343,7 @@ class Batch:
             res[i] = Result(0, 0.0, 0, _ptr(o["dsf"]), _ptr(o["frequency"]), _ptr(o["mean"]), _ptr(o["minimum"]),
                             _ptr(o["squared_mean"]), _ptr(o["fitness"]))
             outs.append(o)
-        st = lib().deac_batch_run(self.handle, states, opts, res)
+        st = (lib().deac_batch_continue if resume else lib().deac_batch_run)(self.handle, states, opts, res)
         if st:
             raise DeacError(st, _last_message())
         prof = RunProfile()

@@ -181,6 +181,8 @@ typedef struct deac_run_profile {
     double evolve_ms;            /* fused generation kernels */
     size_t generations;          /* generations evolved */
     size_t evolve_launches;      /* launches of the fused generation kernel */
+    double eval_ms;              /* batch, mixed K.S: exact FP64 re-evaluation launches */
+    size_t eval_launches;
 } deac_run_profile;
 deac_status deac_last_run_profile(const deac_handle *handle, deac_run_profile *profile);
 
@@ -206,6 +208,19 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
  * deac_run's. */
 deac_status deac_batch_run(deac_batch *batch, uint64_t *rng_states, const deac_run_options *options,
                            deac_result *results);
+
+/* Continues every instance from where the last deac_batch_run or
+ * deac_batch_continue on this batch left it (population, fitness,
+ * self-adapted CR and F and random state stay on the device) for
+ * options[i].number_of_generations more generations, counted like a run's:
+ * N - 1 more evolution steps, stats histories of this segment (entry 0 is
+ * the last entry of the previous segment), generation relative to the
+ * segment.  At FP64 a run of N1 generations continued by N2 is bit-identical
+ * to one run of N1 + N2 - 1.  rng_states is output only.  No counterpart in
+ * the reference (whose runs cannot be resumed); used to time generations
+ * past a warm-up within one run. */
+deac_status deac_batch_continue(deac_batch *batch, uint64_t *rng_states, const deac_run_options *options,
+                                deac_result *results);
 
 void deac_batch_destroy(deac_batch *batch);
 

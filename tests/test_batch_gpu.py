@@ -292,3 +292,30 @@ def test_all_zero_rng_state_is_refused():
             b.run([4], 10, 0.0)                  # one seed per problem
         outs = b.run([4, 9], 10, 0.0)
     assert not _compare(outs[0], ok)
+
+
+def test_continue_equals_one_longer_run():
+    # deac_batch_continue: a run of N1 generations continued by N2 is bit for
+    # bit one run of N1 + N2 - 1 (population, fitness, CR/F and random state
+    # stay on the device); histories join at the shared entry
+    print("[gpu] batch continue = one longer run", flush=True)
+    p = _synthetic_problem(65, 300, 32)
+    q = _synthetic_problem(18, 200, 12)
+    seeds = [5, 6, 7]
+    with da.Batch([p, q, p]) as b:
+        whole = b.run(seeds, 60, 0.0, True)
+        first = b.run(seeds, 21, 0.0, True)
+        rest = b.continue_run(40, 0.0, True)
+        more = b.continue_run(3, 0.0, True)  # and again
+    with da.Batch([p, q, p]) as b:
+        whole2 = b.run(seeds, 62, 0.0, True)
+    for w, f, r in zip(whole, first, rest):
+        for k in ("minimum", "mean", "squared_mean"):
+            assert f[k][-1] == r[k][0]
+            joined = np.concatenate([f[k], r[k][1:]])
+            assert gc.bit_diff(joined, w[k]) is None, k
+        assert gc.bit_diff(r["dsf"], w["dsf"]) is None
+        assert r["rng"] == w["rng"] and r["generation"] == 39 and w["generation"] == 59
+    for w2, m in zip(whole2, more):
+        assert gc.bit_diff(m["dsf"], w2["dsf"]) is None and m["rng"] == w2["rng"]
+        assert m["minimum"][-1] == w2["minimum"][-1]
