@@ -124,8 +124,10 @@ def test_run_rejects_unknown_precision():
         assert e.run(1, 10, 0.0)["generation"] == 9  # the handle still works
 
 
-def test_batch_rejects_mixed_precision():
-    print("[gpu] deac_batch_run with ks_precision 1", flush=True)
+def test_batch_refuses_unknown_precision():
+    # the batch takes FP64, FP32 and BF16X3 (tests/test_batch_mixed_gpu.py);
+    # any other ks_precision is an invalid argument, refused before a launch
+    print("[gpu] deac_batch_run with ks_precision 7", flush=True)
     case = gc.load("smoke_default")
     a = case["meta"]["args"]
     freq = gc.frequency_of(case, da.frequency_grid)
@@ -134,9 +136,9 @@ def test_batch_rejects_mixed_precision():
         nout = da.lib().deac_output_length(C.byref(p))
         dsf, fr = np.zeros(nout), np.zeros(nout)
         r = (da.Result * 1)(da.Result(0, 0.0, 0, da._ptr(dsf), da._ptr(fr), None, None, None, None))
-        o = (da.RunOptions * 1)(da.RunOptions(10, 0.0, 0, 1, 0))
+        o = (da.RunOptions * 1)(da.RunOptions(10, 0.0, 0, 7, 0))
         st = da.lib().deac_batch_run(b.handle, da.seed_state(1), o, r)
-        assert st == 5  # UNSUPPORTED
+        assert st == 1 and "ks_precision" in da._last_message()
 
 
 @pytest.mark.parametrize("T,G,P", [(12, 48, 520), (4100, 24, 6)])
