@@ -95,6 +95,7 @@ struct deac_batch {
     uint64_t *rng = nullptr;
     long *status = nullptr;                    // per instance: stopped, generation reached
     deac::BatchDesc *descs = nullptr;          // device copy of the launch descriptors
+    double *scratch = nullptr;                 // this device's statistics scratch (deac_batch_run_noalloc)
     std::vector<long> reached;                 // per instance: generation reached by the last run (absolute)
     std::vector<char> epar_valid;              // per instance: epar holds the current population's model ISF
     deac_run_profile last{};
@@ -385,6 +386,7 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
     hipDeviceProp_t prop;
     BATCH_HIP(hipGetDeviceProperties(&prop, h->device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DEAC_ERR_NO_DEVICE;
+    BATCH_HIP(deac::batch_scratch(h->device, &h->scratch));  // once per device, this device current
     BATCH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     for (hipEvent_t *e : {&h->ev0, &h->ev1, &h->kev0, &h->kev1, &h->kev2, &h->kev3}) BATCH_HIP(hipEventCreate(e));
     BATCH_HIP(hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
@@ -808,6 +810,30 @@ deac_status deac_batch_continue(deac_batch *h, uint64_t *rng_states, const deac_
                                 deac_result *results)
 {
     return batch_run_impl(h, rng_states, options, results, true, nullptr);
+}
+
+deac_status deac_batch_run_noalloc(deac_batch *h, uint64_t *rng_states, const deac_run_options *options,
+                                   deac_result *results)
+{
+    if (!h || !rng_states || !options || !results) return DEAC_ERR_INVALID_ARGUMENT;
+    const size_t n = h->model_of.size();
+    size_t need = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (options[i].track_stats) need += 3 * options[i].number_of_generations;
+        const int prec = options[i].ks_precision;
+        if ((prec == DEAC_PRECISION_FP32 || prec == DEAC_PRECISION_BF16X3) && !h->models[h->model_of[i]]->kmix[prec]) {
+            deac::last_message() = "instance " + std::to_string(i) + ": the mixed K.S tiles of this precision are "
+                                   "built (allocated) by the first deac_batch_run at that precision";
+            return DEAC_ERR_UNSUPPORTED;
+        }
+    }
+    if (need > deac::kBatchScratchDoubles) {
+        deac::last_message() = "the statistics of this call need " + std::to_string(need) +
+                               " doubles; deac_batch_run_noalloc holds at most " +
+                               std::to_string(deac::kBatchScratchDoubles);
+        return DEAC_ERR_INVALID_ARGUMENT;
+    }
+    return batch_run_impl(h, rng_states, options, results, false, h->scratch);
 }
 
 

@@ -1,7 +1,40 @@
-// Exact FP64 instantiations of the batched DEAC kernel (batch_device.hpp).
+// Exact FP64 instantiations of the batched DEAC kernel (batch_device.hpp),
+// and the device scratch of the allocation-free batch entry point.
+#include <mutex>
+
 #include "batch_device.hpp"
 
 namespace deac {
+
+namespace {
+
+// Statistics staging of deac_batch_run_noalloc: one array per GPU, held
+// until the process ends.  In host code `scratch` names a host-side stand-in,
+// not device memory: only the address from hipGetSymbolAddress (kept per
+// device below) may reach a kernel or a copy.
+static __device__ __align__(256) double scratch[kBatchScratchDoubles];
+std::mutex g_scratch_lock;
+double *g_scratch_ptr[kBatchMaxDevices] = {};
+
+}  // namespace
+
+hipError_t batch_scratch(int device, double **ptr)
+{
+    if (device < 0 || device >= kBatchMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    if (!g_scratch_ptr[device]) {
+        int current = -1;
+        hipError_t e = hipGetDevice(&current);
+        if (e != hipSuccess) return e;
+        if (current != device) return hipErrorInvalidDevice;  // the caller makes the device current
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
+}
 
 namespace {
 

@@ -57,6 +57,13 @@ struct BatchDesc {
 // tiles (1 or 2); rem: 1 when the kernel has 1..4 rows past its last full tile.
 hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, int rem, hipStream_t s);
 
+// Device scratch of deac_batch_run_noalloc: a file-scope array of kBatchScratchDoubles
+// doubles per GPU (32 MiB), its address looked up once per device with that device
+// current (hipGetSymbolAddress).  Calls that overlap on one GPU share it.
+constexpr size_t kBatchScratchDoubles = size_t(1) << 22;
+constexpr int kBatchMaxDevices = 64;
+hipError_t batch_scratch(int device, double **ptr);
+
 // Mixed K.S kernel (precision DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3).  nq: (kernel
 // tile, member tile) pairs per chain wavefront, one of kBatchMixedNq; nm as above.
 constexpr int kBatchMixedNq[] = {1, 2, 3, 5, 9};

@@ -24,7 +24,7 @@ STATUS = {0: "success", 1: "invalid argument", 2: "no usable gfx950 device", 3: 
 
 EXPORTED = ["deac_create", "deac_run", "deac_destroy", "deac_output_length", "deac_seed_state",
             "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_error_message", "deac_last_run_profile",
-            "deac_batch_create", "deac_batch_run", "deac_batch_continue", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile", "deac_genome_length", "deac_model_isf"]
+            "deac_batch_create", "deac_batch_run", "deac_batch_continue", "deac_batch_run_noalloc", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile", "deac_genome_length", "deac_model_isf"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -119,6 +119,8 @@ def lib():
         L.deac_batch_run.restype = C.c_int
         L.deac_batch_continue.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOptions), C.POINTER(Result)]
         L.deac_batch_continue.restype = C.c_int
+        L.deac_batch_run_noalloc.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOptions), C.POINTER(Result)]
+        L.deac_batch_run_noalloc.restype = C.c_int
         L.deac_batch_destroy.argtypes = [C.c_void_p]
         L.deac_batch_size.argtypes = [C.c_void_p]
         L.deac_batch_size.restype = C.c_size_t

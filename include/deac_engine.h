@@ -222,6 +222,19 @@ deac_status deac_batch_run(deac_batch *batch, uint64_t *rng_states, const deac_r
 deac_status deac_batch_continue(deac_batch *batch, uint64_t *rng_states, const deac_run_options *options,
                                 deac_result *results);
 
+/* deac_batch_run without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Results
+ * are those of deac_batch_run.  The statistics histories are staged in a
+ * device array of the library, 4194304 doubles (32 MiB) per GPU, held until
+ * the process ends: calls whose tracked histories (3 N doubles per instance
+ * with track_stats) exceed it return DEAC_ERR_INVALID_ARGUMENT before any
+ * launch.  Mixed K.S precisions need their kernel tiles, which the first
+ * deac_batch_run at that precision builds (DEAC_ERR_UNSUPPORTED otherwise).
+ * Calls on one GPU share the array: calls that overlap on one GPU race. */
+deac_status deac_batch_run_noalloc(deac_batch *batch, uint64_t *rng_states, const deac_run_options *options,
+                                   deac_result *results);
+
 void deac_batch_destroy(deac_batch *batch);
 
 size_t deac_batch_size(const deac_batch *batch);
