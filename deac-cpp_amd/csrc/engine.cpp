@@ -41,6 +41,7 @@ struct deac_handle {
     // Two buffer sets, used by batches of even and odd index: grid barrier
     // counter ([0]) and status ([4..6]) per set, and the batch's draws.
     unsigned *sync = nullptr;           // [2][8]
+    long *iter = nullptr;               // [2][2]: split loop generation counters, per buffer set
     int *host_status = nullptr;         // [3][4], pinned: status of the sets, then of the initial fitness
     deac::MemberCtl *ctl[2] = {};
     uint64_t *raw[2] = {}, *gen_start[2] = {};
@@ -310,6 +311,7 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
     A(h->emod, static_cast<size_t>(P) * T);
     A(h->mom, static_cast<size_t>(P) * 4);
     A(h->sync, 16);
+    A(h->iter, 4);
     for (int k = 0; k < 2; k++) {
         A(h->ctl[k], static_cast<size_t>(h->batch) * P);
         A(h->raw[k], static_cast<size_t>(h->batch) * P * Kc);
@@ -578,6 +580,20 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
     // precision, moments and selection run on the evolution stream.  One
     // status word serves the whole run: once a stop sets it, every kernel
     // queued behind (the rest of the batch and the next one) returns at once.
+    // graphs of the split loop (per run: the kernel arguments hold this
+    // run's buffers and options); DEAC_SPLIT_GRAPH sets the generations per
+    // graph, 0 runs plain launches
+    int graph_len = 8;
+    if (const char *e = std::getenv("DEAC_SPLIT_GRAPH")) graph_len = std::max(0, std::atoi(e));
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    struct GraphFree {
+        hipGraphExec_t *g;
+        ~GraphFree()
+        {
+            for (int k = 0; k < 2; k++)
+                if (g[k]) (void)hipGraphExecDestroy(g[k]);
+        }
+    } graph_free{gexec};
     auto split_loop = [&]() -> deac_status {
         const deac_handle::Split &sp = h->split;
         const deac::KsLayout &y = sp.y;
@@ -587,32 +603,66 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         const int *halt = ea.status;
         // model ISF (and moment rows) of the contraction's genomes: the
         // current ones for the initial fitness, the trials otherwise
-        auto evaluate = [&](int init) -> deac_status {
+        auto evaluate = [&](const deac::EvoArgs &a, int init) -> deac_status {
             if (y.precision == DEAC_PRECISION_FP64) {
                 double *B = static_cast<double *>(sp.B);
-                DEAC_HIP(deac::launch_split_pack_fp64(ea, init, B, y.Nt, Kb, s));
-                DEAC_HIP(deac::launch_ks_fp64(static_cast<const double *>(sp.A), B, ea.emod, ea.T, ea.P, y.Mt, y.Nt,
+                DEAC_HIP(deac::launch_split_pack_fp64(a, init, B, y.Nt, Kb, s));
+                DEAC_HIP(deac::launch_ks_fp64(static_cast<const double *>(sp.A), B, a.emod, a.T, a.P, y.Mt, y.Nt,
                                               Kb, s, nullptr, nullptr, halt));
             } else {
                 if (y.precision == DEAC_PRECISION_FP32) {
                     float *B = static_cast<float *>(sp.B);
-                    DEAC_HIP(deac::launch_split_pack_fp32(ea, init, B, y.Nt, Kb, s));
+                    DEAC_HIP(deac::launch_split_pack_fp32(a, init, B, y.Nt, Kb, s));
                     DEAC_HIP(deac::launch_ks_fp32(static_cast<const float *>(sp.A), B, sp.part, y.Mt, y.Nt, Kb,
                                                   y.kb_per_slice, y.slices, s, nullptr, nullptr, halt));
                 } else {
-                    DEAC_HIP(deac::launch_split_pack_bf16x3(ea, init, sp.B, sp.Bl, y.Nt, Kb, s));
+                    DEAC_HIP(deac::launch_split_pack_bf16x3(a, init, sp.B, sp.Bl, y.Nt, Kb, s));
                     DEAC_HIP(deac::launch_ks_bf16x3(sp.A, sp.Al, sp.B, sp.Bl, sp.part, y.Mt, y.Nt, Kb, y.kb_per_slice,
                                                     y.slices, s, nullptr, nullptr, halt));
                 }
-                DEAC_HIP(deac::launch_split_model(ea, sp.part, y.slices, static_cast<int>(y.npad()),
+                DEAC_HIP(deac::launch_split_model(a, sp.part, y.slices, static_cast<int>(y.npad()),
                                                   y.mpad() * y.npad(), sp.ex, s));
             }
-            if (ea.use_first || ea.use_third) DEAC_HIP(deac::launch_split_moments(ea, init, s));
+            if (a.use_first || a.use_third) DEAC_HIP(deac::launch_split_moments(a, init, s));
             return DEAC_OK;
+        };
+        // One generation g of a batch: top and trials, renormalization,
+        // contraction, moments, selection; with a.iter, the generation comes
+        // from the device counter, which the last kernel advances.
+        auto one_generation = [&](const deac::EvoArgs &a, int g) -> deac_status {
+            DEAC_HIP(deac::launch_split_trial(a, g, s));  // with the top of generation g
+            if (a.renorm) DEAC_HIP(deac::launch_split_renorm(a, s));
+            DEAC_TRY(evaluate(a, 0));
+            DEAC_HIP(deac::launch_split_select(a, g, 0, s));
+            if (a.iter) DEAC_HIP(deac::launch_split_iter(a.iter, 0, 0, s));
+            return DEAC_OK;
+        };
+        // Graphs: the kernels of graph_len generations captured once per
+        // buffer set and run, replayed for every full group of a batch's
+        // generations (a launch per group instead of ~5 per generation);
+        // the remainder of a batch runs as plain launches.  A capture the
+        // runtime refuses leaves the run on plain launches.
+        auto capture = [&](int k, const deac::EvoArgs &a) -> bool {
+            hipGraph_t graph = nullptr;
+            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                (void)hipGetLastError();
+                return false;
+            }
+            deac_status st = DEAC_OK;
+            for (int r = 0; r < graph_len && st == DEAC_OK; r++) st = one_generation(a, 0);
+            const hipError_t e = hipStreamEndCapture(s, &graph);
+            bool ok = st == DEAC_OK && e == hipSuccess && graph;
+            if (ok) ok = hipGraphInstantiate(&gexec[k], graph, nullptr, nullptr, 0) == hipSuccess;
+            if (graph) (void)hipGraphDestroy(graph);
+            if (!ok) {
+                (void)hipGetLastError();
+                gexec[k] = nullptr;
+            }
+            return ok;
         };
         // Initial fitness (deac.cpp:857-976).
         DEAC_HIP(deac::launch_zero_u32(h->sync, 8, s));
-        DEAC_TRY(evaluate(1));
+        DEAC_TRY(evaluate(ea, 1));
         DEAC_HIP(deac::launch_split_select(ea, 0, 1, s));
         DEAC_HIP(hipEventRecord(h->ev0, s));
         DEAC_HIP(hipStreamWaitEvent(d, h->ev0, 0));  // RNG state uploaded
@@ -643,12 +693,21 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
             ea.raw = h->raw[k];
             ea.gen0 = static_cast<long>(plan[b].first);
             ea.ngen = nb;
-            for (int g = 0; g < nb; g++) {
-                DEAC_HIP(deac::launch_split_trial(ea, g, s));  // with the top of generation g
-                if (ea.renorm) DEAC_HIP(deac::launch_split_renorm(ea, s));
-                DEAC_TRY(evaluate(0));
-                DEAC_HIP(deac::launch_split_select(ea, g, 0, s));
+            int g = 0;
+            if (graph_len > 0 && nb >= graph_len) {
+                deac::EvoArgs eg = ea;
+                eg.iter = h->iter + 2 * k;
+                DEAC_HIP(deac::launch_split_iter(eg.iter, eg.gen0, 1, s));
+                if (!gexec[k] && !capture(k, eg)) graph_len = 0;
+                if (gexec[k]) {
+                    for (; g + graph_len <= nb; g += graph_len) {
+                        DEAC_HIP(hipGraphLaunch(gexec[k], s));
+                        prof.graph_launches += 1;
+                    }
+                    for (; g < nb; g++) DEAC_TRY(one_generation(eg, 0));
+                }
             }
+            for (; g < nb; g++) DEAC_TRY(one_generation(ea, g));
             DEAC_HIP(hipEventRecord(e[5], s));
             DEAC_HIP(hipMemcpyAsync(h->host_status + 4 * k, h->sync + 4, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
             DEAC_HIP(hipEventRecord(e[6], s));

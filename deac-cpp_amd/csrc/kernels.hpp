@@ -68,6 +68,8 @@ struct EvoArgs {
     int *status;                 // [0] stopped, [1] generation index within the batch, [2] barrier timeout
     long gen0;                   // generation index of the batch's first generation
     int ngen;
+    long *iter;                  // split loop under graph capture: [0] gen0, [1] generation within the batch
+                                 // (read in place of gen0 and the kernels' g), or null
 };
 
 hipError_t launch_walk(const WalkArgs &a, hipStream_t s);

@@ -49,7 +49,7 @@ __device__ void top_scan(const EvoArgs &a, int g, Fit fit)
     double mn = fit(0);
     for (int i = 0; i < a.P; i++)
         if (fit(i) < mn) mn = fit(i);
-    const long ii = a.gen0 + g;
+    const long ii = (a.iter ? a.iter[0] : a.gen0) + g;
     if (a.track_stats) {
         double s = 0.0;
         for (int i = 0; i < a.P; i++) s += fit(i);
@@ -96,6 +96,7 @@ __device__ void top_of_generation(const EvoArgs &a, int g)
 __global__ __launch_bounds__(kThreads) void split_trial_kernel(EvoArgs a, int g)
 {
     if (stopped(a)) return;
+    if (a.iter) g = static_cast<int>(a.iter[1]);
     if (blockIdx.x == 0) top_of_generation(a, g);
     const int half = a.stride / 2;
     const MemberCtl *ctlg = a.ctl + static_cast<size_t>(g) * a.P;
@@ -243,6 +244,7 @@ __global__ __launch_bounds__(kWave) void split_moment_kernel(EvoArgs a, int init
 __global__ __launch_bounds__(kWave) void split_select_kernel(EvoArgs a, int g, int init)
 {
     if (stopped(a)) return;
+    if (a.iter && !init) g = static_cast<int>(a.iter[1]);
     extern __shared__ double dsh[];  // T residuals (T <= kSelectLds), else one window of kWave
     const int i = blockIdx.x, lane = threadIdx.x;
     double chi = 0.0;
@@ -287,6 +289,18 @@ __global__ __launch_bounds__(kWave) void split_select_kernel(EvoArgs a, int g, i
             a.cr_old[i * a.nsign + sg] = cr;
             a.f_old[i * a.nsign + sg] = f;
         }
+    }
+}
+
+// The generation counter of a captured batch: set = 1 starts a batch at
+// gen0 (generation 0 of the batch), set = 0 moves to the next generation.
+__global__ void split_iter_kernel(long *iter, long gen0, int set)
+{
+    if (set) {
+        iter[0] = gen0;
+        iter[1] = 0;
+    } else {
+        iter[1] += 1;
     }
 }
 
@@ -357,6 +371,12 @@ hipError_t launch_split_select(const EvoArgs &a, int g, int init, hipStream_t s)
 {
     const size_t lds = static_cast<size_t>(a.T <= kSelectLds ? a.T : kWave) * sizeof(double);
     hipLaunchKernelGGL(split_select_kernel, dim3(a.P), dim3(kWave), lds, s, a, g, init);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_iter(long *iter, long gen0, int set, hipStream_t s)
+{
+    hipLaunchKernelGGL(split_iter_kernel, dim3(1), dim3(1), 0, s, iter, gen0, set);
     return hipGetLastError();
 }
 

@@ -31,5 +31,8 @@ hipError_t launch_split_moments(const EvoArgs &a, int init, hipStream_t s);
 // Fitness from a.emod and a.mom; greedy selection (init = 0) or the initial
 // fitness (init = 1).
 hipError_t launch_split_select(const EvoArgs &a, int g, int init, hipStream_t s);
+// Generation counter of a batch run through captured graphs (EvoArgs::iter):
+// set = 1: iter = {gen0, 0}; set = 0: the next generation of the batch.
+hipError_t launch_split_iter(long *iter, long gen0, int set, hipStream_t s);
 
 }  // namespace deac

@@ -77,7 +77,7 @@ class Result(C.Structure):
 class RunProfile(C.Structure):
     _fields_ = [("loop_ms", C.c_double), ("walk_ms", C.c_double), ("mask_ms", C.c_double),
                 ("evolve_ms", C.c_double), ("generations", C.c_size_t), ("evolve_launches", C.c_size_t),
-                ("eval_ms", C.c_double), ("eval_launches", C.c_size_t)]
+                ("eval_ms", C.c_double), ("eval_launches", C.c_size_t), ("graph_launches", C.c_size_t)]
 
 
 class DeacError(RuntimeError):

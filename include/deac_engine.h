@@ -183,6 +183,7 @@ typedef struct deac_run_profile {
     size_t evolve_launches;      /* launches of the fused generation kernel */
     double eval_ms;              /* batch, mixed K.S: exact FP64 re-evaluation launches */
     size_t eval_launches;
+    size_t graph_launches;       /* deac_run's split loop: launches of captured HIP graphs */
 } deac_run_profile;
 deac_status deac_last_run_profile(const deac_handle *handle, deac_run_profile *profile);
 

@@ -157,3 +157,30 @@ def test_split_loop_fallback_shapes_match_fused(T, G, P):
     for k in ("minimum", "mean", "squared_mean", "dsf", "frequency"):
         assert gc.bit_diff(fused[k], split[k]) is None, k
     assert fused["rng"] == split["rng"] and fused["generation"] == split["generation"]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_split_loop_graphs_match_plain_launches(prec, monkeypatch):
+    # the split loop's generations run as captured HIP graphs (8 generations
+    # per graph, replayed across batches with plain launches and host
+    # synchronizations between them); every replay's effect is checked
+    # through the whole trajectory against plain launches, bit for bit
+    print(f"[gpu] split loop graphs vs plain launches, {prec}", flush=True)
+    t, f, s = _synthetic("isf_c1_T33.bin")
+    p = da.make_problem(t, f, s, da.frequency_grid(512, 64.0), 1.35, population_size=8, normalize=True)
+    kw = dict(split_loop=True) if prec == "fp64" else dict(precision=prec)
+    with da.Engine(p) as e:
+        graphs = e.run(5, 700, 0.0, **kw)
+        again = e.run(5, 700, 0.0, **kw)
+        monkeypatch.setenv("DEAC_SPLIT_GRAPH", "0")
+        plain = e.run(5, 700, 0.0, **kw)
+    assert graphs["profile"]["graph_launches"] >= 80 and plain["profile"]["graph_launches"] == 0
+    for k in ("minimum", "mean", "squared_mean", "dsf", "frequency"):
+        assert gc.bit_diff(graphs[k], plain[k]) is None, k
+        assert gc.bit_diff(graphs[k], again[k]) is None, k
+    assert graphs["rng"] == plain["rng"] and graphs["generation"] == plain["generation"]
+    if prec == "fp64":
+        fused = None
+        with da.Engine(p) as e:
+            fused = e.run(5, 700, 0.0)
+        assert gc.bit_diff(graphs["minimum"], fused["minimum"]) is None
