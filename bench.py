@@ -178,7 +178,10 @@ def batch_precision_accuracy(da, prob, gens):
 
 
 def uneven_batch(da, gens):
-    """The uneven batch for `gens` timed generations (after one warm-up run)."""
+    """The uneven batch for `gens` timed generations (after one warm-up run):
+    with the closed-form launch schedule, then with the schedule the
+    autotuner (deac_batch_tune, 40 generations per candidate) picks, at FP64
+    and at BF16x3 K.S."""
     import make_uneven as mu
     insts = mu.instances()
     probs = [da.make_problem(x["tau"], x["isf"], x["err"], da.frequency_grid(x["G"], x["omega_max"]), mu.TEMPERATURE,
@@ -186,21 +189,62 @@ def uneven_batch(da, gens):
                              normalize=True, spectra_type="bfull") for x in insts]
     seeds = [x["seed"] for x in insts]
     flop_per_gen = sum(x["P"] * 2 * (1 if x["zero_temperature"] else 2) * x["G"] * x["T"] for x in insts)
-    with da.Batch(probs) as b:
-        b.run(seeds, 3, 0.0, track_stats=False)
+
+    def timed(b, prec="fp64"):
+        b.run(seeds, 3, 0.0, track_stats=False, precision=prec)
         t0 = time.perf_counter()
-        b.run(seeds, gens + 1, 0.0, track_stats=False)
+        b.run(seeds, gens + 1, 0.0, track_stats=False, precision=prec)
         wall = time.perf_counter() - t0
         prof = b.profile
-    loop_s = prof["loop_ms"] / 1e3
-    inst_gens = prof["generations"]
-    return {"workload": f"{len(insts)} instances (64 wavevectors x 16 seeds), T in {{33,65,129,257}}, "
-                        "G in {1024..8192}, omega_max in [128,512], standard/normalization/zero-T, P=16, "
-                        f"{gens} generations each, one deac_batch_run",
-            "instance_generations_per_s": inst_gens / loop_s,
-            "genome_evals_per_s": inst_gens * mu.POPULATION / loop_s,
-            "fp64_tflops": flop_per_gen * gens / loop_s / 1e12,
-            "ms": prof["loop_ms"], "wall_s_including_host_setup": wall}
+        loop_s = prof["loop_ms"] / 1e3
+        return {"instance_generations_per_s": prof["generations"] / loop_s,
+                "genome_evals_per_s": prof["generations"] * mu.POPULATION / loop_s,
+                "fp64_equivalent_tflops": flop_per_gen * gens / loop_s / 1e12,
+                "ms": prof["loop_ms"], "wall_s_including_host_setup": wall}
+
+    with da.Batch(probs) as b:
+        b.schedule = {"group_streams": 3, "group_order": 2, "instance_order": 0}  # the closed-form rule
+        rule = timed(b)
+        t0 = time.perf_counter()
+        chosen = b.tune(seeds, 40)
+        tune_s = time.perf_counter() - t0
+        tuned = timed(b)
+        mixed = timed(b, "bf16x3")
+    out = {"workload": f"{len(insts)} instances (64 wavevectors x 16 seeds), T in {{33,65,129,257}}, "
+                       "G in {1024..8192}, omega_max in [128,512], standard/normalization/zero-T, P=16, "
+                       f"{gens} generations each, one deac_batch_run",
+           "closed_form_schedule": rule, "tuned_schedule": chosen, "tuning_s": tune_s, "tuned": tuned,
+           "tuned_bf16x3": mixed, "tuned_over_closed_form": tuned["genome_evals_per_s"] / rule["genome_evals_per_s"]}
+    out.update(tuned)  # the headline uneven figures: FP64 with the tuned schedule
+    return out
+
+
+def schedule_grid(da, gens):
+    """One JSON line per launch schedule of the uneven batch (FP64, `gens`
+    generations after a warm-up), then the autotuner's choice and its rate."""
+    import make_uneven as mu
+    insts = mu.instances()
+    probs = [da.make_problem(x["tau"], x["isf"], x["err"], da.frequency_grid(x["G"], x["omega_max"]), mu.TEMPERATURE,
+                             model=x["model"], zero_temperature=x["zero_temperature"], population_size=x["P"],
+                             normalize=True, spectra_type="bfull") for x in insts]
+    seeds = [x["seed"] for x in insts]
+    with da.Batch(probs) as b:
+        b.run(seeds, 3, 0.0, track_stats=False)
+        for st in (1, 2, 3):
+            for go in (0, 1, 2):
+                for io in (0, 1):
+                    b.schedule = {"group_streams": st, "group_order": go, "instance_order": io}
+                    b.run(seeds, gens + 1, 0.0, track_stats=False)
+                    p = b.profile
+                    print(json.dumps({"group_streams": st, "group_order": go, "instance_order": io,
+                                      "ms_per_generation": p["loop_ms"] / gens,
+                                      "genome_evals_per_s": p["generations"] * mu.POPULATION / (p["loop_ms"] / 1e3)}),
+                          flush=True)
+        chosen = b.tune(seeds, 40)
+        b.run(seeds, gens + 1, 0.0, track_stats=False)
+        p = b.profile
+        print(json.dumps({"autotuned": chosen, "ms_per_generation": p["loop_ms"] / gens,
+                          "genome_evals_per_s": p["generations"] * mu.POPULATION / (p["loop_ms"] / 1e3)}), flush=True)
 
 
 def precision_sweep(da):
@@ -304,6 +348,8 @@ def main():
                     help="generations of the seed batch's FP64/FP32/BF16x3 accuracy comparison (0: skip)")
     ap.add_argument("--trajectory-gens", type=int, default=2000,
                     help="generations of the T=257, G=16384 precision trajectories (0: skip)")
+    ap.add_argument("--schedule-grid", action="store_true",
+                    help="only time the uneven batch under every launch schedule and the autotuner's choice")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -326,6 +372,9 @@ def main():
         return
 
     import deac_amd as da
+    if args.schedule_grid:
+        schedule_grid(da, args.uneven_gens)
+        return
     t, f, s = load_isf()
     freq = da.frequency_grid(G, OMEGA_MAX)
     flop_per_eval = 4 * G * T

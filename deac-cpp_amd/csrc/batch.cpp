@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -72,6 +73,9 @@ std::string problem_key(const deac_problem &p)
     return k;
 }
 
+std::mutex g_sched_lock;
+std::map<std::string, deac_batch_schedule> g_sched_cache;  // tuned schedules by batch signature
+
 }  // namespace
 
 struct deac_batch {
@@ -98,6 +102,8 @@ struct deac_batch {
     double *scratch = nullptr;                 // this device's statistics scratch (deac_batch_run_noalloc)
     std::vector<long> reached;                 // per instance: generation reached by the last run (absolute)
     std::vector<char> epar_valid;              // per instance: epar holds the current population's model ISF
+    deac_batch_schedule sched{3, 2, 0, 0.0};   // launch schedule (deac_batch_tune)
+    std::string signature;                     // shapes of the instances, the process-wide schedule key
     deac_run_profile last{};
 
     ~deac_batch()
@@ -377,6 +383,19 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
         h->models.push_back(std::move(b));
     }
 
+    {
+        std::vector<std::string> shapes;
+        for (size_t i = 0; i < count; i++) {
+            const deac::Model &m = h->models[h->model_of[i]]->m;
+            shapes.push_back(std::to_string(m.T) + "x" + std::to_string(m.G) + "x" + std::to_string(m.P) + "x" +
+                             std::to_string(m.nsign));
+        }
+        std::sort(shapes.begin(), shapes.end());
+        for (auto &x : shapes) h->signature += x + ";";
+        std::lock_guard<std::mutex> lock(g_sched_lock);
+        auto it = g_sched_cache.find(h->signature);
+        if (it != g_sched_cache.end()) h->sched = it->second;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         (void)hipGetLastError();
@@ -592,19 +611,53 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
         for (size_t i = 0; i < n; i++)
             if (active[i]) groups[key_of(i, false)].push_back(i);
         if (groups.empty()) break;
-        // Longest instances first within a group (workgroups are dispatched
-        // in index order): per-generation cost ~ chunks x tiles per wave.
-        for (auto &[key, members] : groups)
-            std::stable_sort(members.begin(), members.end(), [&](size_t a, size_t b) {
-                const BatchModel &ma = *h->models[h->model_of[a]], &mb = *h->models[h->model_of[b]];
-                return static_cast<long>(ma.nchunk) * (ma.NT * 4 + ma.nrem) >
-                       static_cast<long>(mb.nchunk) * (mb.NT * 4 + mb.nrem);
+        // Order within a group (workgroups are dispatched in index order):
+        // longest instances first -- per-generation cost ~ chunks x tiles
+        // per wave -- or round robin over distinct problems.
+        auto cost = [&](size_t i) {
+            const BatchModel &b = *h->models[h->model_of[i]];
+            return static_cast<long>(b.nchunk) * (b.NT * 4 + b.nrem);
+        };
+        for (auto &[key, members] : groups) {
+            std::stable_sort(members.begin(), members.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+            if (h->sched.instance_order == 1) {
+                std::map<int, std::vector<size_t>> per;
+                std::vector<int> seen;
+                for (size_t i : members) {
+                    if (!per.count(h->model_of[i])) seen.push_back(h->model_of[i]);
+                    per[h->model_of[i]].push_back(i);
+                }
+                members.clear();
+                for (size_t r = 0;; r++) {
+                    bool any = false;
+                    for (int mdl : seen)
+                        if (r < per[mdl].size()) {
+                            members.push_back(per[mdl][r]);
+                            any = true;
+                        }
+                    if (!any) break;
+                }
+            }
+        }
+        // Order of the groups' launches.
+        std::vector<std::pair<Key, std::vector<size_t> *>> glist;
+        for (auto &[key, members] : groups) glist.push_back({key, &members});
+        if (h->sched.group_order != 2) {
+            auto gcost = [&](const std::vector<size_t> *v) {
+                long c = 0;
+                for (size_t i : *v) c += cost(i);
+                return c;
+            };
+            std::stable_sort(glist.begin(), glist.end(), [&](const auto &a, const auto &b) {
+                return h->sched.group_order == 0 ? gcost(a.second) > gcost(b.second) : gcost(a.second) < gcost(b.second);
             });
+        }
         size_t slot = 0;
         std::vector<std::pair<Key, std::pair<size_t, size_t>>> launches, evals_before, evals_after;
         std::vector<size_t> order;
         std::vector<size_t> mixed_slots;  // evolution slots of mixed instances
-        for (auto &[key, members] : groups) {
+        for (auto &[key, mp] : glist) {
+            const std::vector<size_t> &members = *mp;
             const size_t begin = slot;
             for (size_t i : members) {
                 const BatchModel &b = *h->models[h->model_of[i]];
@@ -689,19 +742,19 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
         for (auto &[key, range] : evals_before)
             BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), s));
         BATCH_HIP(hipEventRecord(h->kev1, s));
-        if (launches.size() == 1) {
-            auto &[key, range] = launches[0];
-            BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), s));
+        const int nstreams = std::clamp(h->sched.group_streams, 1, deac_batch::kGroupStreams);
+        if (launches.size() == 1 || nstreams == 1) {
+            for (auto &[key, range] : launches)
+                BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), s));
         } else {
             // shapes side by side, on the group streams
             BATCH_HIP(hipEventRecord(h->fork, s));
-            for (int g = 0; g < deac_batch::kGroupStreams; g++) BATCH_HIP(hipStreamWaitEvent(h->gstream[g], h->fork, 0));
+            for (int g = 0; g < nstreams; g++) BATCH_HIP(hipStreamWaitEvent(h->gstream[g], h->fork, 0));
             for (size_t k = 0; k < launches.size(); k++) {
                 auto &[key, range] = launches[k];
-                BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second),
-                                 h->gstream[k % deac_batch::kGroupStreams]));
+                BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), h->gstream[k % nstreams]));
             }
-            for (int g = 0; g < deac_batch::kGroupStreams; g++) {
+            for (int g = 0; g < nstreams; g++) {
                 BATCH_HIP(hipEventRecord(h->gdone[g], h->gstream[g]));
                 BATCH_HIP(hipStreamWaitEvent(s, h->gdone[g], 0));
             }
@@ -834,6 +887,91 @@ deac_status deac_batch_run_noalloc(deac_batch *h, uint64_t *rng_states, const de
         return DEAC_ERR_INVALID_ARGUMENT;
     }
     return batch_run_impl(h, rng_states, options, results, false, h->scratch);
+}
+
+deac_status deac_batch_set_schedule(deac_batch *h, const deac_batch_schedule *sc)
+{
+    if (!h || !sc || sc->group_streams < 1 || sc->group_streams > deac_batch::kGroupStreams || sc->group_order < 0 ||
+        sc->group_order > 2 || sc->instance_order < 0 || sc->instance_order > 1)
+        return DEAC_ERR_INVALID_ARGUMENT;
+    h->sched = *sc;
+    return DEAC_OK;
+}
+
+deac_status deac_batch_get_schedule(const deac_batch *h, deac_batch_schedule *sc)
+{
+    if (!h || !sc) return DEAC_ERR_INVALID_ARGUMENT;
+    *sc = h->sched;
+    return DEAC_OK;
+}
+
+deac_status deac_batch_tune(deac_batch *h, const uint64_t *rng_states, const deac_run_options *options,
+                            size_t generations, deac_batch_schedule *chosen)
+{
+    if (!h || !rng_states || !options || generations < 1) return DEAC_ERR_INVALID_ARGUMENT;
+    const size_t n = h->model_of.size();
+    // scratch outputs and a bounded run of every instance
+    std::vector<deac_run_options> opts(options, options + n);
+    std::vector<std::vector<double>> out(2 * n);
+    std::vector<deac_result> res(n);
+    for (size_t i = 0; i < n; i++) {
+        opts[i].number_of_generations = generations + 1;
+        opts[i].track_stats = 0;
+        opts[i].stop_minimum_fitness = -1.0;  // every candidate runs every generation
+        const size_t len = h->models[h->model_of[i]]->m.nsign == 2 ? 2 * h->models[h->model_of[i]]->m.G - 1
+                                                                   : h->models[h->model_of[i]]->m.G;
+        out[2 * i].assign(len, 0.0);
+        out[2 * i + 1].assign(len, 0.0);
+        res[i] = deac_result{};
+        res[i].dsf = out[2 * i].data();
+        res[i].frequency = out[2 * i + 1].data();
+    }
+    // candidates: every (streams, group order, instance order), less where
+    // a knob has nothing to order
+    std::map<std::tuple<int, int, int, int>, int> shapes;
+    for (size_t i = 0; i < n; i++) {
+        const BatchModel &b = *h->models[h->model_of[i]];
+        shapes[{options[i].ks_precision, options[i].ks_precision == DEAC_PRECISION_FP64 ? b.NT : b.NQ, b.NM,
+                options[i].ks_precision == DEAC_PRECISION_FP64 && b.nrem > 0 ? 1 : 0}] = 1;
+    }
+    const bool several_groups = shapes.size() > 1, several_models = h->models.size() > 1;
+    std::vector<deac_batch_schedule> cand;
+    for (int st = 1; st <= (several_groups ? deac_batch::kGroupStreams : 1); st++)
+        for (int go = 0; go <= (several_groups ? 2 : 0); go++)
+            for (int io = 0; io <= (several_models ? 1 : 0); io++) cand.push_back(deac_batch_schedule{st, go, io, 0.0});
+    const deac_batch_schedule keep = h->sched;
+    std::vector<uint64_t> states(rng_states, rng_states + 4 * n);
+    // a warm-up run with the current schedule, then two timed continuations
+    // of it per candidate (no host set-up between them)
+    deac_status st = batch_run_impl(h, states.data(), opts.data(), res.data(), false, nullptr);
+    if (st != DEAC_OK) return st;
+    double best_ms = 1e300;
+    deac_batch_schedule best = keep;
+    for (const deac_batch_schedule &c : cand) {
+        h->sched = c;
+        double ms = 1e300;
+        for (int rep = 0; rep < 2; rep++) {
+            st = batch_run_impl(h, states.data(), opts.data(), res.data(), true, nullptr);
+            if (st != DEAC_OK) {
+                h->sched = keep;
+                return st;
+            }
+            ms = std::min(ms, h->last.loop_ms);
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    best.ms_per_generation = best_ms / static_cast<double>(generations);
+    h->sched = best;
+    h->reached.clear();  // the tuning runs are not a run to continue
+    {
+        std::lock_guard<std::mutex> lock(g_sched_lock);
+        g_sched_cache[h->signature] = best;
+    }
+    if (chosen) *chosen = best;
+    return DEAC_OK;
 }
 
 

@@ -24,7 +24,8 @@ STATUS = {0: "success", 1: "invalid argument", 2: "no usable gfx950 device", 3: 
 
 EXPORTED = ["deac_create", "deac_run", "deac_destroy", "deac_output_length", "deac_seed_state",
             "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_error_message", "deac_last_run_profile",
-            "deac_batch_create", "deac_batch_run", "deac_batch_continue", "deac_batch_run_noalloc", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile", "deac_genome_length", "deac_model_isf"]
+            "deac_batch_create", "deac_batch_run", "deac_batch_continue", "deac_batch_run_noalloc", "deac_batch_tune", "deac_batch_set_schedule",
+            "deac_batch_get_schedule", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile", "deac_genome_length", "deac_model_isf"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -80,6 +81,11 @@ class RunProfile(C.Structure):
                 ("eval_ms", C.c_double), ("eval_launches", C.c_size_t), ("graph_launches", C.c_size_t)]
 
 
+class Schedule(C.Structure):
+    _fields_ = [("group_streams", C.c_int), ("group_order", C.c_int), ("instance_order", C.c_int),
+                ("ms_per_generation", C.c_double)]
+
+
 class DeacError(RuntimeError):
     def __init__(self, status, message=None):
         self.status = status
@@ -121,6 +127,13 @@ def lib():
         L.deac_batch_continue.restype = C.c_int
         L.deac_batch_run_noalloc.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOptions), C.POINTER(Result)]
         L.deac_batch_run_noalloc.restype = C.c_int
+        L.deac_batch_tune.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOptions), C.c_size_t,
+                                      C.POINTER(Schedule)]
+        L.deac_batch_tune.restype = C.c_int
+        L.deac_batch_set_schedule.argtypes = [C.c_void_p, C.POINTER(Schedule)]
+        L.deac_batch_set_schedule.restype = C.c_int
+        L.deac_batch_get_schedule.argtypes = [C.c_void_p, C.POINTER(Schedule)]
+        L.deac_batch_get_schedule.restype = C.c_int
         L.deac_batch_destroy.argtypes = [C.c_void_p]
         L.deac_batch_size.argtypes = [C.c_void_p]
         L.deac_batch_size.restype = C.c_size_t
@@ -318,6 +331,36 @@ class Batch:
         if len(seeds) != n:
             raise ValueError(f"one seed or state per problem: {len(seeds)} given for {n} problems")
         return self._run(seeds, number_of_generations, stop_minimum_fitness, track_stats, precision, False)
+
+    def tune(self, seeds, generations=20, precision="fp64"):
+        """deac_batch_tune: time the candidate launch schedules on `generations` generations and keep the
+        fastest; returns it as a dict."""
+        n = len(self.problems)
+        precs = _per_instance(precision, n)
+        states = (C.c_uint64 * (4 * n))()
+        for i, sd in enumerate(seeds):
+            w = seed_state(sd) if isinstance(sd, (int, np.integer)) else sd
+            for k in range(4):
+                states[4 * i + k] = int(w[k])
+        opts = (RunOptions * n)(*[RunOptions(generations + 1, 0.0, 0, PRECISIONS[precs[i]], 0) for i in range(n)])
+        sc = Schedule()
+        st = lib().deac_batch_tune(self.handle, states, opts, generations, C.byref(sc))
+        if st:
+            raise DeacError(st, _last_message())
+        return {k: getattr(sc, k) for k, _ in Schedule._fields_}
+
+    @property
+    def schedule(self):
+        sc = Schedule()
+        lib().deac_batch_get_schedule(self.handle, C.byref(sc))
+        return {k: getattr(sc, k) for k, _ in Schedule._fields_}
+
+    @schedule.setter
+    def schedule(self, d):
+        sc = Schedule(d["group_streams"], d["group_order"], d["instance_order"], d.get("ms_per_generation", 0.0))
+        st = lib().deac_batch_set_schedule(self.handle, C.byref(sc))
+        if st:
+            raise DeacError(st)
 
     def continue_run(self, number_of_generations, stop_minimum_fitness=1.0, track_stats=True, precision="fp64"):
         """deac_batch_continue: N more generations (N - 1 evolution steps) of every instance from where the

@@ -223,6 +223,28 @@ deac_status deac_batch_run(deac_batch *batch, uint64_t *rng_states, const deac_r
 deac_status deac_batch_continue(deac_batch *batch, uint64_t *rng_states, const deac_run_options *options,
                                 deac_result *results);
 
+/* Launch schedule of a batch: how the instances of different kernel shapes
+ * share the device (no effect on any result).  deac_batch_tune picks one by
+ * timing the candidates on the device; it is kept by the batch and, for
+ * batches of the same shapes (T, G, P, signs, precision of every instance),
+ * by the process. */
+typedef struct deac_batch_schedule {
+    int group_streams;         /* 1..3: kernel-shape groups launched side by side on this many streams */
+    int group_order;           /* 0: most expensive group first, 1: least expensive first, 2: by kernel shape */
+    int instance_order;        /* 0: longest instances first, 1: round robin over distinct problems */
+    double ms_per_generation;  /* deac_batch_tune: the chosen schedule's time per generation of the batch */
+} deac_batch_schedule;
+
+/* Times every candidate schedule on `generations` generations of this batch
+ * (after one run of that length from rng_states with the options' problems
+ * and precisions, each candidate continues it twice; outputs discarded,
+ * rng_states not modified), keeps the fastest for later runs and returns it.
+ * No counterpart in the reference. */
+deac_status deac_batch_tune(deac_batch *batch, const uint64_t *rng_states, const deac_run_options *options,
+                            size_t generations, deac_batch_schedule *chosen);
+deac_status deac_batch_set_schedule(deac_batch *batch, const deac_batch_schedule *schedule);
+deac_status deac_batch_get_schedule(const deac_batch *batch, deac_batch_schedule *schedule);
+
 /* deac_batch_run without device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, serving loops, harnesses that forbid hipMalloc).  Results

@@ -165,7 +165,7 @@ def test_binding_structs_match_the_header_layout(tmp_path):
     # the ctypes mirrors of the ABI structs have the C compiler's sizes and
     # field offsets (a C program built against the header prints them)
     structs = {"deac_problem": da.Problem, "deac_run_options": da.RunOptions, "deac_result": da.Result,
-               "deac_run_profile": da.RunProfile}
+               "deac_run_profile": da.RunProfile, "deac_batch_schedule": da.Schedule}
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "deac_engine.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')

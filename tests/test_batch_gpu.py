@@ -319,3 +319,30 @@ def test_continue_equals_one_longer_run():
     for w2, m in zip(whole2, more):
         assert gc.bit_diff(m["dsf"], w2["dsf"]) is None and m["rng"] == w2["rng"]
         assert m["minimum"][-1] == w2["minimum"][-1]
+
+
+def test_tuned_schedule_changes_no_result():
+    # deac_batch_tune times the launch schedules (streams, group order,
+    # instance order) and keeps the fastest; results under any schedule are
+    # those of deac_run, bit for bit
+    print("[gpu] batch schedule tuning", flush=True)
+    probs = [_synthetic_problem(18, 80, 8), _synthetic_problem(65, 300, 16), _synthetic_problem(130, 200, 32),
+             _synthetic_problem(260, 96, 12)]
+    seeds = [2, 3, 4, 5]
+    with da.Batch(probs) as b:
+        ref = b.run(seeds, 30, 0.0, True)
+        chosen = b.tune(seeds, 10)
+        assert 1 <= chosen["group_streams"] <= 3 and chosen["ms_per_generation"] > 0
+        assert b.schedule["group_streams"] == chosen["group_streams"]
+        for sc in ({"group_streams": 1, "group_order": 1, "instance_order": 1},
+                   {"group_streams": 2, "group_order": 0, "instance_order": 0}):
+            b.schedule = sc
+            out = b.run(seeds, 30, 0.0, True)
+            for o, r in zip(out, ref):
+                assert not _compare(o, r)
+                assert o["rng"] == r["rng"]
+        with pytest.raises(da.DeacError):
+            b.schedule = {"group_streams": 4, "group_order": 0, "instance_order": 0}
+    with da.Batch(probs) as b2:  # the process keeps the tuned schedule for these shapes
+        assert b2.schedule["group_streams"] == chosen["group_streams"]
+        assert b2.schedule["group_order"] == chosen["group_order"]
