@@ -219,6 +219,25 @@ def uneven_batch(da, gens):
     return out
 
 
+def role_cycles(da, args):
+    """Per-wavefront cycles of the batch kernels on the seed batch, one JSON
+    line per precision: the first `--warmup` generations of a run, then
+    `--steps` continued generations (profiling build only)."""
+    t, f, s = load_isf()
+    prob = da.make_problem(t, f, s, da.frequency_grid(G, OMEGA_MAX), TEMPERATURE, population_size=BATCH["P"],
+                           normalize=True, spectra_type="bfull")
+    seeds = list(range(1, BATCH["seeds"] + 1))
+    with da.Batch([prob] * len(seeds)) as b:
+        for prec in args.batch_precisions.split(","):
+            da.role_cycles(prec, reset=True)
+            b.run(seeds, args.warmup + 1, BATCH["stop"], track_stats=False, precision=prec)
+            early = da.role_cycles(prec, reset=True)
+            b.continue_run(args.steps + 1, BATCH["stop"], track_stats=False, precision=prec)
+            late = da.role_cycles(prec, reset=True)
+            print(json.dumps({"precision": prec, "generations_1_to": args.warmup, "early": early,
+                              "continued_generations": args.steps, "steady": late}), flush=True)
+
+
 def schedule_grid(da, gens):
     """One JSON line per launch schedule of the uneven batch (FP64, `gens`
     generations after a warm-up), then the autotuner's choice and its rate."""
@@ -348,6 +367,9 @@ def main():
                     help="generations of the seed batch's FP64/FP32/BF16x3 accuracy comparison (0: skip)")
     ap.add_argument("--trajectory-gens", type=int, default=2000,
                     help="generations of the T=257, G=16384 precision trajectories (0: skip)")
+    ap.add_argument("--role-cycles", action="store_true",
+                    help="only report the batch kernels' per-wavefront cycles on the seed batch "
+                         "(needs the profiling build: make -C deac-cpp_amd -B ROLE_TIMERS=1)")
     ap.add_argument("--schedule-grid", action="store_true",
                     help="only time the uneven batch under every launch schedule and the autotuner's choice")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -374,6 +396,9 @@ def main():
     import deac_amd as da
     if args.schedule_grid:
         schedule_grid(da, args.uneven_gens)
+        return
+    if args.role_cycles:
+        role_cycles(da, args)
         return
     t, f, s = load_isf()
     freq = da.frequency_grid(G, OMEGA_MAX)

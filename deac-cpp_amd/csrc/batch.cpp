@@ -889,6 +889,23 @@ deac_status deac_batch_run_noalloc(deac_batch *h, uint64_t *rng_states, const de
     return batch_run_impl(h, rng_states, options, results, false, h->scratch);
 }
 
+deac_status deac_batch_role_cycles(int precision, unsigned long long *cycles, int reset)
+{
+    if (!cycles) return DEAC_ERR_INVALID_ARGUMENT;
+    const hipError_t e = precision == DEAC_PRECISION_FP64 ? deac::batch_role_cycles_fp64(cycles, reset)
+                                                          : deac::batch_role_cycles_mixed(cycles, reset);
+    if (e == hipErrorNotSupported) {
+        (void)hipGetLastError();
+        deac::last_message() = "the library was built without role timers (make ROLE_TIMERS=1)";
+        return DEAC_ERR_UNSUPPORTED;
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return DEAC_ERR_DEVICE;
+    }
+    return DEAC_OK;
+}
+
 deac_status deac_batch_set_schedule(deac_batch *h, const deac_batch_schedule *sc)
 {
     if (!h || !sc || sc->group_streams < 1 || sc->group_streams > deac_batch::kGroupStreams || sc->group_order < 0 ||

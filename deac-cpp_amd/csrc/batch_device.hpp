@@ -134,12 +134,27 @@ __device__ __forceinline__ int trial_index(int m, int sl)
 
 // Workgroup barrier of the chunk pipeline: LDS traffic retired, global loads
 // in flight are kept (each wave waits for what it must before arriving).
+#ifdef DEAC_ROLE_TIMERS
+// Profiling build only (make ROLE_TIMERS=1): per wavefront of workgroup 0,
+// the shader-clock cycles spent waiting in the pipeline's barriers and in
+// the whole launch, and the generations evolved (deac_batch_role_cycles).
+__shared__ unsigned long long g_wait[kBatchWaves];
+__device__ unsigned long long g_role[kBatchWaves][3];
+#endif
+
 __device__ __forceinline__ void step_barrier()
 {
     DEAC_WAIT_LDS();
     asm volatile("" ::: "memory");
+#ifdef DEAC_ROLE_TIMERS
+    const unsigned long long t0 = __builtin_amdgcn_s_memtime();
+#endif
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
+#ifdef DEAC_ROLE_TIMERS
+    const unsigned long long t1 = __builtin_amdgcn_s_memtime();
+    if ((threadIdx.x & 63) == 0) g_wait[threadIdx.x >> 6] += t1 - t0;
+#endif
 }
 
 // A pointer held in scalar registers: its two halves made wave-uniform, so
@@ -1147,6 +1162,10 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
     const int P = d.P;
     const gcdouble *obs = (const gcdouble *)d.obs, *sig = (const gcdouble *)d.sig;
 
+#ifdef DEAC_ROLE_TIMERS
+    if (threadIdx.x < kBatchWaves) g_wait[threadIdx.x] = 0;
+    const unsigned long long tstart = __builtin_amdgcn_s_memtime();
+#endif
     for (int k = threadIdx.x; k < 256 * 4; k += kBatchThreads) sh.rows[k] = ((const gcu64 *)d.walk_rows)[k];
     for (int k = threadIdx.x; k < d.T; k += kBatchThreads) {
         sh.obs[k] = obs[k];
@@ -1330,6 +1349,14 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
             fw[m * 2 + sg] = sh.f_old[m][sg];
         }
     }
+#ifdef DEAC_ROLE_TIMERS
+    if (blockIdx.x == 0 && (threadIdx.x & 63) == 0) {
+        const unsigned long long tend = __builtin_amdgcn_s_memtime();
+        atomicAdd(&g_role[threadIdx.x >> 6][0], g_wait[threadIdx.x >> 6]);
+        atomicAdd(&g_role[threadIdx.x >> 6][1], tend - tstart);
+        atomicAdd(&g_role[threadIdx.x >> 6][2], static_cast<unsigned long long>(g - g0));
+    }
+#endif
     if (threadIdx.x == 0) {
         __attribute__((address_space(1))) uint64_t *rng = (__attribute__((address_space(1))) uint64_t *)d.rng;
         __attribute__((address_space(1))) long *status = (__attribute__((address_space(1))) long *)d.status;

@@ -71,4 +71,19 @@ hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, int r
     return hipErrorInvalidValue;
 }
 
+
+hipError_t batch_role_cycles_fp64(unsigned long long *out, int reset)
+{
+#ifdef DEAC_ROLE_TIMERS
+    hipError_t e = hipMemcpyFromSymbol(out, HIP_SYMBOL(g_role), sizeof(unsigned long long) * 3 * kBatchWaves);
+    if (e != hipSuccess || !reset) return e;
+    const unsigned long long z[3 * kBatchWaves] = {};
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_role), z, sizeof z);
+#else
+    (void)out;
+    (void)reset;
+    return hipErrorNotSupported;
+#endif
+}
+
 }  // namespace deac

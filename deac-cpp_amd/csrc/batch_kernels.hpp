@@ -64,6 +64,12 @@ constexpr size_t kBatchScratchDoubles = size_t(1) << 22;
 constexpr int kBatchMaxDevices = 64;
 hipError_t batch_scratch(int device, double **ptr);
 
+// Profiling builds (DEAC_ROLE_TIMERS): per wavefront of workgroup 0, summed over launches,
+// [wave][0] barrier-wait cycles, [1] launch cycles, [2] generations; hipErrorNotSupported
+// otherwise.  One counter set per kernel family.
+hipError_t batch_role_cycles_fp64(unsigned long long *out, int reset);
+hipError_t batch_role_cycles_mixed(unsigned long long *out, int reset);
+
 // Mixed K.S kernel (precision DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3).  nq: (kernel
 // tile, member tile) pairs per chain wavefront, one of kBatchMixedNq; nm as above.
 constexpr int kBatchMixedNq[] = {1, 2, 3, 5, 9};

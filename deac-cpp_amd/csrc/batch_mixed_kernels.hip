@@ -57,4 +57,19 @@ hipError_t launch_batch_mixed(const BatchDesc *descs, int count, int precision, 
     return hipErrorInvalidValue;
 }
 
+
+hipError_t batch_role_cycles_mixed(unsigned long long *out, int reset)
+{
+#ifdef DEAC_ROLE_TIMERS
+    hipError_t e = hipMemcpyFromSymbol(out, HIP_SYMBOL(g_role), sizeof(unsigned long long) * 3 * kBatchWaves);
+    if (e != hipSuccess || !reset) return e;
+    const unsigned long long z[3 * kBatchWaves] = {};
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_role), z, sizeof z);
+#else
+    (void)out;
+    (void)reset;
+    return hipErrorNotSupported;
+#endif
+}
+
 }  // namespace deac

@@ -25,7 +25,7 @@ STATUS = {0: "success", 1: "invalid argument", 2: "no usable gfx950 device", 3: 
 EXPORTED = ["deac_create", "deac_run", "deac_destroy", "deac_output_length", "deac_seed_state",
             "deac_rng_advance", "deac_frequency_grid", "deac_status_string", "deac_last_error_message", "deac_last_run_profile",
             "deac_batch_create", "deac_batch_run", "deac_batch_continue", "deac_batch_run_noalloc", "deac_batch_tune", "deac_batch_set_schedule",
-            "deac_batch_get_schedule", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile", "deac_genome_length", "deac_model_isf"]
+            "deac_batch_get_schedule", "deac_batch_role_cycles", "deac_batch_destroy", "deac_batch_size", "deac_batch_last_profile", "deac_genome_length", "deac_model_isf"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -130,6 +130,8 @@ def lib():
         L.deac_batch_tune.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(RunOptions), C.c_size_t,
                                       C.POINTER(Schedule)]
         L.deac_batch_tune.restype = C.c_int
+        L.deac_batch_role_cycles.argtypes = [C.c_int, C.POINTER(C.c_ulonglong), C.c_int]
+        L.deac_batch_role_cycles.restype = C.c_int
         L.deac_batch_set_schedule.argtypes = [C.c_void_p, C.POINTER(Schedule)]
         L.deac_batch_set_schedule.restype = C.c_int
         L.deac_batch_get_schedule.argtypes = [C.c_void_p, C.POINTER(Schedule)]
@@ -402,6 +404,24 @@ class Batch:
             o.update(generation=r.generation, minimum_fitness=r.minimum_fitness, best_index=r.best_index,
                      rng=[int(states[4 * i + w]) for w in range(4)])
         return outs
+
+
+ROLES = ["chain0", "chain1", "chain2", "chain3"] + [f"worker{i}" for i in range(8)] + ["walk", "mask0", "mask1",
+                                                                                    "mask2"]
+
+
+def role_cycles(precision="fp64", reset=False):
+    """deac_batch_role_cycles (profiling builds): per wavefront role, cycles
+    per generation spent working (launch cycles minus barrier waits), and the
+    launch cycles per generation."""
+    buf = (C.c_ulonglong * 48)()
+    st = lib().deac_batch_role_cycles(PRECISIONS[precision], buf, int(reset))
+    if st:
+        raise DeacError(st, _last_message())
+    v = np.array(list(buf), dtype=np.float64).reshape(16, 3)
+    gens = max(v[0, 2], 1.0)
+    return {"generations": int(v[0, 2]), "cycles_per_generation": v[0, 1] / gens,
+            "work_cycles_per_generation": {n: (v[i, 1] - v[i, 0]) / gens for i, n in enumerate(ROLES)}}
 
 
 def run(problem, seed, number_of_generations, stop_minimum_fitness=1.0, track_stats=True):

@@ -243,6 +243,15 @@ typedef struct deac_batch_schedule {
 deac_status deac_batch_tune(deac_batch *batch, const uint64_t *rng_states, const deac_run_options *options,
                             size_t generations, deac_batch_schedule *chosen);
 deac_status deac_batch_set_schedule(deac_batch *batch, const deac_batch_schedule *schedule);
+
+/* Profiling builds only (make ROLE_TIMERS=1; DEAC_ERR_UNSUPPORTED otherwise):
+ * per wavefront w of the first workgroup of every batch launch of the
+ * kernel family of `precision` (FP64, or the mixed FP32/BF16X3 kernel),
+ * summed since the last reset: cycles[3w] shader-clock cycles waiting in the
+ * chunk pipeline's barriers, cycles[3w+1] cycles of the launches,
+ * cycles[3w+2] generations evolved.  Wavefronts 0-3 chains, 4-11 workers,
+ * 12 walk, 13-15 crossover masks.  No counterpart in the reference. */
+deac_status deac_batch_role_cycles(int precision, unsigned long long cycles[48], int reset);
 deac_status deac_batch_get_schedule(const deac_batch *batch, deac_batch_schedule *schedule);
 
 /* deac_batch_run without device allocation.  No counterpart in the
