@@ -123,6 +123,22 @@ void dor_kernel_terms(const dor_problem *p, double *kp, double *km) /* deac.cpp:
                 kp[k] = df * exp(-t * w);
                 continue;
             }
+            if (p->spectra_type == DOR_BDSF) {          /* deac.cpp:322, 330, 337, 362, 370, 378 */
+                if (p->model == DOR_MODEL_HYPERBOLIC) {
+                    const double bo2mt = 0.5 * beta - t;
+                    kp[k] = df * cosh(bo2mt * w);
+                } else {
+                    const double bmt = beta - t;
+                    const double num = exp(-bmt * w) + exp(-t * w);
+                    if (p->model == DOR_MODEL_NORMALIZATION) {
+                        const double den = 1.0 + exp(-beta * w);
+                        kp[k] = df * (num / den);
+                    } else {
+                        kp[k] = df * num;
+                    }
+                }
+                continue;
+            }
             switch (p->model) {
             case DOR_MODEL_HYPERBOLIC: {                /* deac.cpp:323-324, 363-364 */
                 /* The reference build hoists -0.5*beta and 1.5*beta out of the
@@ -174,6 +190,20 @@ void dor_moment_terms(const dor_problem *p, int which, double *tp, double *tm)
             tp[j] = a;
             continue;
         }
+        if (p->spectra_type == DOR_BDSF) {              /* deac.cpp:508-526, 645-661, 745-761 */
+            double c;                                   /* the model's factor */
+            if (p->model == DOR_MODEL_HYPERBOLIC)
+                c = which == 0 ? cosh(0.5 * beta * w) : sinh(0.5 * beta * w);
+            else if (p->model == DOR_MODEL_NORMALIZATION)
+                c = which == 0 ? 1.0 : tanh(0.5 * beta * w);
+            else
+                c = which == 0 ? 1.0 + exp(-beta * w) : 1.0 - exp(-beta * w);
+            if (which == 0) a = p->model == DOR_MODEL_NORMALIZATION ? df : df * c;
+            else if (which == 1) a = df * w * c;
+            else a = df * pow(w, 3) * w * c;
+            tp[j] = a;
+            continue;
+        }
         switch (p->model) {
         case DOR_MODEL_HYPERBOLIC: {
             const double e = exp(-0.5 * beta * w);
@@ -200,6 +230,24 @@ void dor_moment_terms(const dor_problem *p, int which, double *tp, double *tm)
         tp[j] = a;
         if (tm) tm[j] = b;
     }
+}
+
+int dor_negative_first_moment(const dor_problem *p, double *dt, double *value, double *error)
+{
+    /* deac.cpp:879-894.  The error keeps only the last timeslice's term (the
+     * reference assigns it in the loop rather than accumulating it). */
+    if (!(p->spectra_type == DOR_BDSF && !p->zero_temperature && p->use_negative_first_moment)) return 0;
+    const size_t T = p->number_of_timeslices;
+    double v = 0.0, e = 0.0;
+    for (size_t j = 0; j < T; j++) {
+        const double d = dor_trapezoid_weight(p->imaginary_time, T, j);
+        dt[j] = d;
+        v = fma(p->isf[j], d, v);
+        e = (p->isf_error[j] * p->isf_error[j]) * (d * d);
+    }
+    *value = v;
+    *error = sqrt(e);
+    return 1;
 }
 
 /* ------------------------------------------------------------ reductions */
@@ -281,12 +329,39 @@ void dor_set_mutant_indices(uint64_t s[4], size_t *mi, size_t i0, size_t P) /* d
 }
 
 /* ------------------------------------------------------------ the run */
+/* The contraction of the trial population (deac.cpp:1389-1394).  The
+ * two-sign builds call matrix_multiply_LxM_by_MxN out of line (a scalar FMA
+ * chain).  The detailed-balance builds (bdsf, the zero-temperature one
+ * included) have one call in the loop, which the reference build inlines
+ * and vectorises like matrix_multiply_MxN_by_Nx1: a fold-left of rounded
+ * products with the last odd element fused (dor_dot_fold4).  Their initial
+ * population (deac.cpp:860) still goes through the out-of-line FMA chain.
+ * Read from the x86 assembly of the reference build; pinned by the bdsf and
+ * zero-temperature goldens. */
+static int loop_contraction_folds(const dor_problem *p)
+{
+    return p->spectra_type == DOR_BDSF;
+}
+
+/* The -1st-moment term of the bosonic detailed-balance build (deac.cpp:
+ * 868-920, 1398-1416): weights, target value and error; used = 0 when off. */
+typedef struct nfm_term {
+    int used;
+    double value, error;
+    double *dt;
+} nfm_term;
 
 /* fitness of one member, deac.cpp:957-976 (initial) and 1539-1555 (loop) */
-static double member_fitness(const dor_problem *p, const double *model_isf, double first_i, double third_i)
+static double member_fitness(const dor_problem *p, const double *model_isf, double first_i, double third_i,
+                             const nfm_term *nfm)
 {
     const size_t T = p->number_of_timeslices;
     double fit = dor_chi_square(p->isf, model_isf, p->isf_error, T) / (double)T;
+    if (nfm->used) {                                    /* deac.cpp:961-965, 1543-1547 */
+        const double m = dor_dot_fold4(0.0, model_isf, nfm->dt, T);
+        const double q = (nfm->value - m) / nfm->error;
+        fit = fma(q, q, fit);
+    }
     if (p->first_moment >= 0.0) {
         const double d = first_i - p->first_moment;
         fit += (d * d) / p->first_moment;
@@ -302,8 +377,10 @@ int dor_run(const dor_problem *p, uint64_t rng[4], dor_result *r)
 {
     const size_t T = p->number_of_timeslices, G = p->genome_size, P = p->population_size;
     const size_t N = p->number_of_generations;
-    const int nsign = p->zero_temperature ? 1 : 2;
+    const int nsign = (p->zero_temperature || p->spectra_type == DOR_BDSF) ? 1 : 2;
     const size_t row = (size_t)nsign * G;       /* doubles per member, [sign][gene] */
+    nfm_term nfm = {0, 0.0, 0.0, malloc(sizeof(double) * T)};
+    nfm.used = dor_negative_first_moment(p, nfm.dt, &nfm.value, &nfm.error);
     const int use_first = p->first_moment >= 0.0, use_third = p->third_moment >= 0.0;
     const double zeroth_moment = p->isf[0];     /* deac.cpp:306 */
 
@@ -365,7 +442,7 @@ int dor_run(const dor_problem *p, uint64_t rng[4], dor_result *r)
             for (int s = 0; s < nsign; s++) e = dor_fma_chain(e, kern + s * T * G + t * G, x + s * G, G);
             model_isf[i * T + t] = e;
         }
-        fitness[i] = member_fitness(p, model_isf + i * T, first[i], third[i]);
+        fitness[i] = member_fitness(p, model_isf + i * T, first[i], third[i], &nfm);
     }
 
     /* control parameters, deac.cpp:990-1032 */
@@ -443,10 +520,13 @@ int dor_run(const dor_problem *p, uint64_t rng[4], dor_result *r)
                 for (int s = 0; s < nsign; s++) tm = dor_dot_fold4(tm, y + s * G, tterm + s * G, G);
             for (size_t t = 0; t < T; t++) {
                 double e = 0.0;
-                for (int s = 0; s < nsign; s++) e = dor_fma_chain(e, kern + s * T * G + t * G, y + s * G, G);
+                if (loop_contraction_folds(p))
+                    e = dor_dot_fold4(0.0, kern + t * G, y, G);
+                else
+                    for (int s = 0; s < nsign; s++) e = dor_fma_chain(e, kern + s * T * G + t * G, y + s * G, G);
                 model_isf[i * T + t] = e;
             }
-            const double fit = member_fitness(p, model_isf + i * T, fm, tm);
+            const double fit = member_fitness(p, model_isf + i * T, fm, tm, &nfm);
             if (fit <= fitness[i]) {
                 fitness[i] = fit;
                 for (int s = 0; s < nsign; s++) {
@@ -467,9 +547,14 @@ int dor_run(const dor_problem *p, uint64_t rng[4], dor_result *r)
     const double *xb = pop_old + best * row;
     for (size_t i = 0; i < G; i++) {
         const double w = p->frequency[i];
-        if (nsign == 1) {
+        if (nsign == 1) {                               /* deac.cpp:1605-1608, 1620-1671 */
+            double v = xb[i];
+            if (!p->zero_temperature) {                 /* bdsf */
+                if (p->model == DOR_MODEL_HYPERBOLIC) v = 0.5 * xb[i] * exp(0.5 * beta * w);
+                else if (p->model == DOR_MODEL_NORMALIZATION) v = xb[i] / (1.0 + exp(-beta * w));
+            }
             r->frequency_out[i] = w;
-            r->dsf[i] = xb[i];
+            r->dsf[i] = v;
             continue;
         }
         const size_t ip = G + i - 1, in = G - i - 1;
@@ -537,6 +622,6 @@ int dor_run(const dor_problem *p, uint64_t rng[4], dor_result *r)
 
     free(kern); free(pop_old); free(pop_new); free(cr_old); free(cr_new);
     free(fw_old); free(fw_new); free(fitness); free(model_isf); free(first);
-    free(third); free(nterm); free(fterm); free(tterm); free(mask);
+    free(third); free(nterm); free(fterm); free(tterm); free(mask); free(nfm.dt);
     return 0;
 }

@@ -24,8 +24,11 @@ extern "C" {
 
 /* spectral models (reference: -DDEAC_MODEL, src/deac/CMakeLists.txt:97-109) */
 enum { DOR_MODEL_STANDARD = 0, DOR_MODEL_HYPERBOLIC = 1, DOR_MODEL_NORMALIZATION = 2 };
-/* spectra types (reference: --spectra_type, deac.cpp:2040-2058) */
-enum { DOR_SPBSF = 0, DOR_SPFSF = 1, DOR_BFULL = 2, DOR_FFULL = 3 };
+/* spectra types (reference: --spectra_type, deac.cpp:2040-2058).  DOR_BDSF is
+ * the only type of the bosonic detailed-balance build
+ * (USE_BOSONIC_DETAILED_BALANCE_CONDITION_DSF, deac.cpp:2016-2018, 2041-2052):
+ * one sign, the detailed-balance kernel, G-length output. */
+enum { DOR_SPBSF = 0, DOR_SPFSF = 1, DOR_BFULL = 2, DOR_FFULL = 3, DOR_BDSF = 4 };
 
 typedef struct dor_problem {
     size_t number_of_timeslices;          /* T */
@@ -52,13 +55,14 @@ typedef struct dor_problem {
     double self_adapting_differential_weight_probability;
     double stop_minimum_fitness;
     int track_stats;
+    int use_negative_first_moment;        /* bdsf at T > 0: the -1st-moment term (deac.cpp:868-920) */
 } dor_problem;
 
 typedef struct dor_result {
     size_t generation;
     double minimum_fitness;
     size_t best_index;
-    double *dsf;                 /* out: 2G-1 values (G when zero_temperature) */
+    double *dsf;                 /* out: 2G-1 values (G when zero_temperature or bdsf) */
     double *frequency_out;       /* out: same length */
     double *stats_mean;          /* out: N values when track_stats, may be NULL */
     double *stats_minimum;
@@ -85,6 +89,10 @@ double dor_trapezoid_weight(const double *frequency, size_t genome_size, size_t 
 void dor_kernel_terms(const dor_problem *p, double *kp, double *km);
 /* deac.cpp:495-538 (which = 0), 628-668 (which = 1), 728-768 (which = 3) */
 void dor_moment_terms(const dor_problem *p, int which, double *tp, double *tm);
+
+/* deac.cpp:875-894: the -1st-moment weights dt (T), the moment of the ISF
+ * and its error; returns 0 unless the term is used (bdsf, T > 0). */
+int dor_negative_first_moment(const dor_problem *p, double *dt, double *value, double *error);
 
 /* deac.cpp:102-108 as inlined into deac() by the reference build */
 double dor_dot_fold4(double acc, const double *a, const double *b, size_t n);

@@ -68,6 +68,27 @@ CASES["variant_single_particle"] = ("standard_sp", smooth_isf(9, -1.0), dict(
 CASES["variant_negative_weight"] = ("standard_neg", smooth_isf(9, -1.0), dict(
     T=1.3, N=300, P=8, M=16, omega_max=6.0, seed=32, stop=0.0, normalize=True))
 
+# the bosonic detailed-balance builds (one sign, spectra type bdsf): every
+# model with normalization, both moments and the -1st-moment term; a plain
+# run; odd sizes; and the zero-temperature build (which forces the -1st
+# moment off, deac.cpp:230-236)
+for model in ("standard", "normalization", "hyperbolic"):
+    CASES[f"bdsf_{model}"] = (f"{model}_bdsf", smooth_isf(9, 1.0), dict(
+        T=1.3, N=300, P=8, M=16, omega_max=6.0, seed=41, stop=0.0, spectra="bdsf", normalize=True,
+        first_moment=0.6, third_moment=3.0, third_moment_error=0.7, use_negative_first_moment=True))
+CASES["bdsf_standard_plain"] = ("standard_bdsf", smooth_isf(9, 1.0), dict(
+    T=1.3, N=300, P=8, M=16, omega_max=6.0, seed=42, stop=0.0, spectra="bdsf"))
+CASES["bdsf_normalization_odd_T7_P5_G9"] = ("normalization_bdsf", smooth_isf(7, 1.0), dict(
+    T=1.3, N=200, P=5, M=9, omega_max=5.0, seed=43, stop=0.0, spectra="bdsf", use_negative_first_moment=True))
+CASES["bdsf_hyperbolic_odd_T6_P6_G5"] = ("hyperbolic_bdsf", smooth_isf(6, 1.0), dict(
+    T=1.3, N=200, P=6, M=5, omega_max=5.0, seed=44, stop=0.0, spectra="bdsf", normalize=True,
+    use_negative_first_moment=True))
+CASES["zerot_bdsf_moments"] = ("zerot_bdsf", smooth_isf(9, 1.0), dict(
+    T=1.3, N=300, P=8, M=16, omega_max=6.0, seed=45, stop=0.0, spectra="bdsf", normalize=True,
+    first_moment=0.6, third_moment=3.0, third_moment_error=0.7, use_negative_first_moment=True))
+CASES["zerot_bdsf_odd_T5_P7_G6"] = ("zerot_bdsf", smooth_isf(5, 1.0), dict(
+    T=1.3, N=200, P=7, M=6, omega_max=5.0, seed=46, stop=0.0, spectra="bdsf"))
+
 
 def synthetic(name):
     a = np.fromfile(os.path.join(HERE, name)).reshape(3, -1)
@@ -88,6 +109,11 @@ CASES["c2_full"] = ("standard", "isf_c2_T65.bin", dict(T=1.35, N=200000, P=8, M=
 for _seed in (1, 200):
     CASES[f"c4_seed{_seed}_prefix2000"] = ("standard", "isf_c2_T65.bin", dict(
         T=1.35, N=2000, P=32, M=4096, omega_max=512.0, seed=_seed, stop=1.0, normalize=True, spectra="bfull"))
+CASES["bdsf_c1_shape_prefix2000"] = ("standard_bdsf", "isf_c1_T33.bin", dict(
+    T=1.35, N=2000, P=8, M=512, omega_max=64.0, seed=1, stop=0.0, normalize=True, spectra="bdsf",
+    use_negative_first_moment=True))
+CASES["zerot_c1_shape_prefix2000"] = ("zerot_bdsf", "isf_c1_T33.bin", dict(
+    T=1.35, N=2000, P=8, M=512, omega_max=64.0, seed=1, stop=0.0, normalize=True, spectra="bdsf"))
 CASES["c3_hyperbolic_prefix4"] = ("hyperbolic", "isf_c3_T257.bin", dict(T=1.35, N=4, P=64, M=16384, omega_max=512.0,
                                                                       seed=1, stop=0.0, spectra="bfull"))
 
@@ -109,13 +135,16 @@ def run_case(name, variant, data, args, workdir):
         cmd += ["-M", str(args["M"]), "--omega_max", repr(args["omega_max"])]
     if args.get("normalize"):
         cmd.append("--normalize")
+    if args.get("use_negative_first_moment"):
+        cmd.append("--use_negative_first_moment")
     for k in ("first_moment", "third_moment", "third_moment_error"):
         if k in args:
             cmd += ["--" + k, repr(args[k])]
     cmd.append(isf_path)
     out = subprocess.run(cmd, capture_output=True, text=True, check=True)
     uuid = str(args["seed"])
-    pre = os.path.join(workdir, f"deac-{spectra}_")
+    zt = "zT" if variant.startswith("zerot") else ""  # deac.cpp:1688-1692
+    pre = os.path.join(workdir, f"deac-{zt}{spectra}_")
     res = {k: np.fromfile(f"{pre}stats_fitness-{k}_{uuid}.bin") for k in ("mean", "minimum", "squared-mean")}
     res["dsf"] = np.fromfile(f"{pre}dsf_{uuid}.bin")
     res["frequency_out"] = np.fromfile(f"{pre}frequency_{uuid}.bin")

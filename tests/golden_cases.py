@@ -16,6 +16,10 @@ VARIANT_FLAGS = {
     "hyperbolic": dict(model="hyperbolic"),
     "standard_sp": dict(model="standard", renormalize=True),
     "standard_neg": dict(model="standard", allow_negative=True),
+    "standard_bdsf": dict(model="standard"),
+    "normalization_bdsf": dict(model="normalization"),
+    "hyperbolic_bdsf": dict(model="hyperbolic"),
+    "zerot_bdsf": dict(model="standard", zero_temperature=True),
 }
 
 
@@ -44,7 +48,8 @@ def problem_kwargs(case):
     kw = dict(VARIANT_FLAGS[m["variant"]])
     kw.update(spectra_type=m["spectra"], population_size=a["P"], normalize=bool(a.get("normalize", False)),
               first_moment=a.get("first_moment", -1.0), third_moment=a.get("third_moment", -1.0),
-              third_moment_error=a.get("third_moment_error", 0.0))
+              third_moment_error=a.get("third_moment_error", 0.0),
+              use_negative_first_moment=bool(a.get("use_negative_first_moment", False)))
     return kw
 
 

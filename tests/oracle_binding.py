@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "libdeac_oracle.so")
 
 MODELS = {"standard": 0, "hyperbolic": 1, "normalization": 2}
-SPECTRA = {"spbsf": 0, "spfsf": 1, "bfull": 2, "ffull": 3}
+SPECTRA = {"spbsf": 0, "spfsf": 1, "bfull": 2, "ffull": 3, "bdsf": 4}
 
 _dp = C.POINTER(C.c_double)
 
@@ -43,6 +43,7 @@ class DorProblem(C.Structure):
         ("self_adapting_differential_weight_probability", C.c_double),
         ("stop_minimum_fitness", C.c_double),
         ("track_stats", C.c_int),
+        ("use_negative_first_moment", C.c_int),
     ]
 
 
@@ -117,7 +118,7 @@ def make_problem(tau, isf, err, frequency, temperature, *, model="standard", spe
                  first_moment=-1.0, third_moment=-1.0, third_moment_error=0.0,
                  crossover_probability=0.9, self_adapting_crossover_probability=0.1,
                  differential_weight=0.9, self_adapting_differential_weight_probability=0.1,
-                 stop_minimum_fitness=1.0, track_stats=True):
+                 stop_minimum_fitness=1.0, track_stats=True, use_negative_first_moment=False):
     keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (tau, isf, err, frequency)]
     p = DorProblem(
         len(keep[0]), ptr(keep[0]), ptr(keep[1]), ptr(keep[2]), len(keep[3]), ptr(keep[3]),
@@ -125,7 +126,7 @@ def make_problem(tau, isf, err, frequency, temperature, *, model="standard", spe
         int(renormalize), number_of_generations, population_size, int(normalize), first_moment,
         third_moment, third_moment_error, crossover_probability, self_adapting_crossover_probability,
         differential_weight, self_adapting_differential_weight_probability, stop_minimum_fitness,
-        int(track_stats))
+        int(track_stats), int(use_negative_first_moment))
     p._keep = keep
     return p
 
@@ -133,7 +134,7 @@ def make_problem(tau, isf, err, frequency, temperature, *, model="standard", spe
 def run(problem, seed, want_population=False):
     """Run the oracle; returns a dict of numpy outputs."""
     G, P, N = problem.genome_size, problem.population_size, problem.number_of_generations
-    nsign = 1 if problem.zero_temperature else 2
+    nsign = 1 if problem.zero_temperature or problem.spectra_type == SPECTRA["bdsf"] else 2
     nout = G if nsign == 1 else 2 * G - 1
     out = {k: np.zeros(n) for k, n in (("dsf", nout), ("frequency", nout), ("mean", N), ("minimum", N),
                                         ("squared_mean", N), ("fitness", P))}
