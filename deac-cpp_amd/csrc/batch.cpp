@@ -162,6 +162,12 @@ deac_status plan_model(BatchModel &b, std::string &why)
         why = "batched runs do not renormalize each generation";
         return DEAC_ERR_UNSUPPORTED;
     }
+    if (m.bdsf) {
+        // the detailed-balance build sums the trials' model ISF as a fold of
+        // rounded products, which the batch kernel's MFMA chains do not
+        why = "batched runs do not take the detailed-balance spectra (bdsf)";
+        return DEAC_ERR_UNSUPPORTED;
+    }
     const int T = static_cast<int>(m.T), L = static_cast<int>(m.Kc);
     b.Ppad = m.P <= 16 ? 16 : 32;
     b.NM = b.Ppad / 16;

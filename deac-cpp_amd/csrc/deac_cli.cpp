@@ -4,7 +4,9 @@
 // log keys as the reference's main() (deac.cpp:1925-2197) and its output
 // section (deac.cpp:1687-1769).  The reference's compile-time variants are
 // runtime flags here: --model (DEAC_MODEL), --zero_temperature (the ZeroT
-// build, files prefixed deac-zT), --allow_negative_spectral_weight and
+// build, files prefixed deac-zT), --spectra_type bdsf (the bosonic
+// detailed-balance build, with --use_negative_first_moment),
+// --allow_negative_spectral_weight and
 // --single_particle_fermionic_spectral_function.
 //
 // Seed batches: --number_of_seeds K runs seeds seed .. seed+K-1 in one
@@ -15,8 +17,12 @@
 // uses (run keys sorted as strings).
 //
 // --ks_precision fp32 | bf16x3 runs the reconstruction with the model ISF's
-// K.S contraction in that precision (the split generation loop); the random
-// draws, chi^2, selection and statistics stay the reference's, in fp64.
+// K.S contraction in that precision; the random draws, chi^2, selection and
+// statistics stay the reference's, in fp64.  One seed runs the split
+// generation loop; a seed batch runs the batch kernel's delta form (E of the
+// parent plus K times the trial's change, FP64 re-evaluation every 1024
+// generations and at the end), so at these precisions a batched seed follows
+// a trajectory of its own, not that of a single --seed run.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -242,7 +248,7 @@ int main(int argc, char **argv)
     prog.add("", "--save_state", "Save state of DEAC algorithm. Saves the random number generator, population, and population fitness.", "", true);
     prog.add("", "--save_directory", "Directory to save results in.", "\"./deacresults\"");
     prog.add("", "--track_stats", "Track minimum fitness and other stats.", "", true);
-    prog.add("", "--spectra_type", "Choose spectral type for kernel factors [spbsf, spfsf, bfull, ffull].", "\"spfsf\"");
+    prog.add("", "--spectra_type", "Choose spectral type for kernel factors [spbsf, spfsf, bfull, ffull, bdsf].", "\"spfsf\"");
     prog.add("", "--model", "Spectral model [standard, hyperbolic, normalization].", "\"standard\"");
     prog.add("", "--zero_temperature", "Zero-temperature kernel exp(-tau*omega), positive frequencies only.", "", true);
     prog.add("", "--allow_negative_spectral_weight", "Allow negative spectral weights.", "", true);
@@ -269,9 +275,11 @@ int main(int argc, char **argv)
         if (nseeds == 1) std::cout << "uuid: " << uuid << std::endl;
 
         const std::string spectra = prog.get("--spectra_type").present ? prog.get("--spectra_type").value : "spfsf";
-        const std::map<std::string, int> spectra_ids = {{"spbsf", DEAC_SPBSF}, {"spfsf", DEAC_SPFSF}, {"bfull", DEAC_BFULL}, {"ffull", DEAC_FFULL}};
+        // bdsf: the reference's bosonic detailed-balance build (deac.cpp:2016-2018)
+        const std::map<std::string, int> spectra_ids = {{"spbsf", DEAC_SPBSF}, {"spfsf", DEAC_SPFSF}, {"bfull", DEAC_BFULL},
+                                                        {"ffull", DEAC_FFULL}, {"bdsf", DEAC_BDSF}};
         if (!spectra_ids.count(spectra)) {  // deac.cpp:2040-2058
-            std::cout << "Please choose spectra_type from the following options: spbsf, spfsf, bfull, ffull" << std::endl;
+            std::cout << "Please choose spectra_type from the following options: spbsf, spfsf, bfull, ffull, bdsf" << std::endl;
             std::exit(1);
         }
         const std::string model_name = prog.get("--model").present ? prog.get("--model").value : "standard";
@@ -354,6 +362,7 @@ int main(int argc, char **argv)
         p.self_adapting_crossover_probability = to_double(prog.get("--self_adapting_crossover_probability"), 0.1);
         p.differential_weight = to_double(prog.get("--differential_weight"), 0.9);
         p.self_adapting_differential_weight_probability = to_double(prog.get("--self_adapting_differential_weight_probability"), 0.1);
+        p.use_negative_first_moment = use_negative_first_moment && !zero_t;  // deac.cpp:230-236
         const double shift = to_double(prog.get("--self_adapting_differential_weight_shift"), 0.1);
         const double sadw = to_double(prog.get("--self_adapting_differential_weight"), 0.9);
         const double stop = to_double(prog.get("--stop_minimum_fitness"), 1.0);
@@ -412,9 +421,7 @@ int main(int argc, char **argv)
         const deac_run_options ro{N, stop, track_stats ? 1 : 0, ks_ids.at(ksp), 0};
         const char *msg = nullptr;
         bool done = false;
-        if (nseeds > 1 && ro.ks_precision != DEAC_PRECISION_FP64) {
-            std::cout << "seeds run one after another (the batch kernel computes K.S in fp64)" << std::endl;
-        } else if (nseeds > 1) {
+        if (nseeds > 1) {
             std::vector<deac_problem> probs(nseeds, p);
             deac_batch *b = nullptr;
             deac_status st = deac_batch_create(probs.data(), nseeds, &b, &msg);

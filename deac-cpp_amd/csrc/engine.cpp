@@ -35,6 +35,7 @@ struct deac_handle {
     bool overlap = false;               // walk beside the evolution (a CU is left free for it)
     std::vector<void *> allocations;
     double *kmat = nullptr, *nterm = nullptr, *fterm = nullptr, *tterm = nullptr, *obs = nullptr, *sig = nullptr;
+    double *nfm_dt = nullptr;           // -1st-moment weights over tau (bdsf), or null
     double *pop0 = nullptr, *pop1 = nullptr, *fitness = nullptr, *cr_old = nullptr, *f_old = nullptr;
     double *emod = nullptr, *mom = nullptr;
     int *cur = nullptr;
@@ -201,7 +202,7 @@ const char *deac_last_error_message(void) { return deac::last_message().c_str();
 size_t deac_output_length(const deac_problem *p)
 {
     if (!p) return 0;
-    return p->zero_temperature ? p->genome_size : 2 * p->genome_size - 1;
+    return (p->zero_temperature || p->spectra_type == DEAC_BDSF) ? p->genome_size : 2 * p->genome_size - 1;
 }
 
 void deac_seed_state(uint64_t seed, uint64_t state[4]) { deac::seed_state(seed, state); }
@@ -323,6 +324,7 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
     if (!m.nterm.empty()) A(h->nterm, Kc);
     if (!m.fterm.empty()) A(h->fterm, Kc);
     if (!m.tterm.empty()) A(h->tterm, Kc);
+    if (m.use_nfm) A(h->nfm_dt, T);
     if (st != DEAC_OK) return fail(st);
 
     // K rows [tau][stride], zero padded to whole tau blocks and chunks.
@@ -349,6 +351,7 @@ deac_status deac_create(const deac_problem *p, deac_handle **out, const char **m
     if (h->nterm) up(h->nterm, m.nterm.data(), m.Kc * sizeof(double));
     if (h->fterm) up(h->fterm, m.fterm.data(), m.Kc * sizeof(double));
     if (h->tterm) up(h->tterm, m.tterm.data(), m.Kc * sizeof(double));
+    if (h->nfm_dt) up(h->nfm_dt, m.nfm_dt.data(), m.T * sizeof(double));
     if (st != DEAC_OK) return fail(st);
     *out = h;
     return DEAC_OK;
@@ -381,8 +384,11 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         deac::last_message() = "rng_state must not be all zero (a fixed point of xoshiro256+)";
         return DEAC_ERR_INVALID_ARGUMENT;
     }
-    const bool split = o->ks_precision != DEAC_PRECISION_FP64 || o->split_loop != 0;
     const deac::Model &m = h->m;
+    // bdsf always runs the split loop: its trial contraction is the
+    // reference build's rounded-product fold (split_fold_kernel), not the
+    // fused kernel's exact FMA chains
+    const bool split = o->ks_precision != DEAC_PRECISION_FP64 || o->split_loop != 0 || m.bdsf;
     const size_t N = o->number_of_generations, P = m.P, Kc = m.Kc;
     const int nsign = m.nsign;
     hipStream_t s = h->stream;
@@ -429,6 +435,11 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
     ea.first_moment = m.first_moment;
     ea.third_moment = m.third_moment;
     ea.third_moment_error = m.third_moment_error;
+    ea.use_nfm = m.use_nfm;
+    ea.nfm_value = m.nfm_value;
+    ea.nfm_error = m.nfm_error;
+    ea.nfm_dt = h->nfm_dt;
+    ea.fold_ks = m.bdsf;
     ea.kmat = h->kmat;
     ea.nterm = h->nterm;
     ea.fterm = h->fterm;
@@ -604,7 +615,9 @@ deac_status deac_run(deac_handle *h, uint64_t rng_state[4], const deac_run_optio
         // model ISF (and moment rows) of the contraction's genomes: the
         // current ones for the initial fitness, the trials otherwise
         auto evaluate = [&](const deac::EvoArgs &a, int init) -> deac_status {
-            if (y.precision == DEAC_PRECISION_FP64) {
+            if (y.precision == DEAC_PRECISION_FP64 && a.fold_ks && !init) {
+                DEAC_HIP(deac::launch_split_fold(a, s));
+            } else if (y.precision == DEAC_PRECISION_FP64) {
                 double *B = static_cast<double *>(sp.B);
                 DEAC_HIP(deac::launch_split_pack_fp64(a, init, B, y.Nt, Kb, s));
                 DEAC_HIP(deac::launch_ks_fp64(static_cast<const double *>(sp.A), B, a.emod, a.T, a.P, y.Mt, y.Nt,

@@ -170,8 +170,8 @@ double squared_mean_of(const double *a, size_t n)
 deac_status validate(const deac_problem &p, std::string &msg)
 {
     const bool single_particle = p.spectra_type == DEAC_SPBSF || p.spectra_type == DEAC_SPFSF;
-    if (p.spectra_type < DEAC_SPBSF || p.spectra_type > DEAC_FFULL) {
-        msg = "Please choose spectra_type from the following options: spbsf, spfsf, bfull, ffull";
+    if (p.spectra_type < DEAC_SPBSF || p.spectra_type > DEAC_BDSF) {
+        msg = "Please choose spectra_type from the following options: spbsf, spfsf, bfull, ffull, bdsf";
         return DEAC_ERR_INVALID_ARGUMENT;
     }
     if (p.model < DEAC_MODEL_STANDARD || p.model > DEAC_MODEL_NORMALIZATION) {
@@ -254,7 +254,8 @@ Model build_model(const deac_problem &p)
     m.G = p.genome_size;
     m.P = p.population_size;
     m.zero_t = p.zero_temperature != 0;
-    m.nsign = m.zero_t ? 1 : 2;
+    m.bdsf = p.spectra_type == DEAC_BDSF;
+    m.nsign = (m.zero_t || m.bdsf) ? 1 : 2;
     m.Kc = static_cast<size_t>(m.nsign) * m.G;
     m.model = p.model;
     m.spectra = p.spectra_type;
@@ -295,6 +296,17 @@ Model build_model(const deac_problem &p)
                 kp[j] = df * std::exp(-t * w);
                 continue;
             }
+            if (m.bdsf) {  // detailed balance, deac.cpp:322, 330, 337, 362, 370, 378
+                if (m.model == DEAC_MODEL_HYPERBOLIC) {
+                    const double bo2mt = 0.5 * beta - t;
+                    kp[j] = df * std::cosh(bo2mt * w);
+                } else {
+                    const double bmt = beta - t;
+                    const double num = std::exp(-bmt * w) + std::exp(-t * w);
+                    kp[j] = m.model == DEAC_MODEL_NORMALIZATION ? df * (num / (1.0 + std::exp(-beta * w))) : df * num;
+                }
+                continue;
+            }
             if (m.model == DEAC_MODEL_HYPERBOLIC) {  // deac.cpp:323-324, 363-364
                 // -0.5*beta and 1.5*beta are rounded before the subtraction
                 // (loop invariants in the reference build, never fused).
@@ -328,6 +340,17 @@ Model build_model(const deac_problem &p)
             double a, b = 0.0;
             if (m.zero_t) {
                 a = which == 0 ? df : which == 1 ? df * w : df * std::pow(w, 3) * w;
+            } else if (m.bdsf) {  // deac.cpp:508-526, 645-661, 745-761
+                double c;  // the model's detailed-balance factor
+                if (m.model == DEAC_MODEL_HYPERBOLIC)
+                    c = which == 0 ? std::cosh(0.5 * beta * w) : std::sinh(0.5 * beta * w);
+                else if (m.model == DEAC_MODEL_NORMALIZATION)
+                    c = which == 0 ? 1.0 : std::tanh(0.5 * beta * w);
+                else
+                    c = which == 0 ? 1.0 + std::exp(-beta * w) : 1.0 - std::exp(-beta * w);
+                if (which == 0) a = m.model == DEAC_MODEL_NORMALIZATION ? df : df * c;
+                else if (which == 1) a = df * w * c;
+                else a = df * std::pow(w, 3) * w * c;
             } else if (m.model == DEAC_MODEL_HYPERBOLIC) {
                 const double e = std::exp(-0.5 * beta * w);
                 if (which == 0) { a = 0.5 * df / e; b = 0.5 * df * e; }
@@ -352,6 +375,25 @@ Model build_model(const deac_problem &p)
     if (m.normalize || m.renormalize) terms(0, m.nterm);
     if (m.use_first) terms(1, m.fterm);
     if (m.use_third) terms(3, m.tterm);
+
+    // The -1st-moment term of the detailed-balance build (deac.cpp:875-894;
+    // the zero-temperature build turns it off, deac.cpp:230-236): trapezoid
+    // weights over tau, the ISF's moment summed with fused products, and an
+    // error that keeps the last timeslice's term only (the reference assigns
+    // rather than accumulates it).
+    m.use_nfm = m.bdsf && !m.zero_t && p.use_negative_first_moment != 0;
+    if (m.use_nfm) {
+        m.nfm_dt.assign(m.T, 0.0);
+        double v = 0.0, e = 0.0;
+        for (size_t j = 0; j < m.T; j++) {
+            const double d = trapezoid_weight(m.tau, j);
+            m.nfm_dt[j] = d;
+            v = std::fma(m.isf[j], d, v);
+            e = (m.err[j] * m.err[j]) * (d * d);
+        }
+        m.nfm_value = v;
+        m.nfm_error = std::sqrt(e);
+    }
     return m;
 }
 
@@ -383,9 +425,12 @@ void output_transform(const Model &m, const double *xb, double *dsf, double *fre
     const double beta = m.beta;
     for (size_t i = 0; i < G; i++) {
         const double w = m.freq[i];
-        if (m.nsign == 1) {  // zero temperature: positive half only
+        if (m.nsign == 1) {  // zero temperature or bdsf: one sign (deac.cpp:1605-1671)
+            double v = xb[i];
+            if (!m.zero_t && m.model == DEAC_MODEL_HYPERBOLIC) v = 0.5 * xb[i] * std::exp(0.5 * beta * w);
+            else if (!m.zero_t && m.model == DEAC_MODEL_NORMALIZATION) v = xb[i] / (1.0 + std::exp(-beta * w));
             freq[i] = w;
-            dsf[i] = xb[i];
+            dsf[i] = v;
             continue;
         }
         const size_t ip = G + i - 1, in = G - i - 1;  // deac.cpp:1600-1608

@@ -47,11 +47,15 @@ void nibble_table(const Gf2Matrix &m, uint64_t *tab);
 // ---------------------------------------------------------------- model
 struct Model {
     size_t T = 0, G = 0, P = 0;
-    int nsign = 2;          // 2: positive and negative frequencies, 1: zero temperature
+    int nsign = 2;          // 2: positive and negative frequencies, 1: zero temperature or bdsf
     size_t Kc = 0;          // nsign * G, genome length across both signs
     int model = DEAC_MODEL_STANDARD, spectra = DEAC_SPFSF;
     bool zero_t = false, allow_negative = false, renormalize = false, normalize = false;
+    bool bdsf = false;      // detailed-balance spectra: one sign; the trial contraction is a rounded-product fold
     bool use_first = false, use_third = false;
+    bool use_nfm = false;   // -1st-moment term (bdsf above zero temperature)
+    double nfm_value = 0, nfm_error = 0;
+    std::vector<double> nfm_dt;   // T trapezoid weights over tau
     double temperature = 0, beta = 0, zeroth_moment = 0;
     double first_moment = -1, third_moment = -1, third_moment_error = 0;
     double cr0 = 0.9, tau_cr = 0.1, f0 = 0.9, tau_f = 0.1;

@@ -51,6 +51,10 @@ struct EvoArgs {
     int MG, TB;                  // member groups of kChainRows, tau blocks of kChainTau
     int use_first, use_third, renorm, allow_neg, init_mode, track_stats;
     double stop, zeroth, first_moment, third_moment, third_moment_error;
+    int use_nfm;                 // -1st-moment term of the fitness (bdsf): weights nfm_dt [T]
+    double nfm_value, nfm_error;
+    const double *nfm_dt;
+    int fold_ks;                 // the trials' model ISF as a rounded-product fold (bdsf; split loop only)
     const double *kmat;          // [TB*16][stride], zero padded
     const double *nterm, *fterm, *tterm;   // [Kc] or null
     const double *obs, *sig;     // [T]

@@ -69,7 +69,7 @@ double reduced_chi2(const deac::Model &m, const double *e)
 extern "C" size_t deac_genome_length(const deac_problem *p)
 {
     if (!p) return 0;
-    return p->zero_temperature ? p->genome_size : 2 * p->genome_size;
+    return (p->zero_temperature || p->spectra_type == DEAC_BDSF) ? p->genome_size : 2 * p->genome_size;
 }
 
 extern "C" deac_status deac_model_isf(const deac_problem *p, const double *genomes, size_t count, int precision,

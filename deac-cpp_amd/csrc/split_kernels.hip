@@ -11,6 +11,8 @@
 //   split_pack_*          trial rows into the contraction's operand layout
 //   ks_*_kernel           the contraction: FP64 exact chains straight into
 //                         the model ISF, or FP32 / BF16x3 genome slices
+//   split_fold_kernel     (bdsf at FP64, in place of the two above) the
+//                         reference build's rounded-product fold
 //   split_model_kernel    FP32 slices summed in slice order and scaled back
 //                         per tau row into the FP64 model ISF
 //   split_moment_kernel   first / third moment rows, when used
@@ -223,6 +225,20 @@ __global__ __launch_bounds__(kThreads) void split_model_kernel(EvoArgs a, const 
     }
 }
 
+// The trials' model ISF as the detailed-balance (bdsf) reference build sums
+// it (its loop inlines matrix_multiply_LxM_by_MxN, deac.cpp:110-118,
+// 1389-1390, as a fold of rounded products, the last of an odd count fused):
+// one thread per (member, tau), straight into emod.
+__global__ __launch_bounds__(kThreads) void split_fold_kernel(EvoArgs a)
+{
+    if (stopped(a)) return;
+    const size_t total = static_cast<size_t>(a.P) * a.T;
+    for (size_t idx = grid_thread(); idx < total; idx += grid_threads()) {
+        const int n = static_cast<int>(idx / a.T), t = static_cast<int>(idx % a.T);
+        a.emod[idx] = fold_row(chain_row(a, n, false), a.kmat + static_cast<size_t>(t) * a.stride, a.Kc, a.G);
+    }
+}
+
 // First and third moment rows of the contraction's genomes, one thread per
 // (member, moment) (deac.cpp:1420-1476).
 __global__ __launch_bounds__(kWave) void split_moment_kernel(EvoArgs a, int init)
@@ -269,6 +285,11 @@ __global__ __launch_bounds__(kWave) void split_select_kernel(EvoArgs a, int g, i
     }
     if (lane != 0) return;
     double fit = chi / static_cast<double>(a.T);
+    if (a.use_nfm) {  // deac.cpp:961-965, 1398-1416, 1543-1547
+        const double mi = fold_row(a.emod + static_cast<size_t>(i) * a.T, a.nfm_dt, a.T, a.T);
+        const double q = (a.nfm_value - mi) / a.nfm_error;
+        fit = fma(q, q, fit);
+    }
     if (a.use_first) {
         const double d = a.mom[i * 4 + 1] - a.first_moment;
         fit += (d * d) / a.first_moment;
@@ -357,6 +378,14 @@ hipError_t launch_split_model(const EvoArgs &a, const float *part, int slices, i
     const size_t total = static_cast<size_t>(a.P) * a.T;
     hipLaunchKernelGGL(split_model_kernel, dim3(grid_for(total, kThreads)), dim3(kThreads), 0, s, a, part, slices,
                        Npad, mn, ex);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_fold(const EvoArgs &a, hipStream_t s)
+{
+    const size_t total = static_cast<size_t>(a.P) * a.T;
+    const int blocks = static_cast<int>(std::min<size_t>((total + kThreads - 1) / kThreads, 4096));
+    hipLaunchKernelGGL(split_fold_kernel, dim3(blocks), dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 

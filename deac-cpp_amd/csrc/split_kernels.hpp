@@ -27,6 +27,8 @@ hipError_t launch_split_pack_bf16x3(const EvoArgs &a, int init, void *Bh, void *
 hipError_t launch_split_model(const EvoArgs &a, const float *part, int slices, int Npad, size_t mn, const int *ex,
                               hipStream_t s);
 // First / third moment rows (call only when a moment term is used).
+// bdsf: the trials' model ISF as a fold of rounded products (FP64).
+hipError_t launch_split_fold(const EvoArgs &a, hipStream_t s);
 hipError_t launch_split_moments(const EvoArgs &a, int init, hipStream_t s);
 // Fitness from a.emod and a.mom; greedy selection (init = 0) or the initial
 // fitness (init = 1).

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdeac_amd.so")
 
 MODELS = {"standard": 0, "hyperbolic": 1, "normalization": 2}
-SPECTRA = {"spbsf": 0, "spfsf": 1, "bfull": 2, "ffull": 3}
+SPECTRA = {"spbsf": 0, "spfsf": 1, "bfull": 2, "ffull": 3, "bdsf": 4}
 STATUS = {0: "success", 1: "invalid argument", 2: "no usable gfx950 device", 3: "out of memory",
           4: "device error", 5: "unsupported"}
 
@@ -53,6 +53,7 @@ class Problem(C.Structure):
         ("self_adapting_crossover_probability", C.c_double),
         ("differential_weight", C.c_double),
         ("self_adapting_differential_weight_probability", C.c_double),
+        ("use_negative_first_moment", C.c_int),
     ]
 
 
@@ -210,13 +211,16 @@ def make_problem(tau, isf, err, frequency, temperature, *, model="standard", spe
                  zero_temperature=False, allow_negative=False, renormalize=False, population_size=8,
                  normalize=False, first_moment=-1.0, third_moment=-1.0, third_moment_error=0.0,
                  crossover_probability=0.9, self_adapting_crossover_probability=0.1,
-                 differential_weight=0.9, self_adapting_differential_weight_probability=0.1):
+                 differential_weight=0.9, self_adapting_differential_weight_probability=0.1,
+                 use_negative_first_moment=False):
+    """spectra_type "bdsf" is the reference's bosonic detailed-balance build (one sign of genes, G-length
+    output); use_negative_first_moment adds its -1st-moment term above zero temperature."""
     keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (tau, isf, err, frequency)]
     p = Problem(len(keep[0]), _ptr(keep[0]), _ptr(keep[1]), _ptr(keep[2]), len(keep[3]), _ptr(keep[3]),
                 temperature, MODELS[model], SPECTRA[spectra_type], int(zero_temperature), int(allow_negative),
                 int(renormalize), population_size, int(normalize), first_moment, third_moment,
                 third_moment_error, crossover_probability, self_adapting_crossover_probability,
-                differential_weight, self_adapting_differential_weight_probability)
+                differential_weight, self_adapting_differential_weight_probability, int(use_negative_first_moment))
     p._keep = keep
     return p
 

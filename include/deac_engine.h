@@ -46,12 +46,22 @@ typedef enum deac_model {
     DEAC_MODEL_NORMALIZATION = 2
 } deac_model;
 
-/* Spectra type (reference: --spectra_type, deac.cpp:2040-2058). */
+/* Spectra type (reference: --spectra_type, deac.cpp:2040-2058).  DEAC_BDSF
+ * is the reference's bosonic detailed-balance build
+ * (USE_BOSONIC_DETAILED_BALANCE_CONDITION_DSF, src/deac/CMakeLists.txt:21,
+ * 33-34), whose only spectra type is "bdsf" (deac.cpp:2016-2018): one sign
+ * of genes, the detailed-balance kernel of the chosen model
+ * (deac.cpp:322-394), G-length output (deac.cpp:1596-1606), the optional
+ * -1st-moment term (deac.cpp:868-920), and the trial population's model ISF
+ * summed as that build sums it: rounded products added in order, the last
+ * of an odd count fused.  With zero_temperature it is the reference's ZeroT
+ * build. */
 typedef enum deac_spectra_type {
     DEAC_SPBSF = 0,
     DEAC_SPFSF = 1,
     DEAC_BFULL = 2,
-    DEAC_FFULL = 3
+    DEAC_FFULL = 3,
+    DEAC_BDSF = 4
 } deac_spectra_type;
 
 /* Everything deac() takes except the RNG, the generation count, the stop
@@ -78,6 +88,8 @@ typedef struct deac_problem {
     double self_adapting_crossover_probability;           /* -u, tau_CR */
     double differential_weight;                           /* -F, initial F */
     double self_adapting_differential_weight_probability; /* -v, tau_F */
+    int use_negative_first_moment;        /* --use_negative_first_moment: used with DEAC_BDSF above zero
+                                             temperature (deac.cpp:230-236, 868-920), ignored otherwise */
 } deac_problem;
 
 /* Precision of the K.S contraction of the model ISF (deac.cpp:1385-1395):
@@ -149,8 +161,8 @@ deac_status deac_run(deac_handle *handle, uint64_t rng_state[4],
 
 void deac_destroy(deac_handle *handle);
 
-/* Length of the dsf/frequency outputs: 2G-1, or G at zero temperature
- * (deac.cpp:1583-1591). */
+/* Length of the dsf/frequency outputs: 2G-1, or G at zero temperature and
+ * for DEAC_BDSF (deac.cpp:1583-1606). */
 size_t deac_output_length(const deac_problem *problem);
 
 /* xoshiro256+ state for a --seed value: seeded with 1407513600 + seed
@@ -283,7 +295,10 @@ deac_status deac_batch_last_profile(const deac_batch *batch, deac_run_profile *p
  * |dchi^2|/chi^2 of the mixed-precision forms against FP64.
  *
  * genomes: count rows of deac_genome_length(problem) doubles (positive
- * frequencies then negative, the population layout of deac.cpp:430-458).
+ * frequencies then negative, the population layout of deac.cpp:430-458;
+ * positive only at zero temperature and for DEAC_BDSF).  FP64 is the exact
+ * FMA chain, also for DEAC_BDSF (whose reference build sums its trials, not
+ * its initial population, as a fold).
  * Outputs (each may be NULL): model_isf, count x T; chi2, count reduced
  * chi^2 values (deac.cpp:120-126, 1540-1542; FP64 in the reference's order,
  * no moment terms); kernel_ms, device time of the contraction, from the

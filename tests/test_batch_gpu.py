@@ -24,7 +24,7 @@ def batch_compatible(case):
     # (one instance is a single workgroup here); config 4's long seeds run
     # inside a 256-seed batch below
     return (a["P"] <= 32 and a.get("first_moment", -1.0) < 0 and a.get("third_moment", -1.0) < 0
-            and m["variant"] != "standard_sp" and a["N"] <= 20000)
+            and m["variant"] != "standard_sp" and a["N"] <= 20000 and m["spectra"] != "bdsf")
 
 
 def _engine_problem(case):

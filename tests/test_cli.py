@@ -295,7 +295,33 @@ def test_cli_seed_batch_unbatchable_runs_seed_by_seed(tmp_path):
 
 
 @pytest.mark.gpu
-def test_cli_mixed_precision_seeds_run_seed_by_seed(tmp_path):
-    # the batch kernel computes K.S in fp64: fp32 seeds go through deac_run
-    # one after another, each exactly its own single run
-    _seed_batch_vs_single_runs(tmp_path, ["--normalize", "--ks_precision", "fp32"], [5, 6], expect_batched=False)
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_cli_mixed_precision_seed_batch(tmp_path, prec):
+    # fp32 / bf16x3 seeds run in the batch kernel (delta form): each seed's
+    # files are that seed's instance of deac_batch_run at the same precision,
+    # bit for bit, and its reported fitness is the FP64 chi^2 of its genome
+    import deac_amd as da
+    fixture = tmp_path / "tiny-isf.bin"
+    write_fixture(fixture)
+    seeds = [5, 6, 7]
+    cmd = deac_command(tmp_path, fixture, N="300", seed=str(seeds[0]),
+                       extra=["--normalize", "--ks_precision", prec, "--track_stats",
+                              "--number_of_seeds", str(len(seeds))])
+    r = run(cmd, tmp_path)
+    assert r.returncode == 0, r.stdout
+    assert "seeds run one after another" not in r.stdout, r.stdout
+    tau, isf, err = np.fromfile(fixture).reshape(3, -1)
+    p = da.make_problem(tau, isf, err, da.frequency_grid(8, 4.0), 1.0, population_size=8, normalize=True,
+                        spectra_type="spfsf")
+    with da.Batch([p] * len(seeds)) as b:
+        outs = b.run(seeds, 300, 1.0, track_stats=True, precision=prec)
+    res = tmp_path / "results"
+    for s, o in zip(seeds, outs):
+        n = o["generation"] + 1
+        assert np.array_equal(np.fromfile(res / f"deac-spfsf_dsf_{s}.bin").view(np.uint64), o["dsf"].view(np.uint64))
+        for kind, key in (("mean", "mean"), ("minimum", "minimum"), ("squared-mean", "squared_mean")):
+            got = np.fromfile(res / f"deac-spfsf_stats_fitness-{kind}_{s}.bin")
+            assert np.array_equal(got.view(np.uint64), o[key][:n].view(np.uint64)), (s, kind)
+        log = dict(l.split(": ", 1) for l in (res / f"deac-spfsf_log_{s}.dat").read_text().splitlines() if ": " in l)
+        assert int(log["generation"]) == o["generation"]
+        assert abs(float(log["minimum_fitness"]) - o["minimum_fitness"]) <= 1e-5 * o["minimum_fitness"]
