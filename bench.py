@@ -375,6 +375,8 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if HEADLINE not in args.batch_precisions.split(","):
+        ap.error(f"--batch-precisions must include {HEADLINE}, the headline's K.S precision")
     if args.gpus != 1:
         print("bench.py measures one GPU: instances are independent (across GPUs they would shard by "
               "instance, with no communication per generation) and nothing here spans devices; "
@@ -436,11 +438,9 @@ def main():
     evals_per_launch = head["evals"] / max(head["evolve_launches"], 1)
     kern_s = head["kernel_ms_per_launch"] / 1e3
     per_gen = traffic_per_generation(HEADLINE_PROFILE)
-    gens_per_launch = args.steps
+    gens_per_launch = evals_per_launch / (nb * Pb)  # the timed segment is sliced into launches
     traffic = per_gen * gens_per_launch if per_gen else None
-    fp64b = batch.get("fp64", head)
-    fp64_kern_s = fp64b["kernel_ms_per_launch"] / 1e3
-    fp64_evals = fp64b["evals"] / max(fp64b["evolve_launches"], 1)
+    fp64b = batch.get("fp64")
 
     # ---- plain FP64 single instance
     plain_value = plain32_value = float("nan")
@@ -502,7 +502,7 @@ def main():
                    "batched_over_plain_batch_shape": value / plain32_value,
                    "batching_vs_precision": {"fp64_batch_over_plain_batch_shape": fp64b["value"] / plain32_value,
                                              **{f"{p}_over_fp64_batch": batch[p]["value"] / fp64b["value"]
-                                                for p in batch if p != "fp64"}},
+                                                for p in batch if p != "fp64"}} if fp64b else None,
                    "batch_precision_accuracy": accuracy,
                    "uneven_batch": uneven,
                    "precision_sweep": sweep,
@@ -521,11 +521,11 @@ def main():
                      "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (calibrated "
                                      f"against a plain copy) in {HEADLINE_PROFILE}, per generation x the launch's "
                                      "generations"},
-        "roofline_fp64_batch": {"bound": "mfma", "kernel": "batch_kernel<0,1,2,1>",
-                                "achieved": fp64_evals * flop_per_eval / fp64_kern_s / 1e12,
-                                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                                "frac": fp64_evals * flop_per_eval / fp64_kern_s / 1e12 / FP64_PEAK_TFLOPS},
     }
+    if fp64b:  # the exact FP64 batch kernel against FP64 MFMA, when it was timed
+        fp64_tf = fp64b["evals"] / max(fp64b["evolve_launches"], 1) * flop_per_eval / (fp64b["kernel_ms_per_launch"] / 1e3) / 1e12
+        line["roofline_fp64_batch"] = {"bound": "mfma", "kernel": "batch_kernel<0,1,2,1>", "achieved": fp64_tf,
+                                       "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": fp64_tf / FP64_PEAK_TFLOPS}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_gens)
     print(json.dumps(line), flush=True)
