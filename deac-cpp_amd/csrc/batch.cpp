@@ -43,6 +43,7 @@ struct BatchModel {
     int Ppad = 0, NM = 0, NT = 0, C = 0, nchunk = 0, Lpad = 0, Lw = 0, ntt = 0, nrem = 0, S = 0, Lsub = 0;
     int ntm = 0, NQ = 0;                   // mixed K.S: kernel tiles (rows past T zero), pairs per chain wave
     double *ktile = nullptr, *krem = nullptr, *obs = nullptr, *sig = nullptr;
+    double *mterm = nullptr;               // [2][Lpad] first- and third-moment weights, when used
     uint64_t *walk_rows = nullptr, *sub_tab = nullptr;
     void *kmix[3] = {nullptr, nullptr, nullptr};  // mixed K tiles by precision, built on first use
     double *kscale = nullptr;                     // 2^ex[t] of the mixed K rows
@@ -62,8 +63,8 @@ std::string problem_key(const deac_problem &p)
     put(p.frequency, G * sizeof(double));
     const deac_problem *q = &p;
     put(&q->temperature, sizeof q->temperature);
-    const int ints[6] = {p.model, p.spectra_type, p.zero_temperature, p.allow_negative_spectral_weight,
-                         p.renormalize_each_generation, p.normalize};
+    const int ints[7] = {p.model, p.spectra_type, p.zero_temperature, p.allow_negative_spectral_weight,
+                         p.renormalize_each_generation, p.normalize, p.use_negative_first_moment};
     put(ints, sizeof ints);
     put(&p.population_size, sizeof p.population_size);
     const double dbl[7] = {p.first_moment, p.third_moment, p.third_moment_error, p.crossover_probability,
@@ -152,10 +153,6 @@ deac_status plan_model(BatchModel &b, std::string &why)
     const deac::Model &m = b.m;
     if (m.P > static_cast<size_t>(deac::kBatchMaxP)) {
         why = "batched runs take at most 32 members per population";
-        return DEAC_ERR_UNSUPPORTED;
-    }
-    if (m.use_first || m.use_third) {
-        why = "batched runs do not take moment constraints";
         return DEAC_ERR_UNSUPPORTED;
     }
     if (m.normalize && m.renormalize) {
@@ -311,6 +308,16 @@ deac_status upload_model(deac_batch *h, BatchModel &b)
     A(b.sig, m.T);
     A(b.walk_rows, rows.size());
     A(b.sub_tab, tabs.size());
+    // moment weights (deac.cpp:628-668, 728-768), zero past L
+    std::vector<double> mt;
+    if (m.use_first || m.use_third) {
+        mt.assign(2 * static_cast<size_t>(b.Lpad), 0.0);
+        for (size_t k = 0; k < m.Kc; k++) {
+            if (m.use_first) mt[k] = m.fterm[k];
+            if (m.use_third) mt[b.Lpad + k] = m.tterm[k];
+        }
+        A(b.mterm, mt.size());
+    }
     if (st != DEAC_OK) return st;
     auto up = [&](void *dst, const void *src, size_t bytes) {
         if (st == DEAC_OK && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) st = DEAC_ERR_DEVICE;
@@ -321,6 +328,7 @@ deac_status upload_model(deac_batch *h, BatchModel &b)
     up(b.sig, m.err.data(), m.T * sizeof(double));
     up(b.walk_rows, rows.data(), rows.size() * sizeof(uint64_t));
     up(b.sub_tab, tabs.data(), tabs.size() * sizeof(uint64_t));
+    if (b.mterm) up(b.mterm, mt.data(), mt.size() * sizeof(double));
     return st;
 }
 
@@ -509,6 +517,13 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
             deac::last_message() = "instance " + std::to_string(i) + ": no mixed K.S kernel for this shape";
             return DEAC_ERR_UNSUPPORTED;
         }
+        if (prec != DEAC_PRECISION_FP64 && h->models[h->model_of[i]]->mterm) {
+            // the moments fold the FP64 trials, which the mixed kernel keeps
+            // only as their FP32 / BF16 change from the parent
+            deac::last_message() = "instance " + std::to_string(i) + ": mixed K.S in the batch does not take moment "
+                              "constraints (run FP64, or the instance through deac_run)";
+            return DEAC_ERR_UNSUPPORTED;
+        }
     }
     BATCH_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
@@ -677,6 +692,12 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.kscale = b.kscale;
                 d.obs = b.obs;
                 d.sig = b.sig;
+                d.mterm = b.mterm;
+                d.use_first = m.use_first ? 1 : 0;
+                d.use_third = m.use_third ? 1 : 0;
+                d.first_moment = m.first_moment;
+                d.third_moment = m.third_moment;
+                d.third_moment_error = m.third_moment_error;
                 d.walk_rows = b.walk_rows;
                 d.sub_tab = b.sub_tab;
                 d.pop = h->pop + h->pop_off[i];

@@ -113,6 +113,7 @@ struct BatchShared {
     double cr_trial[2][kBatchMaxP][2];
     alignas(16) double f_trial[2][kBatchMaxP][2];   // read as pairs
     double y0t[kBatchMaxP], y0r[kBatchMaxP];  // trial value at position 0 (zero-frequency match)
+    double mom[kBatchMaxP][2];                // first and third moments of the pass's genomes (FP64 kernel)
     int accepted[3][kBatchMaxP];
     int flag;                                 // 0 evolve, 1 stop test fired, 2 end of the launch
     // Crossover words of the chunks in flight, [chunk % 4][generation: this,
@@ -1133,14 +1134,47 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         __syncthreads();
     } else {
         // ------------------------------------------------ walk wave
+        // It also folds the moment rows of the pass's genomes (FP64 kernel,
+        // moment constraints): lane m reads member m's values of chunk c - 4
+        // from the trial buffer in the step the chains contract it, and adds
+        // the products in the reference's order (deac.cpp:102-108 as the
+        // reference build inlines it, deac.cpp:1420-1477): rounded products
+        // one at a time, the last position of each sign fused when G is odd.
         const int items = 2 * d.P;
         const int per = wgen >= 0 ? (items + nsteps - 1) / nsteps : 0;
         uint64_t ws[4] = {sh.walk_state[0], sh.walk_state[1], sh.walk_state[2], sh.walk_state[3]};
         int next = 0;
+        const bool moments = PREC == kPrecFp64 && chains && (d.use_first || d.use_third) && lane < d.P;
+        const double *const mt = uniform_ptr(d.mterm);
+        const int nL = __builtin_amdgcn_readfirstlane(d.L), nG = __builtin_amdgcn_readfirstlane(d.G);
+        const int nLpad = __builtin_amdgcn_readfirstlane(d.Lpad);
+        const bool godd = (nG & 1) != 0;
+        double m1 = 0.0, m3 = 0.0;
         for (int c = 0; c < nsteps; c++) {
+            const int cm = c - 4;
+            if (moments && cm >= 0 && cm < nchunk) {
+                const double *tb = lds + kOffTrial + (cm & 1) * kCD;
+                const int k0 = cm * C, n = min(C, nL - k0);
+                for (int sl = 0; sl < n; sl++) {
+                    const int k = k0 + sl;
+                    const double v = tb[trial_index<NM>(lane, sl)];
+                    const double a = mt[k], b = mt[nLpad + k];
+                    if (godd && (k == nG - 1 || k == nL - 1)) {
+                        m1 = fma(v, a, m1);
+                        m3 = fma(v, b, m3);
+                    } else {
+                        m1 = m1 + v * a;
+                        m3 = m3 + v * b;
+                    }
+                }
+            }
             const int stop = min(items, (c + 1) * per);
             for (; next < stop; next++) walk_item(d, sh, ws, wgen, next, lane);
             step_barrier();
+        }
+        if (moments) {
+            sh.mom[lane][0] = m1;
+            sh.mom[lane][1] = m3;
         }
         if (lane == 0)
             for (int w = 0; w < 4; w++) sh.walk_state[w] = ws[w];
@@ -1260,7 +1294,15 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
                         const double v = lds[t * Ppad + lane];
                         chi = t == tlast ? fma(v, v, chi) : chi + v;
                     }
-                    const double fit = chi / static_cast<double>(d.T);
+                    double fit = chi / static_cast<double>(d.T);
+                    if (d.use_first) {  // deac.cpp:969, 1551
+                        const double dm = sh.mom[lane][0] - d.first_moment;
+                        fit += (dm * dm) / d.first_moment;
+                    }
+                    if (d.use_third) {  // deac.cpp:972, 1554
+                        const double q = (d.third_moment - sh.mom[lane][1]) / d.third_moment_error;
+                        fit = fma(q, q, fit);
+                    }
                     if (kind == kPassInit) {
                         sh.fitness[lane] = fit;
                     } else if (fit <= sh.fitness[lane]) {

@@ -31,6 +31,7 @@ struct BatchDesc {
                                // bfloat16 (BF16X3), rows scaled by 2^-ex[t], zero past T and L
     const double *kscale;      // mixed K.S: [T] 2^ex[t]
     const double *obs, *sig;   // [T] ISF and its error
+    const double *mterm;       // [2][Lpad] first- and third-moment weights (zero past L), or null (FP64 kernel)
     const uint64_t *walk_rows; // [256][4] rows of T^L, T the xoshiro transition
     const uint64_t *sub_tab;   // [S-1][64][16][4] nibble tables of T^(k*Lsub), k = 1..S-1
     // state
@@ -46,11 +47,13 @@ struct BatchDesc {
     long stats_len, stats_base;  // entry for generation g: stats[g - stats_base] (and + stats_len, + 2 stats_len)
     long gen0, gen_end;        // evolve generations gen0 .. gen_end-1 unless the stop test fires
     double stop, tau_cr, tau_f;
+    double first_moment, third_moment, third_moment_error;  // used when use_first / use_third
     uint64_t mod_magic;        // floor((2^64-1)/P)
     int init;                  // evaluate the initial population first (gen0 == 0; FP64 kernel only)
     int mode;                  // 0: evolve; 1: evaluation only (FP64 kernel, init set): fitness and epar of the
                                // stored population, nothing else written
     int T, G, nsign, L, Lpad, Lw, nchunk, ntt, nrem, ntm, P, S, Lsub, allow_neg;
+    int use_first, use_third;  // moment constraints (deac.cpp:1420-1477, 1550-1555), FP64 kernel only
 };
 
 // Exact FP64 kernel.  NT: 16-row kernel tiles per chain wavefront (1, 2 or 4); NM: 16-member

@@ -429,17 +429,19 @@ int main(int argc, char **argv)
                 std::vector<deac_run_options> opts(nseeds, ro);
                 st = deac_batch_run(b, states.data(), opts.data(), res.data());
                 deac_batch_destroy(b);
-                if (st != DEAC_OK) {
+                if (st == DEAC_ERR_UNSUPPORTED) {  // refused before any launch (e.g. mixed K.S with moments)
+                    msg = deac_last_error_message();
+                } else if (st != DEAC_OK) {
                     std::cerr << "deac engine: " << deac_status_string(st) << std::endl;
                     std::exit(1);
+                } else {
+                    done = true;
                 }
-                done = true;
             } else if (st != DEAC_ERR_UNSUPPORTED) {
                 std::cerr << "deac engine: " << deac_status_string(st) << (msg ? std::string(": ") + msg : "") << std::endl;
                 std::exit(1);
-            } else {
-                std::cout << "seeds run one after another (" << (msg ? msg : "not batchable") << ")" << std::endl;
             }
+            if (!done) std::cout << "seeds run one after another (" << (msg ? msg : "not batchable") << ")" << std::endl;
         }
         if (!done) {
             deac_handle *h = nullptr;

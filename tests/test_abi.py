@@ -108,6 +108,13 @@ def test_zero_temperature_skips_temperature_check_and_halves_output():
     assert da.lib().deac_output_length(C.byref(p2)) == 15
 
 
+def test_bdsf_has_one_sign():
+    # the detailed-balance build: G genes and G output values (deac.cpp:1596-1606)
+    p = _tiny(spectra_type="bdsf", isf=np.array([1, .85, .72, .61]), use_negative_first_moment=True)
+    assert da.lib().deac_output_length(C.byref(p)) == 8
+    assert da.lib().deac_genome_length(C.byref(p)) == 8
+
+
 def test_status_strings():
     for code, text in da.STATUS.items():
         assert da.lib().deac_status_string(code).decode() == text
@@ -121,7 +128,7 @@ def _batch_problem(**kw):
 
 @pytest.mark.parametrize("kw,status,text", [
     (dict(population_size=40), 5, "at most 32 members"),
-    (dict(first_moment=1.0), 5, "moment constraints"),
+    (dict(spectra_type="bdsf"), 5, "detailed-balance spectra (bdsf)"),
     (dict(normalize=True, renormalize=True), 5, "renormalize each generation"),
     (dict(population_size=2), 1, "population_size"),
 ])

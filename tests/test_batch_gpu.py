@@ -23,8 +23,7 @@ def batch_compatible(case):
     # the complete 200000-generation config-2 run is the plain engine's case
     # (one instance is a single workgroup here); config 4's long seeds run
     # inside a 256-seed batch below
-    return (a["P"] <= 32 and a.get("first_moment", -1.0) < 0 and a.get("third_moment", -1.0) < 0
-            and m["variant"] != "standard_sp" and a["N"] <= 20000 and m["spectra"] != "bdsf")
+    return (a["P"] <= 32 and m["variant"] != "standard_sp" and a["N"] <= 20000 and m["spectra"] != "bdsf")
 
 
 def _engine_problem(case):
@@ -346,3 +345,26 @@ def test_tuned_schedule_changes_no_result():
     with da.Batch(probs) as b2:  # the process keeps the tuned schedule for these shapes
         assert b2.schedule["group_streams"] == chosen["group_streams"]
         assert b2.schedule["group_order"] == chosen["group_order"]
+
+
+def test_batch_moments_match_oracle():
+    # moment constraints in the batch kernel (the walk wave folds each
+    # member's trial values chunk by chunk in the reference's order) at the
+    # seed batch's T with an odd genome (the fused last position of each
+    # sign, a partial last chunk), two signs with P = 32 and one sign
+    # (zero temperature) with P = 12, in one batch, against the oracle
+    print("[gpu] batch with moment constraints vs oracle", flush=True)
+    t, f, s = np.fromfile(f"{gc.GOLDEN_DIR}/isf_c2_T65.bin").reshape(3, -1)
+    G, N = 1023, 120
+    kw = dict(normalize=True, first_moment=30.0, third_moment=1e9, third_moment_error=1e8)
+    shapes = [dict(P=32, zero_temperature=False), dict(P=12, zero_temperature=True)]
+    probs = [da.make_problem(t, f, s, da.frequency_grid(G, 512.0), 1.35, population_size=x["P"],
+                             zero_temperature=x["zero_temperature"], **kw) for x in shapes]
+    with da.Batch(probs) as b:
+        outs = b.run([3, 4], N, 0.0, True)
+    for sd, x, o in zip((3, 4), shapes, outs):
+        r = ob.run(ob.make_problem(t, f, s, ob.frequency_grid(G, 512.0), 1.35, population_size=x["P"],
+                                   zero_temperature=x["zero_temperature"], number_of_generations=N,
+                                   stop_minimum_fitness=0.0, **kw), sd)
+        assert not _compare(o, r), (x, _compare(o, r))
+        assert o["rng"] == r["rng"]

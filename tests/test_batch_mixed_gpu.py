@@ -126,3 +126,16 @@ def test_batch_refuses_the_split_loop():
         states = (C.c_uint64 * 4)(*da.seed_state(1))
         st = da.lib().deac_batch_run(b.handle, states, opts, res)
         assert st == 5 and "split loop" in da._last_message()
+
+
+def test_mixed_batch_refuses_moment_constraints():
+    # the moments fold FP64 trials, which the mixed kernel keeps only as their
+    # change from the parent: refused before any launch, FP64 runs them
+    print("[gpu] mixed batch with moment constraints refused", flush=True)
+    t, f, s = np.fromfile(f"{gc.GOLDEN_DIR}/isf_c1_T33.bin").reshape(3, -1)
+    p = da.make_problem(t, f, s, da.frequency_grid(256, 64.0), 1.35, population_size=8, normalize=True,
+                        first_moment=3.0)
+    with da.Batch([p]) as b:
+        with pytest.raises(da.DeacError, match="moment"):
+            b.run([1], 20, 0.0, precision="fp32")
+        assert b.run([1], 20, 0.0)[0]["generation"] == 19

@@ -289,9 +289,25 @@ def test_cli_seed_batch_matches_single_runs(tmp_path):
 
 
 @pytest.mark.gpu
+def test_cli_seed_batch_with_moments_matches_single_runs(tmp_path):
+    # moment constraints in the batch kernel (FP64): each seed exactly its own run
+    _seed_batch_vs_single_runs(tmp_path, ["--normalize", "--first_moment", "0.5", "--third_moment", "2.0",
+                                          "--third_moment_error", "0.5"], [3, 4], expect_batched=True)
+
+
+@pytest.mark.gpu
 def test_cli_seed_batch_unbatchable_runs_seed_by_seed(tmp_path):
-    # moment constraints run through the single-instance engine
-    _seed_batch_vs_single_runs(tmp_path, ["--first_moment", "0.5"], [3, 4], expect_batched=False)
+    # per-generation renormalization runs through the single-instance engine
+    _seed_batch_vs_single_runs(tmp_path, ["--normalize", "--single_particle_fermionic_spectral_function"], [3, 4],
+                               expect_batched=False)
+
+
+@pytest.mark.gpu
+def test_cli_mixed_precision_seeds_with_moments_run_seed_by_seed(tmp_path):
+    # the mixed batch refuses moment constraints at run time; deac.e falls
+    # back to single runs, each exactly its own --seed run
+    _seed_batch_vs_single_runs(tmp_path, ["--ks_precision", "fp32", "--first_moment", "0.5"], [5, 6],
+                               expect_batched=False)
 
 
 @pytest.mark.gpu
