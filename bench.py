@@ -167,10 +167,24 @@ def batch_precision_accuracy(da, prob, gens):
             g = np.concatenate([dsf[G - 1:], dsf[G - 1::-1]])
             _, chi, _ = da.model_isf(prob, g, "fp64")
             exact.append(abs(chi[0] - o[k]["minimum_fitness"]) / chi[0])
+        # DSF L2 error per seed, and of the seed-averaged DSF against the
+        # FP64 batch's average (which is the plain path's: bit-identical per
+        # seed), where the spectrum has weight (0 <= omega <= 100: the far
+        # negative-frequency genes do not enter chi^2 and drift without bound)
+        sel = (ref[0]["frequency"] >= 0.0) & (ref[0]["frequency"] <= 100.0)
+        rel = lambda a, r: float(np.linalg.norm(a[sel] - r[sel]) / np.linalg.norm(r[sel]))
+        l2 = np.array([rel(a["dsf"], r["dsf"]) for a, r in zip(o, ref)])
+        avg, avgr = np.mean([a["dsf"] for a in o], axis=0), np.mean([r["dsf"] for r in ref], axis=0)
+        half = len(ref) // 2
+        halves = rel(np.mean([r["dsf"] for r in ref[:half]], axis=0), np.mean([r["dsf"] for r in ref[half:]], axis=0))
         out[prec] = {"first_divergent_generation_median": float(np.median(div)) if div else None,
                      "seeds_not_diverged": sum(1 for x in first if x is None),
                      "final_chi2_median": float(np.median(fin)), "final_chi2_median_fp64": float(np.median(finr)),
                      "final_chi2_rel_diff_median": float(np.median(np.abs(fin - finr) / finr)),
+                     "final_chi2_rel_diff_max": float(np.max(np.abs(fin - finr) / finr)),
+                     "dsf_l2_rel_w0_100_median": float(np.median(l2)), "dsf_l2_rel_w0_100_max": float(np.max(l2)),
+                     "seed_average_dsf_l2_rel_w0_100": rel(avg, avgr),
+                     "fp64_seed_halves_l2_rel_w0_100": halves,
                      "reported_vs_fp64_reevaluated_rel_max": float(max(exact)),
                      "generation1_best_rel_diff_max": float(max(abs(a["minimum"][1] - r["minimum"][1]) /
                                                                 r["minimum"][1] for a, r in zip(o, ref)))}
