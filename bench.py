@@ -233,6 +233,32 @@ def uneven_batch(da, gens):
     return out
 
 
+def small_population_batch(da, gens):
+    """Seeds with small populations: 512 seeds of the G=4096, T=65 workload at
+    P=8, exact FP64, for `gens` timed generations (after an 8-generation
+    warm-up run), one instance per workgroup and with the automatic stacking
+    (two instances in the 16 member rows of each workgroup)."""
+    t, f, s = load_isf()
+    n, P = 512, 8
+    prob = da.make_problem(t, f, s, da.frequency_grid(G, OMEGA_MAX), TEMPERATURE, population_size=P, normalize=True,
+                           spectra_type="bfull")
+    seeds = list(range(1, n + 1))
+    out = {"workload": f"{n} seeds x (G=4096 per sign, T=65, P=8, bfull, --normalize), FP64, {gens} generations"}
+    with da.Batch([prob] * n) as b:
+        for name, pack in (("one_per_workgroup", 1), ("stacked", 0)):
+            sc = b.schedule
+            sc["pack"] = pack
+            b.schedule = sc
+            b.run(seeds, 8, 0.0, track_stats=False)
+            b.run(seeds, gens + 1, 0.0, track_stats=False)
+            prof = b.profile
+            out[name] = {"value": P * prof["generations"] / (prof["loop_ms"] / 1e3), "unit": "genome-evals/s",
+                         "workgroups_per_launch": prof["evolve_workgroups"] / max(prof["evolve_launches"], 1),
+                         "ms_per_generation": prof["loop_ms"] / gens}
+    out["stacked_over_one_per_workgroup"] = out["stacked"]["value"] / out["one_per_workgroup"]["value"]
+    return out
+
+
 def role_cycles(da, args):
     """Per-wavefront cycles of the batch kernels on the seed batch, one JSON
     line per precision: the first `--warmup` generations of a run, then
@@ -377,6 +403,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--uneven-gens", type=int, default=200, help="timed generations of the uneven batch (0: skip)")
     ap.add_argument("--no-precision-sweep", action="store_true")
+    ap.add_argument("--small-pop-gens", type=int, default=300,
+                    help="timed generations of the 512-seed P=8 batch, stacked and not (0: skip)")
     ap.add_argument("--accuracy-gens", type=int, default=2000,
                     help="generations of the seed batch's FP64/FP32/BF16x3 accuracy comparison (0: skip)")
     ap.add_argument("--trajectory-gens", type=int, default=2000,
@@ -477,6 +505,7 @@ def main():
         plain32_value = Pb * qprof["generations"] / (qprof["loop_ms"] / 1e3)
     accuracy = batch_precision_accuracy(da, prob, args.accuracy_gens) if args.accuracy_gens > 0 else None
     uneven = uneven_batch(da, args.uneven_gens) if args.uneven_gens > 0 else None
+    small = small_population_batch(da, args.small_pop_gens) if args.small_pop_gens > 0 else None
     sweep = None if args.no_precision_sweep else precision_sweep(da)
     trajectories = precision_trajectories(da, args.trajectory_gens) if args.trajectory_gens > 0 else None
 
@@ -519,6 +548,7 @@ def main():
                                                 for p in batch if p != "fp64"}} if fp64b else None,
                    "batch_precision_accuracy": accuracy,
                    "uneven_batch": uneven,
+                   "small_population_batch": small,
                    "precision_sweep": sweep,
                    "precision_trajectories": trajectories},
         "roofline": {"bound": "hbm", "achieved": evals_per_launch * bytes_per_eval / kern_s / 1e9,

@@ -81,6 +81,7 @@ std::map<std::string, deac_batch_schedule> g_sched_cache;  // tuned schedules by
 
 struct deac_batch {
     int device = 0;
+    int cu_count = 256;                        // compute units of the device
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, kev0 = nullptr, kev1 = nullptr, kev2 = nullptr, kev3 = nullptr;
     hipEvent_t fork = nullptr;
@@ -103,7 +104,7 @@ struct deac_batch {
     double *scratch = nullptr;                 // this device's statistics scratch (deac_batch_run_noalloc)
     std::vector<long> reached;                 // per instance: generation reached by the last run (absolute)
     std::vector<char> epar_valid;              // per instance: epar holds the current population's model ISF
-    deac_batch_schedule sched{3, 2, 0, 0.0};   // launch schedule (deac_batch_tune)
+    deac_batch_schedule sched{3, 2, 0, 0.0, 0};  // launch schedule (deac_batch_tune)
     std::string signature;                     // shapes of the instances, the process-wide schedule key
     deac_run_profile last{};
 
@@ -419,6 +420,7 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
     hipDeviceProp_t prop;
     BATCH_HIP(hipGetDeviceProperties(&prop, h->device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DEAC_ERR_NO_DEVICE;
+    h->cu_count = prop.multiProcessorCount;
     BATCH_HIP(deac::batch_scratch(h->device, &h->scratch));  // once per device, this device current
     BATCH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     for (hipEvent_t *e : {&h->ev0, &h->ev1, &h->kev0, &h->kev1, &h->kev2, &h->kev3}) BATCH_HIP(hipEventCreate(e));
@@ -626,11 +628,48 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
             return deac::launch_batch(dd, cnt, std::get<1>(k), std::get<2>(k), std::get<3>(k), st);
         return deac::launch_batch_mixed(dd, cnt, std::get<0>(k), std::get<1>(k), std::get<2>(k), st);
     };
+    // Instances of instance i's problem that may share its workgroup (FP64;
+    // populations that leave room in the 16 member rows of their kernel
+    // shape), and whether adding i keeps a workgroup's populations within
+    // the kernel's 32-bit byte offsets.
+    // Automatic packing: a workgroup's time per generation grows ~6% per
+    // stacked instance (its random-stream walk) while every workgroup
+    // streams the whole kernel matrix from L2 each generation, so stacking
+    // pays once the workgroups would fill most of the device: as many as
+    // bring a launch round to one workgroup per CU, one more while that
+    // leaves more than 3/4 of the CUs busy (measured at G = 4096, T = 65,
+    // P = 4, 5, 8 and 32 to 1024 instances).
+    const size_t n_cu = static_cast<size_t>(std::max(h->cu_count, 1));
+    auto pack_limit = [&](size_t i, size_t n_active) {
+        const BatchModel &b = *h->models[h->model_of[i]];
+        if (options[i].ks_precision != DEAC_PRECISION_FP64 || h->sched.pack == 1 || b.Ppad != 16) return 1;
+        int q = std::min<int>(deac::kBatchPackMax, 16 / static_cast<int>(b.m.P));
+        if (h->sched.pack > 1) q = std::min(q, h->sched.pack);
+        else {
+            size_t a = (n_active + n_cu - 1) / n_cu;
+            if ((n_active + a - 1) / a * 4 > 3 * n_cu) a++;
+            q = std::min<int>(q, static_cast<int>(a));
+        }
+        return std::max(q, 1);
+    };
+    auto pack_span_ok = [&](const std::vector<size_t> &pk, size_t i) {
+        const size_t rows = h->models[h->model_of[i]]->m.P * static_cast<size_t>(h->models[h->model_of[i]]->Lpad);
+        size_t lo = h->pop_off[i], hi = h->pop_off[i] + rows;
+        for (size_t j : pk) {
+            lo = std::min(lo, h->pop_off[j]);
+            hi = std::max(hi, h->pop_off[j] + rows);
+        }
+        return (hi - lo) * sizeof(double) <= UINT32_MAX;
+    };
     for (;;) {
         // Group the active instances by kernel instantiation.
         std::map<Key, std::vector<size_t>> groups;
+        size_t n_active = 0;
         for (size_t i = 0; i < n; i++)
-            if (active[i]) groups[key_of(i, false)].push_back(i);
+            if (active[i]) {
+                groups[key_of(i, false)].push_back(i);
+                n_active++;
+            }
         if (groups.empty()) break;
         // Order within a group (workgroups are dispatched in index order):
         // longest instances first -- per-generation cost ~ chunks x tiles
@@ -675,12 +714,32 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
         }
         size_t slot = 0;
         std::vector<std::pair<Key, std::pair<size_t, size_t>>> launches, evals_before, evals_after;
-        std::vector<size_t> order;
+        std::vector<size_t> order;                       // lead instance of each slot
+        std::vector<std::pair<size_t, size_t>> placed;   // (instance, slot) of this round's evolution launches
         std::vector<size_t> mixed_slots;  // evolution slots of mixed instances
         for (auto &[key, mp] : glist) {
             const std::vector<size_t> &members = *mp;
-            const size_t begin = slot;
+            // Workgroups: FP64 instances of one problem with small populations
+            // at the same generation share one, up to the member rows their
+            // kernel shape holds anyway; the rest run one per workgroup.
+            std::vector<std::vector<size_t>> packs;
+            std::map<std::pair<int, long>, size_t> open;
             for (size_t i : members) {
+                const int q = pack_limit(i, n_active);
+                if (q > 1) {
+                    auto it = open.find({h->model_of[i], next_gen[i]});
+                    if (it != open.end() && packs[it->second].size() < static_cast<size_t>(q) &&
+                        pack_span_ok(packs[it->second], i)) {
+                        packs[it->second].push_back(i);
+                        continue;
+                    }
+                    open[{h->model_of[i], next_gen[i]}] = packs.size();
+                }
+                packs.push_back({i});
+            }
+            const size_t begin = slot;
+            for (const std::vector<size_t> &pk : packs) {
+                const size_t i = pk[0];
                 const BatchModel &b = *h->models[h->model_of[i]];
                 const deac::Model &m = b.m;
                 const int prec = options[i].ks_precision;
@@ -700,23 +759,35 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.third_moment_error = m.third_moment_error;
                 d.walk_rows = b.walk_rows;
                 d.sub_tab = b.sub_tab;
-                d.pop = h->pop + h->pop_off[i];
-                d.fitness = h->fitness + i * deac::kBatchMaxP;
-                d.cr_old = h->cr_old + i * deac::kBatchMaxP * 2;
-                d.f_old = h->f_old + i * deac::kBatchMaxP * 2;
+                size_t pop_lo = h->pop_off[i];
+                for (size_t j : pk) pop_lo = std::min(pop_lo, h->pop_off[j]);
+                d.pop = h->pop + pop_lo;
                 d.masks = h->masks + h->mask_off[i];
-                d.rng = h->rng + 4 * i;
                 d.epar = prec == DEAC_PRECISION_FP64 ? nullptr : h->epar + h->epar_off[i];
-                d.stats = stats[i];
-                d.status = h->status + 2 * i;
-                d.stats_len = static_cast<long>(options[i].number_of_generations);
-                d.stats_base = base[i];
                 d.gen0 = next_gen[i];
                 const long len = prec == DEAC_PRECISION_FP64 ? slice : std::min(slice, refresh);
                 d.gen_end = std::min<long>(next_gen[i] + len, n_abs[i] - 1);
-                d.stop = options[i].stop_minimum_fitness;
+                d.Q = static_cast<int>(pk.size());
+                for (int q = 0; q < d.Q; q++) {
+                    const size_t j = pk[q];
+                    d.pop_row0[q] = static_cast<uint32_t>((h->pop_off[j] - pop_lo) * sizeof(double));
+                    d.fitness[q] = h->fitness + j * deac::kBatchMaxP;
+                    d.cr_old[q] = h->cr_old + j * deac::kBatchMaxP * 2;
+                    d.f_old[q] = h->f_old + j * deac::kBatchMaxP * 2;
+                    d.rng[q] = h->rng + 4 * j;
+                    d.stats[q] = stats[j];
+                    d.status[q] = h->status + 2 * j;
+                    d.stats_len[q] = static_cast<long>(options[j].number_of_generations);
+                    d.stats_base[q] = base[j];
+                    d.stop[q] = options[j].stop_minimum_fitness;
+                    // the launch ends with its shortest instance; the others go on in the next
+                    d.gen_end = std::min<long>(d.gen_end, n_abs[j] - 1);
+                    placed.push_back({j, slot});
+                }
                 d.tau_cr = m.tau_cr;
                 d.tau_f = m.tau_f;
+                d.Pi = static_cast<int>(m.P);
+                d.P = d.Q * d.Pi;
                 d.mod_magic = ~UINT64_C(0) / m.P;
                 d.init = (first && !resume && prec == DEAC_PRECISION_FP64) ? 1 : 0;
                 d.mode = 0;
@@ -730,7 +801,6 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.ntt = b.ntt;
                 d.nrem = b.nrem;
                 d.ntm = b.ntm;
-                d.P = static_cast<int>(m.P);
                 d.S = b.S;
                 d.Lsub = b.Lsub;
                 d.allow_neg = m.allow_negative ? 1 : 0;
@@ -799,10 +869,10 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
         BATCH_HIP(hipEventElapsedTime(&ms_after, h->kev2, h->kev3));
         prof.evolve_ms += ms_evolve;
         prof.evolve_launches += launches.size();
+        for (auto &[key, range] : launches) prof.evolve_workgroups += range.second;
         prof.eval_ms += ms_before + ms_after;
         prof.eval_launches += evals_before.size() + evals_after.size();
-        for (size_t k = 0; k < order.size(); k++) {
-            const size_t i = order[k];
+        for (auto [i, k] : placed) {
             const long reached = st[2 * i + 1];
             // the kernel ends an instance inside its slice: anything else is
             // a device-side failure, not a result
@@ -936,7 +1006,8 @@ deac_status deac_batch_role_cycles(int precision, unsigned long long *cycles, in
 deac_status deac_batch_set_schedule(deac_batch *h, const deac_batch_schedule *sc)
 {
     if (!h || !sc || sc->group_streams < 1 || sc->group_streams > deac_batch::kGroupStreams || sc->group_order < 0 ||
-        sc->group_order > 2 || sc->instance_order < 0 || sc->instance_order > 1)
+        sc->group_order > 2 || sc->instance_order < 0 || sc->instance_order > 1 || sc->pack < 0 ||
+        sc->pack > deac::kBatchPackMax)
         return DEAC_ERR_INVALID_ARGUMENT;
     h->sched = *sc;
     return DEAC_OK;
@@ -982,7 +1053,8 @@ deac_status deac_batch_tune(deac_batch *h, const uint64_t *rng_states, const dea
     std::vector<deac_batch_schedule> cand;
     for (int st = 1; st <= (several_groups ? deac_batch::kGroupStreams : 1); st++)
         for (int go = 0; go <= (several_groups ? 2 : 0); go++)
-            for (int io = 0; io <= (several_models ? 1 : 0); io++) cand.push_back(deac_batch_schedule{st, go, io, 0.0});
+            for (int io = 0; io <= (several_models ? 1 : 0); io++)
+                cand.push_back(deac_batch_schedule{st, go, io, 0.0, h->sched.pack});
     const deac_batch_schedule keep = h->sched;
     std::vector<uint64_t> states(rng_states, rng_states + 4 * n);
     // a warm-up run with the current schedule, then two timed continuations

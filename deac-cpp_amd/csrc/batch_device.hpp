@@ -104,8 +104,8 @@ struct WalkRec {
 struct BatchShared {
     uint64_t rows[256 * 4];                  // T^L by rows
     WalkRec rec[4][kBatchMaxP];              // ring over generations
-    uint64_t gen_start[4][4];                // state at the top of each generation
-    uint64_t walk_state[4];                  // walk position between passes
+    uint64_t gen_start[4][kBatchPackMax][4]; // each stream's state at the top of each generation
+    uint64_t walk_state[kBatchPackMax][4];   // each stream's walk position between passes
     double obs[kMaxT], sig[kMaxT];           // ISF and its errors
     double kscale[kMaxT];                    // 2^ex[t]: K row t is stored scaled by 2^-ex[t] (mixed K.S)
     double fitness[kBatchMaxP];
@@ -116,6 +116,7 @@ struct BatchShared {
     double mom[kBatchMaxP][2];                // first and third moments of the pass's genomes (FP64 kernel)
     int accepted[3][kBatchMaxP];
     int flag;                                 // 0 evolve, 1 stop test fired, 2 end of the launch
+    int qstop[kBatchPackMax];                 // instances whose stop test fired
     // Crossover words of the chunks in flight, [chunk % 4][generation: this,
     // previous][member][candidate][C/32], staged from global memory.
     uint32_t mring[4][kMaskWords];
@@ -271,13 +272,16 @@ __device__ __forceinline__ void nibble_jump(const gcu64 *tab, uint64_t (&s)[4])
 // jDE redraws of member i (CR+, F+, CR-, F-: deac.cpp:1180-1213), items
 // P..2P-1 the donor indices of member i (deac.cpp:200-216) followed by a skip
 // of its nsign*G crossover draws.  Every lane carries the same state.
-__device__ void walk_item(const BatchDesc &d, BatchShared &sh, uint64_t (&s)[4], long gen, int item, int lane)
+__device__ void walk_item(const BatchDesc &d, BatchShared &sh, uint64_t (&s)[4], long gen, int q, int item, int lane)
 {
+    // item j of instance q's stream: j < Pi the jDE draws of member j, then
+    // member j - Pi's donors and mask-stream skip; members are numbered in
+    // the workgroup (q Pi + local), donors drawn within the instance
     const int slot = static_cast<int>(gen & 3);
-    const int P = d.P;
+    const int Pi = d.Pi, base = q * Pi;
     if (item == 0 && lane == 0)
-        for (int w = 0; w < 4; w++) sh.gen_start[slot][w] = s[w];
-    if (item < P) {
+        for (int w = 0; w < 4; w++) sh.gen_start[slot][q][w] = s[w];
+    if (item < Pi) {
         uint32_t flags = 0;
         double crv[2] = {0.0, 0.0}, fv[2] = {0.0, 0.0};
         for (int sg = 0; sg < d.nsign; sg++) {
@@ -291,7 +295,7 @@ __device__ void walk_item(const BatchDesc &d, BatchShared &sh, uint64_t (&s)[4],
             }
         }
         if (lane == 0) {
-            WalkRec &r = sh.rec[slot][item];
+            WalkRec &r = sh.rec[slot][base + item];
             r.flags = flags;
             r.cr_value[0] = crv[0];
             r.cr_value[1] = crv[1];
@@ -299,16 +303,16 @@ __device__ void walk_item(const BatchDesc &d, BatchShared &sh, uint64_t (&s)[4],
             r.f_value[1] = fv[1];
         }
     } else {
-        const uint32_t i0 = static_cast<uint32_t>(item - P), Pu = static_cast<uint32_t>(P);
+        const uint32_t i0 = static_cast<uint32_t>(item - Pi), Pu = static_cast<uint32_t>(Pi);
         uint32_t m0 = i0, m1 = i0, m2 = i0;
         while (m0 == i0) m0 = mod_p(xnext(s), d.mod_magic, Pu);
         while (m1 == i0 || m1 == m0) m1 = mod_p(xnext(s), d.mod_magic, Pu);
         while (m2 == i0 || m2 == m0 || m2 == m1) m2 = mod_p(xnext(s), d.mod_magic, Pu);
         if (lane == 0) {
-            WalkRec &r = sh.rec[slot][i0];
-            r.mutant[0] = m0;
-            r.mutant[1] = m1;
-            r.mutant[2] = m2;
+            WalkRec &r = sh.rec[slot][base + i0];
+            r.mutant[0] = base + m0;
+            r.mutant[1] = base + m1;
+            r.mutant[2] = base + m2;
             for (int w = 0; w < 4; w++) r.mask_state[w] = s[w];
         }
         // s <- T^L s: lane l forms output bits l, 64+l, 128+l, 192+l as
@@ -509,7 +513,7 @@ struct Stager {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int own = (wave + 4 * r) * 128 + 2 * lane;
-            poff[r] = static_cast<uint32_t>(min(own / C, nP - 1) * nLpad + own % C) * 8u;
+            poff[r] = d.pop_row0[0] + static_cast<uint32_t>(min(own / C, nP - 1) * nLpad + own % C) * 8u;
         }
         const long pg = kind == kPassEvolve ? g - 1 : g;
         const int gslot = static_cast<int>((g + 3) % 3), pslot = static_cast<int>((pg + 3) % 3);
@@ -881,7 +885,8 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
 #pragma unroll
         for (int u = 0; u < kT; u++) {
             const int m = mrow(u), cm = fm(u);
-            poff[u] = static_cast<uint32_t>(cm * nLpad + p) * 8u;
+            const int cq = cm / d.Pi;  // instance of the row's member, its row in that instance
+            poff[u] = d.pop_row0[cq] + static_cast<uint32_t>((cm - cq * d.Pi) * nLpad + p) * 8u;
             od[u][0] = od[u][1] = od[u][2] = m * C + p;
             odb[u][0] = odb[u][1] = odb[u][2] = m * C + p;
             mo[u] = mob[u] = 0;
@@ -1140,9 +1145,12 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         // the products in the reference's order (deac.cpp:102-108 as the
         // reference build inlines it, deac.cpp:1420-1477): rounded products
         // one at a time, the last position of each sign fused when G is odd.
-        const int items = 2 * d.P;
+        // The streams of the workgroup's instances one after another: items
+        // 2 Pi q .. 2 Pi (q + 1) - 1 are instance q's.
+        const int items = 2 * d.P, item_q = 2 * d.Pi;
         const int per = wgen >= 0 ? (items + nsteps - 1) / nsteps : 0;
-        uint64_t ws[4] = {sh.walk_state[0], sh.walk_state[1], sh.walk_state[2], sh.walk_state[3]};
+        int cq = 0;
+        uint64_t ws[4] = {sh.walk_state[0][0], sh.walk_state[0][1], sh.walk_state[0][2], sh.walk_state[0][3]};
         int next = 0;
         const bool moments = PREC == kPrecFp64 && chains && (d.use_first || d.use_third) && lane < d.P;
         const double *const mt = uniform_ptr(d.mterm);
@@ -1169,7 +1177,16 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                 }
             }
             const int stop = min(items, (c + 1) * per);
-            for (; next < stop; next++) walk_item(d, sh, ws, wgen, next, lane);
+            for (; next < stop; next++) {
+                const int q = next / item_q;
+                if (q != cq) {  // the next instance's stream
+                    if (lane == 0)
+                        for (int w = 0; w < 4; w++) sh.walk_state[cq][w] = ws[w];
+                    for (int w = 0; w < 4; w++) ws[w] = sh.walk_state[q][w];
+                    cq = q;
+                }
+                walk_item(d, sh, ws, wgen, q, next - q * item_q, lane);
+            }
             step_barrier();
         }
         if (moments) {
@@ -1177,7 +1194,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             sh.mom[lane][1] = m3;
         }
         if (lane == 0)
-            for (int w = 0; w < 4; w++) sh.walk_state[w] = ws[w];
+            for (int w = 0; w < 4; w++) sh.walk_state[cq][w] = ws[w];
         __syncthreads();
     }
     __syncthreads();
@@ -1210,12 +1227,13 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
     for (int k = threadIdx.x; k < 2 * kCD; k += kBatchThreads) lds[kOffTrial + k] = 0.0;  // padded members stay 0
     if (threadIdx.x < kBatchMaxP) {
         const int m = threadIdx.x;
-        const gcdouble *fit = (const gcdouble *)d.fitness, *cr = (const gcdouble *)d.cr_old,
-                       *fw = (const gcdouble *)d.f_old;
-        sh.fitness[m] = m < P ? fit[m] : 0.0;
+        const int q = m < P ? m / d.Pi : 0, i = m - q * d.Pi;  // instance, its member
+        const gcdouble *fit = (const gcdouble *)d.fitness[q], *cr = (const gcdouble *)d.cr_old[q],
+                       *fw = (const gcdouble *)d.f_old[q];
+        sh.fitness[m] = m < P ? fit[i] : 0.0;
         for (int sg = 0; sg < 2; sg++) {
-            sh.cr_old[m][sg] = m < P ? cr[m * 2 + sg] : 0.0;
-            sh.f_old[m][sg] = m < P ? fw[m * 2 + sg] : 0.0;
+            sh.cr_old[m][sg] = m < P ? cr[i * 2 + sg] : 0.0;
+            sh.f_old[m][sg] = m < P ? fw[i * 2 + sg] : 0.0;
         }
         for (int k = 0; k < 3; k++) sh.accepted[k][m] = 0;
     }
@@ -1228,11 +1246,13 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
     // Prologue: the walk of the first generation whose masks are drawn.
     const long g0 = d.gen0;
     if (wave == kWalkWave && !evaluate) {
-        const gcu64 *rng = (const gcu64 *)d.rng;
-        uint64_t ws[4] = {rng[0], rng[1], rng[2], rng[3]};
-        for (int item = 0; item < 2 * P; item++) walk_item(d, sh, ws, g0, item, lane);
-        if (lane == 0)
-            for (int w = 0; w < 4; w++) sh.walk_state[w] = ws[w];
+        for (int q = 0; q < d.Q; q++) {
+            const gcu64 *rng = (const gcu64 *)d.rng[q];
+            uint64_t ws[4] = {rng[0], rng[1], rng[2], rng[3]};
+            for (int item = 0; item < 2 * d.Pi; item++) walk_item(d, sh, ws, g0, q, item, lane);
+            if (lane == 0)
+                for (int w = 0; w < 4; w++) sh.walk_state[q][w] = ws[w];
+        }
     }
     __syncthreads();
 
@@ -1323,7 +1343,10 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
             __syncthreads();
             if (evaluate) {
                 // evaluation launch: the fitness of the stored population
-                if (threadIdx.x < P) ((gdouble *)d.fitness)[threadIdx.x] = sh.fitness[threadIdx.x];
+                if (threadIdx.x < P) {
+                    const int q = threadIdx.x / d.Pi;
+                    ((gdouble *)d.fitness[q])[threadIdx.x - q * d.Pi] = sh.fitness[threadIdx.x];
+                }
                 return;
             }
             if constexpr (PREC != kPrecFp64) {
@@ -1339,32 +1362,35 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         // Top of generation g: best fitness, statistics, stop test
         // (deac.cpp:1145-1166); this generation's CR and F (deac.cpp:1180-1213).
         if (wave == kWorker0) {
-            if (lane == 0) {
-                int flag = 0;
-                if (g >= d.gen_end) {
-                    flag = 2;
-                } else {
-                    double mn = sh.fitness[0];
-                    for (int i = 0; i < P; i++)
-                        if (sh.fitness[i] < mn) mn = sh.fitness[i];
-                    if (d.stats) {
-                        gdouble *st = (gdouble *)d.stats;
-                        double sum = 0.0, sq = 0.0;
-                        const int p4 = P & ~3;
-                        for (int i = 0; i < P; i++) {
-                            const double v = sh.fitness[i];
-                            sum += v;
-                            sq = i < p4 ? sq + v * v : fma(v, v, sq);
-                        }
-                        const long gi = g - d.stats_base;
-                        st[gi] = sum / static_cast<double>(P);
-                        st[d.stats_len + gi] = mn;
-                        st[2 * d.stats_len + gi] = sq / static_cast<double>(P);
+            // lane q: instance q; the workgroup stops when any instance's
+            // stop test fires (the others resume from generation g)
+            const int Pi = d.Pi;
+            bool stop = false;
+            if (lane < d.Q && g < d.gen_end) {
+                const int q = lane;
+                const double *f = sh.fitness + q * Pi;
+                double mn = f[0];
+                for (int i = 0; i < Pi; i++)
+                    if (f[i] < mn) mn = f[i];
+                if (d.stats[q]) {
+                    gdouble *st = (gdouble *)d.stats[q];
+                    double sum = 0.0, sq = 0.0;
+                    const int p4 = Pi & ~3;
+                    for (int i = 0; i < Pi; i++) {
+                        const double v = f[i];
+                        sum += v;
+                        sq = i < p4 ? sq + v * v : fma(v, v, sq);
                     }
-                    if (mn <= d.stop) flag = 1;
+                    const long gi = g - d.stats_base[q], len = d.stats_len[q];
+                    st[gi] = sum / static_cast<double>(Pi);
+                    st[len + gi] = mn;
+                    st[2 * len + gi] = sq / static_cast<double>(Pi);
                 }
-                sh.flag = flag;
+                stop = mn <= d.stop[q];
+                sh.qstop[q] = stop ? 1 : 0;
             }
+            const bool any = __ballot(stop) != 0;
+            if (lane == 0) sh.flag = g >= d.gen_end ? 2 : any ? 1 : 0;
             if (lane < P) {
                 const WalkRec &r = sh.rec[g & 3][lane];
                 for (int sg = 0; sg < 2; sg++) {
@@ -1383,12 +1409,12 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
     }
 
     if (threadIdx.x < P) {
-        const int m = threadIdx.x;
-        gdouble *fit = (gdouble *)d.fitness, *cr = (gdouble *)d.cr_old, *fw = (gdouble *)d.f_old;
-        fit[m] = sh.fitness[m];
+        const int m = threadIdx.x, q = m / d.Pi, i = m - q * d.Pi;
+        gdouble *fit = (gdouble *)d.fitness[q], *cr = (gdouble *)d.cr_old[q], *fw = (gdouble *)d.f_old[q];
+        fit[i] = sh.fitness[m];
         for (int sg = 0; sg < 2; sg++) {
-            cr[m * 2 + sg] = sh.cr_old[m][sg];
-            fw[m * 2 + sg] = sh.f_old[m][sg];
+            cr[i * 2 + sg] = sh.cr_old[m][sg];
+            fw[i * 2 + sg] = sh.f_old[m][sg];
         }
     }
 #ifdef DEAC_ROLE_TIMERS
@@ -1399,11 +1425,12 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         atomicAdd(&g_role[threadIdx.x >> 6][2], static_cast<unsigned long long>(g - g0));
     }
 #endif
-    if (threadIdx.x == 0) {
-        __attribute__((address_space(1))) uint64_t *rng = (__attribute__((address_space(1))) uint64_t *)d.rng;
-        __attribute__((address_space(1))) long *status = (__attribute__((address_space(1))) long *)d.status;
-        for (int w = 0; w < 4; w++) rng[w] = sh.gen_start[g & 3][w];
-        status[0] = sh.flag == 1 ? 1 : 0;
+    if (threadIdx.x < d.Q) {
+        const int q = threadIdx.x;
+        __attribute__((address_space(1))) uint64_t *rng = (__attribute__((address_space(1))) uint64_t *)d.rng[q];
+        __attribute__((address_space(1))) long *status = (__attribute__((address_space(1))) long *)d.status[q];
+        for (int w = 0; w < 4; w++) rng[w] = sh.gen_start[g & 3][q][w];
+        status[0] = sh.flag == 1 && sh.qstop[q] ? 1 : 0;
         status[1] = g;
     }
 }

@@ -19,6 +19,7 @@ constexpr int kBatchChunkDoubles = 2048;    // members (padded) x genome positio
 constexpr int kBatchMaxP = 32;
 constexpr int kBatchRegionDoubles = 6 * kBatchChunkDoubles;  // 4 population + 2 trial buffers
 constexpr int kBatchMaxRem = 4;             // kernel rows past the last full tile run on 4x4x4 MFMAs
+constexpr int kBatchPackMax = 4;            // instances of one model stacked in one workgroup (FP64)
 
 // Everything one workgroup needs about its reconstruction.  Model arrays may
 // be shared between instances; state arrays are the instance's own.
@@ -34,25 +35,32 @@ struct BatchDesc {
     const double *mterm;       // [2][Lpad] first- and third-moment weights (zero past L), or null (FP64 kernel)
     const uint64_t *walk_rows; // [256][4] rows of T^L, T the xoshiro transition
     const uint64_t *sub_tab;   // [S-1][64][16][4] nibble tables of T^(k*Lsub), k = 1..S-1
-    // state
-    double *pop;               // [P][Lpad] genomes, positive sign then negative sign
-    double *fitness;           // [P]
-    double *cr_old, *f_old;    // [P][2] self-adapted CR and F per sign
+    // state of Q instances of one model stacked along the members: member m
+    // of the workgroup is member m % Pi of instance m / Pi (Q = 1 unless the
+    // host packs small populations, FP64 kernel only)
+    double *pop;               // genomes, [Pi][Lpad] per instance (positive sign then negative sign);
+                               // instance q's row 0 lies pop_row0[q] bytes past pop
+    uint32_t pop_row0[kBatchPackMax];
+    double *fitness[kBatchPackMax];                          // [Pi] per instance
+    double *cr_old[kBatchPackMax], *f_old[kBatchPackMax];    // [Pi][2] self-adapted CR and F per sign
     uint32_t *masks;           // [3][Ppad][2][Lw] crossover bits, two candidates per generation
-    uint64_t *rng;             // [4] in: state at the top of gen0; out: at the first generation not evolved
+    uint64_t *rng[kBatchPackMax];  // [4] in: state at the top of gen0; out: at the first generation not evolved
     double *epar;              // [T][Ppad] model ISF of each member's genome, FP64 (mixed K.S; written by
                                // evaluation launches), or null
-    double *stats;             // [3][stats_len] mean, minimum, squared mean per generation, or null
-    long *status;              // [0] stopped, [1] generation reached
-    long stats_len, stats_base;  // entry for generation g: stats[g - stats_base] (and + stats_len, + 2 stats_len)
+    double *stats[kBatchPackMax];  // [3][stats_len] mean, minimum, squared mean per generation, or null
+    long *status[kBatchPackMax];   // [0] stopped, [1] generation reached
+    long stats_len[kBatchPackMax], stats_base[kBatchPackMax];  // entry for generation g: stats[g - stats_base]
     long gen0, gen_end;        // evolve generations gen0 .. gen_end-1 unless the stop test fires
-    double stop, tau_cr, tau_f;
+    double stop[kBatchPackMax];    // stop_minimum_fitness per instance
+    double tau_cr, tau_f;
     double first_moment, third_moment, third_moment_error;  // used when use_first / use_third
-    uint64_t mod_magic;        // floor((2^64-1)/P)
+    uint64_t mod_magic;        // floor((2^64-1)/Pi)
     int init;                  // evaluate the initial population first (gen0 == 0; FP64 kernel only)
     int mode;                  // 0: evolve; 1: evaluation only (FP64 kernel, init set): fitness and epar of the
                                // stored population, nothing else written
-    int T, G, nsign, L, Lpad, Lw, nchunk, ntt, nrem, ntm, P, S, Lsub, allow_neg;
+    int T, G, nsign, L, Lpad, Lw, nchunk, ntt, nrem, ntm, S, Lsub, allow_neg;
+    int P;                     // members in the workgroup, Q Pi
+    int Q, Pi;                 // instances and members per instance
     int use_first, use_third;  // moment constraints (deac.cpp:1420-1477, 1550-1555), FP64 kernel only
 };
 

@@ -79,12 +79,13 @@ class Result(C.Structure):
 class RunProfile(C.Structure):
     _fields_ = [("loop_ms", C.c_double), ("walk_ms", C.c_double), ("mask_ms", C.c_double),
                 ("evolve_ms", C.c_double), ("generations", C.c_size_t), ("evolve_launches", C.c_size_t),
-                ("eval_ms", C.c_double), ("eval_launches", C.c_size_t), ("graph_launches", C.c_size_t)]
+                ("eval_ms", C.c_double), ("eval_launches", C.c_size_t), ("graph_launches", C.c_size_t),
+                ("evolve_workgroups", C.c_size_t)]
 
 
 class Schedule(C.Structure):
     _fields_ = [("group_streams", C.c_int), ("group_order", C.c_int), ("instance_order", C.c_int),
-                ("ms_per_generation", C.c_double)]
+                ("ms_per_generation", C.c_double), ("pack", C.c_int)]
 
 
 class DeacError(RuntimeError):
@@ -363,7 +364,8 @@ class Batch:
 
     @schedule.setter
     def schedule(self, d):
-        sc = Schedule(d["group_streams"], d["group_order"], d["instance_order"], d.get("ms_per_generation", 0.0))
+        sc = Schedule(d["group_streams"], d["group_order"], d["instance_order"], d.get("ms_per_generation", 0.0),
+                      d.get("pack", 0))
         st = lib().deac_batch_set_schedule(self.handle, C.byref(sc))
         if st:
             raise DeacError(st)

@@ -196,6 +196,8 @@ typedef struct deac_run_profile {
     double eval_ms;              /* batch, mixed K.S: exact FP64 re-evaluation launches */
     size_t eval_launches;
     size_t graph_launches;       /* deac_run's split loop: launches of captured HIP graphs */
+    size_t evolve_workgroups;    /* batch: workgroups of the evolution launches, summed (instances of one
+                                    problem with small populations may share one) */
 } deac_run_profile;
 deac_status deac_last_run_profile(const deac_handle *handle, deac_run_profile *profile);
 
@@ -246,6 +248,10 @@ typedef struct deac_batch_schedule {
     int group_order;           /* 0: most expensive group first, 1: least expensive first, 2: by kernel shape */
     int instance_order;        /* 0: longest instances first, 1: round robin over distinct problems */
     double ms_per_generation;  /* deac_batch_tune: the chosen schedule's time per generation of the batch */
+    int pack;                  /* FP64 instances of one problem (populations of at most 8) stacked in one
+                                  workgroup, up to 4 and as many as fit its 16 member rows: 0 automatic
+                                  (by the active instances per compute unit: none while they fill under
+                                  3/4 of the device), 1 never, 2..4 at most this many */
 } deac_batch_schedule;
 
 /* Times every candidate schedule on `generations` generations of this batch
