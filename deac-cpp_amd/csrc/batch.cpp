@@ -153,7 +153,7 @@ deac_status plan_model(BatchModel &b, std::string &why)
 {
     const deac::Model &m = b.m;
     if (m.P > static_cast<size_t>(deac::kBatchMaxP)) {
-        why = "batched runs take at most 32 members per population";
+        why = "batched runs take at most 64 members per population";
         return DEAC_ERR_UNSUPPORTED;
     }
     if (m.normalize && m.renormalize) {
@@ -167,7 +167,7 @@ deac_status plan_model(BatchModel &b, std::string &why)
         return DEAC_ERR_UNSUPPORTED;
     }
     const int T = static_cast<int>(m.T), L = static_cast<int>(m.Kc);
-    b.Ppad = m.P <= 16 ? 16 : 32;
+    b.Ppad = m.P <= 16 ? 16 : m.P <= 32 ? 32 : 64;
     b.NM = b.Ppad / 16;
     b.C = deac::kBatchChunkDoubles / b.Ppad;
     b.nchunk = (L + b.C - 1) / b.C;
@@ -191,6 +191,11 @@ deac_status plan_model(BatchModel &b, std::string &why)
     b.NT = per_wave <= 1 ? 1 : per_wave <= 2 ? 2 : per_wave <= 4 ? 4 : 0;
     if (b.NT == 0 || T * b.Ppad > deac::kBatchRegionDoubles) {
         why = "batched runs take at most 260 timeslices";
+        return DEAC_ERR_UNSUPPORTED;
+    }
+    if (b.NM == 4 && b.NT > 2) {
+        // 4 member tiles x NT kernel tiles of accumulators per chain wave
+        why = "batched runs with more than 32 members take at most 132 timeslices";
         return DEAC_ERR_UNSUPPORTED;
     }
     b.S = deac::kBatchMaskLanes / b.Ppad;

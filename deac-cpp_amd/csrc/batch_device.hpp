@@ -77,7 +77,7 @@ constexpr int kMaxT = 4 * 4 * 16 + kBatchMaxRem;  // timeslices: 16 full tiles (
 
 enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3 };
 enum { kPrecFp64 = 0, kPrecFp32 = 1, kPrecBf16 = 2 };  // DEAC_PRECISION_*
-constexpr int kEk = (kMaxT * kBatchMaxP + kBatchThreads - 1) / kBatchThreads;  // chi^2 terms per thread
+constexpr int kEk = (kMaxT * kBatchMixedMaxP + kBatchThreads - 1) / kBatchThreads;  // chi^2 terms per thread (mixed)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -818,7 +818,9 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             };
             if (!REM || wave >= NM) chain_loop(std::integral_constant<int, 0>{}, std::false_type{});
             else if (rmt == 0) chain_loop(std::integral_constant<int, 0>{}, std::true_type{});
-            else chain_loop(std::integral_constant<int, NM - 1>{}, std::true_type{});
+            else if (NM < 4 || rmt == NM - 1) chain_loop(std::integral_constant<int, NM - 1>{}, std::true_type{});
+            else if (rmt == 1) chain_loop(std::integral_constant<int, NM == 4 ? 1 : 0>{}, std::true_type{});
+            else chain_loop(std::integral_constant<int, NM == 4 ? 2 : 0>{}, std::true_type{});
             __syncthreads();
             // E[t][m] into LDS (the chunk buffers are free now): lane l holds
             // rows 4v + l/16 of the tile, column l%16.
@@ -893,7 +895,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             if (kind == kPassEvolve) {
                 const WalkRec &r = sh.rec[g & 3][cm];
 #pragma unroll
-                for (int k = 0; k < 3; k++) od[u][k] = static_cast<int>(r.mutant[k] & 31) * C + p;
+                for (int k = 0; k < 3; k++) od[u][k] = static_cast<int>(r.mutant[k] & (kBatchMaxP - 1)) * C + p;
                 mo[u] = (cm * 2 + (sh.accepted[aslot][cm] & 1)) * kWpc + (p >> 5);
             }
             // The walk record of pg is read only for accepted members: the
@@ -903,7 +905,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             if (acc) {
                 const WalkRec &r = sh.rec[pg & 3][cm];
 #pragma unroll
-                for (int k = 0; k < 3; k++) odb[u][k] = static_cast<int>(r.mutant[k] & 31) * C + p;
+                for (int k = 0; k < 3; k++) odb[u][k] = static_cast<int>(r.mutant[k] & (kBatchMaxP - 1)) * C + p;
                 mob[u] = Ppad * 2 * kWpc + (cm * 2 + (sh.accepted[pprev][cm] & 1)) * kWpc + (p >> 5);
             }
             bm[u] = __ballot(acc);
@@ -1071,7 +1073,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                 if constexpr (PREC == kPrecFp64) {
 #pragma unroll
                     for (int u = 0; u < kT; u++) {
-                        const int t0 = NM == 2 ? td0 + 64 * u : td0 ^ (8 * u);  // position p
+                        const int t0 = NM >= 2 ? td0 + 32 * NM * u : td0 ^ (8 * u);  // position p
                         tb[t0] = v[u].x;
                         tb[t0 + 16] = v[u].y;                                     // position p + 1
                     }

@@ -58,7 +58,9 @@ hipError_t launch_nt(const BatchDesc *descs, int count, int nt, hipStream_t s)
 {
     if (nt == 1) return launch_one<1, NM, REM>(descs, count, s);
     if (nt == 2) return launch_one<2, NM, REM>(descs, count, s);
-    if (nt == 4) return launch_one<4, NM, REM>(descs, count, s);
+    // four member tiles: at most two kernel tiles per chain wave (accumulator registers)
+    if constexpr (NM < 4)
+        if (nt == 4) return launch_one<4, NM, REM>(descs, count, s);
     return hipErrorInvalidValue;
 }
 
@@ -68,6 +70,7 @@ hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, int r
 {
     if (nm == 1) return rem ? launch_nt<1, 1>(descs, count, nt, s) : launch_nt<1, 0>(descs, count, nt, s);
     if (nm == 2) return rem ? launch_nt<2, 1>(descs, count, nt, s) : launch_nt<2, 0>(descs, count, nt, s);
+    if (nm == 4) return rem ? launch_nt<4, 1>(descs, count, nt, s) : launch_nt<4, 0>(descs, count, nt, s);
     return hipErrorInvalidValue;
 }
 

@@ -16,7 +16,8 @@ constexpr int kBatchWaves = 16;             // 4 chain, 8 worker, 1 walk, 3 mask
 constexpr int kBatchThreads = kBatchWaves * 64;
 constexpr int kBatchMaskLanes = 192;        // 3 mask wavefronts
 constexpr int kBatchChunkDoubles = 2048;    // members (padded) x genome positions per chunk
-constexpr int kBatchMaxP = 32;
+constexpr int kBatchMaxP = 64;             // members per workgroup (the mixed kernel takes 32)
+constexpr int kBatchMixedMaxP = 32;
 constexpr int kBatchRegionDoubles = 6 * kBatchChunkDoubles;  // 4 population + 2 trial buffers
 constexpr int kBatchMaxRem = 4;             // kernel rows past the last full tile run on 4x4x4 MFMAs
 constexpr int kBatchPackMax = 4;            // instances of one model stacked in one workgroup (FP64)
@@ -65,7 +66,7 @@ struct BatchDesc {
 };
 
 // Exact FP64 kernel.  NT: 16-row kernel tiles per chain wavefront (1, 2 or 4); NM: 16-member
-// tiles (1 or 2); rem: 1 when the kernel has 1..4 rows past its last full tile.
+// tiles (1, 2, or 4 with NT <= 2); rem: 1 when the kernel has 1..4 rows past its last full tile.
 hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, int rem, hipStream_t s);
 
 // Device scratch of deac_batch_run_noalloc: a file-scope array of kBatchScratchDoubles

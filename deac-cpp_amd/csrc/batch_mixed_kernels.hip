@@ -40,6 +40,7 @@ int batch_mixed_nq(int ntm, int nm)
 {
     // pairs per chain wave: ceil(ntm / 2) with two member tiles (a wave keeps
     // one member tile), ceil(ntm / 4) with one
+    if (nm > 2) return 0;  // at most 32 members
     const int need = nm == 2 ? (ntm + 1) / 2 : (ntm + 3) / 4;
     for (int nq : kBatchMixedNq)
         if (nq >= need && (nm == 2 || nq <= 5)) return nq;
@@ -48,6 +49,7 @@ int batch_mixed_nq(int ntm, int nm)
 
 hipError_t launch_batch_mixed(const BatchDesc *descs, int count, int precision, int nq, int nm, hipStream_t s)
 {
+    if (nm != 1 && nm != 2) return hipErrorInvalidValue;
     if (precision == kPrecFp32)
         return nm == 1 ? launch_mixed_nq<kPrecFp32, 1>(descs, count, nq, s)
                        : launch_mixed_nq<kPrecFp32, 2>(descs, count, nq, s);

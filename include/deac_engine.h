@@ -211,7 +211,8 @@ deac_status deac_last_run_profile(const deac_handle *handle, deac_run_profile *p
  * random stream with the delta-form mixed contraction (deac_run_options).
  * Each instance has its own generation count, stop test and precision; an
  * instance that stops leaves the launch.  Identical problems share one
- * device copy of the kernel matrix.  Population size at most 32;
+ * device copy of the kernel matrix.  Population size at most 64 (33 to 64:
+ * FP64 and T <= 132; mixed K.S takes at most 32);
  * per-generation renormalization and DEAC_BDSF run only through deac_run
  * (DEAC_ERR_UNSUPPORTED here); moment constraints run at FP64 (deac_batch_run
  * refuses them with FP32 or BF16X3, DEAC_ERR_UNSUPPORTED before any launch). */

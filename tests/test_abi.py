@@ -120,14 +120,15 @@ def test_status_strings():
         assert da.lib().deac_status_string(code).decode() == text
 
 
-def _batch_problem(**kw):
-    tau = np.linspace(0.0, 0.37, 17)
+def _batch_problem(T=17, **kw):
+    tau = np.linspace(0.0, 0.37, T)
     isf = np.exp(-tau)
     return da.make_problem(tau, isf, 1e-3 * isf, da.frequency_grid(64, 8.0), 1.0, **kw)
 
 
 @pytest.mark.parametrize("kw,status,text", [
-    (dict(population_size=40), 5, "at most 32 members"),
+    (dict(population_size=70), 5, "at most 64 members"),
+    (dict(population_size=40, T=140), 5, "at most 132 timeslices"),
     (dict(spectra_type="bdsf"), 5, "detailed-balance spectra (bdsf)"),
     (dict(normalize=True, renormalize=True), 5, "renormalize each generation"),
     (dict(population_size=2), 1, "population_size"),

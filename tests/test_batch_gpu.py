@@ -233,8 +233,9 @@ def _synthetic_oracle(T, G, P, seed, N):
 def test_batch_shape_limits():
     # the largest T the batch kernel takes (16 full 16-row tiles on four
     # chain waves plus 4 rows on 4x4x4 MFMAs) runs bit-exact against
-    # deac_run; one more timeslice, or one more member, is refused
-    print("[gpu] batch shape limits: T=260 vs deac_run, T=261 and P=33 refused", flush=True)
+    # deac_run; one more timeslice, 65 members, or 33 members past T = 132
+    # is refused
+    print("[gpu] batch shape limits: T=260 vs deac_run, T=261, P=65, P=33 at T=133 refused", flush=True)
     p = _synthetic_problem(260, 96, 32)
     with da.Engine(p) as e:
         plain = [e.run(sd, 40, 0.0) for sd in (2, 3)]
@@ -246,12 +247,14 @@ def test_batch_shape_limits():
         r = _synthetic_oracle(260, 96, 32, sd, 40)
         assert not _compare(o, r), _compare(o, r)
         assert o["rng"] == r["rng"]
-    for T, P in ((261, 32), (65, 33)):
+    for T, P in ((261, 32), (65, 65), (133, 33), (143, 64)):
         with pytest.raises(da.DeacError) as err:
             da.Batch([_synthetic_problem(T, 96, P)])
         assert err.value.status == 5  # DEAC_ERR_UNSUPPORTED
     with pytest.raises(da.DeacError, match="260 timeslices"):
         da.Batch([_synthetic_problem(261, 96, 32)])
+    for T, P in ((132, 64), (127, 33), (128, 48)):
+        da.Batch([_synthetic_problem(T, 96, P)]).close()  # taken
 
 
 @pytest.mark.parametrize("T,P", [(18, 8), (35, 32), (51, 20)])
@@ -368,3 +371,41 @@ def test_batch_moments_match_oracle():
                                    stop_minimum_fitness=0.0, **kw), sd)
         assert not _compare(o, r), (x, _compare(o, r))
         assert o["rng"] == r["rng"]
+
+
+# 33 to 64 members: four 16-member tiles per workgroup (FP64, at most two
+# kernel tiles per chain wave).  T = 51 puts the three remainder rows on all
+# four member tiles' 4x4x4 chains; T = 132 is the largest T with four tiles
+# of members (two full kernel tiles per chain wave plus four remainder rows).
+WIDE = [
+    (65, 300, 64, dict(normalize=True)),
+    (51, 200, 40, dict(model="normalization", spectra_type="ffull", zero_temperature=True, normalize=True)),
+    (132, 120, 64, dict(model="hyperbolic", allow_negative=True)),
+    (128, 97, 33, dict(first_moment=30.0, third_moment=1e9, third_moment_error=1e8, normalize=True)),
+]
+
+
+@pytest.mark.parametrize("T,G,P,kw", WIDE)
+def test_batch_up_to_64_members_matches_oracle(T, G, P, kw):
+    print(f"[gpu] batch with {P} members, T={T} G={G} {kw}", flush=True)
+    t, f, s = _synthetic_arrays(T)
+    temp = 0.0 if kw.get("zero_temperature") else 1.35
+    N, seeds = 150, [5, 6]
+    prob = da.make_problem(t, f, s, da.frequency_grid(G, 64.0), temp, population_size=P, **kw)
+    with da.Batch([prob] * len(seeds)) as b:
+        outs = b.run(seeds, [N, N - 37], [0.0, 0.0], True)
+    for sd, n_gen, o in zip(seeds, (N, N - 37), outs):
+        r = ob.run(ob.make_problem(t, f, s, ob.frequency_grid(G, 64.0), temp, population_size=P,
+                                   number_of_generations=n_gen, stop_minimum_fitness=0.0, **kw), sd)
+        assert not _compare(o, r), (sd, _compare(o, r))
+        assert o["rng"] == r["rng"] and o["generation"] == r["generation"]
+
+
+def test_batch_64_members_refuses_mixed_precision():
+    t, f, s = _synthetic_arrays(65)
+    prob = da.make_problem(t, f, s, da.frequency_grid(128, 64.0), 1.35, population_size=48)
+    with da.Batch([prob]) as b:
+        with pytest.raises(da.DeacError) as e:
+            b.run([1], 10, 0.0, precision="fp32")
+        assert e.value.status == 5
+        b.run([1], 10, 0.0)  # FP64 runs

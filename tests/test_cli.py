@@ -249,18 +249,18 @@ def _log_without_paths(path):
     return [l for l in path.read_text().splitlines() if "filename:" not in l]
 
 
-def _seed_batch_vs_single_runs(tmp_path, extra, seeds, expect_batched):
+def _seed_batch_vs_single_runs(tmp_path, extra, seeds, expect_batched, P="8"):
     fixture = tmp_path / "tiny-isf.bin"
     write_fixture(fixture)
     batch_dir, single_dir = tmp_path / "batch", tmp_path / "single"
-    cmd = deac_command(tmp_path, fixture, N="60", seed=str(seeds[0]),
+    cmd = deac_command(tmp_path, fixture, N="60", P=P, seed=str(seeds[0]),
                        extra=list(extra) + ["--track_stats", "--number_of_seeds", str(len(seeds)), "--average_dsf"])
     cmd[cmd.index("--save_directory") + 1] = str(batch_dir)
     r = run(cmd, tmp_path)
     assert r.returncode == 0, r.stdout
     assert ("seeds run one after another" not in r.stdout) == expect_batched, r.stdout
     for s in seeds:
-        cmd = deac_command(tmp_path, fixture, N="60", seed=str(s), extra=list(extra) + ["--track_stats"])
+        cmd = deac_command(tmp_path, fixture, N="60", P=P, seed=str(s), extra=list(extra) + ["--track_stats"])
         cmd[cmd.index("--save_directory") + 1] = str(single_dir)
         rs = run(cmd, tmp_path)
         assert rs.returncode == 0, rs.stdout
@@ -293,6 +293,18 @@ def test_cli_seed_batch_with_moments_matches_single_runs(tmp_path):
     # moment constraints in the batch kernel (FP64): each seed exactly its own run
     _seed_batch_vs_single_runs(tmp_path, ["--normalize", "--first_moment", "0.5", "--third_moment", "2.0",
                                           "--third_moment_error", "0.5"], [3, 4], expect_batched=True)
+
+
+@pytest.mark.gpu
+def test_cli_seed_batch_64_members_matches_single_runs(tmp_path):
+    # 33 to 64 members run in the FP64 batch kernel (four member tiles)
+    _seed_batch_vs_single_runs(tmp_path, ["--normalize"], [3, 4, 5], expect_batched=True, P="64")
+
+
+@pytest.mark.gpu
+def test_cli_mixed_precision_seeds_with_48_members_run_seed_by_seed(tmp_path):
+    # the mixed batch kernel takes at most 32 members
+    _seed_batch_vs_single_runs(tmp_path, ["--ks_precision", "bf16x3"], [3, 4], expect_batched=False, P="48")
 
 
 @pytest.mark.gpu
