@@ -43,7 +43,8 @@ struct BatchModel {
     int Ppad = 0, NM = 0, NT = 0, C = 0, nchunk = 0, Lpad = 0, Lw = 0, ntt = 0, nrem = 0, S = 0, Lsub = 0;
     int ntm = 0, NQ = 0;                   // mixed K.S: kernel tiles (rows past T zero), pairs per chain wave
     double *ktile = nullptr, *krem = nullptr, *obs = nullptr, *sig = nullptr;
-    double *mterm = nullptr;               // [2][Lpad] first- and third-moment weights, when used
+    double *mterm = nullptr;               // [3][Lpad] first-moment, third-moment and normalization weights,
+                                           // when moments or per-generation renormalization are used
     uint64_t *walk_rows = nullptr, *sub_tab = nullptr;
     void *kmix[3] = {nullptr, nullptr, nullptr};  // mixed K tiles by precision, built on first use
     double *kscale = nullptr;                     // 2^ex[t] of the mixed K rows
@@ -154,10 +155,6 @@ deac_status plan_model(BatchModel &b, std::string &why)
     const deac::Model &m = b.m;
     if (m.P > static_cast<size_t>(deac::kBatchMaxP)) {
         why = "batched runs take at most 64 members per population";
-        return DEAC_ERR_UNSUPPORTED;
-    }
-    if (m.normalize && m.renormalize) {
-        why = "batched runs do not renormalize each generation";
         return DEAC_ERR_UNSUPPORTED;
     }
     if (m.bdsf) {
@@ -314,13 +311,17 @@ deac_status upload_model(deac_batch *h, BatchModel &b)
     A(b.sig, m.T);
     A(b.walk_rows, rows.size());
     A(b.sub_tab, tabs.size());
-    // moment weights (deac.cpp:628-668, 728-768), zero past L
+    // moment weights (deac.cpp:628-668, 728-768) and the normalization
+    // weights of per-generation renormalization (deac.cpp:1342-1352), zero
+    // past L
     std::vector<double> mt;
-    if (m.use_first || m.use_third) {
-        mt.assign(2 * static_cast<size_t>(b.Lpad), 0.0);
+    const bool renorm = m.normalize && m.renormalize;
+    if (m.use_first || m.use_third || renorm) {
+        mt.assign(3 * static_cast<size_t>(b.Lpad), 0.0);
         for (size_t k = 0; k < m.Kc; k++) {
             if (m.use_first) mt[k] = m.fterm[k];
             if (m.use_third) mt[b.Lpad + k] = m.tterm[k];
+            if (renorm) mt[2 * static_cast<size_t>(b.Lpad) + k] = m.nterm[k];
         }
         A(b.mterm, mt.size());
     }
@@ -524,11 +525,19 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
             deac::last_message() = "instance " + std::to_string(i) + ": no mixed K.S kernel for this shape";
             return DEAC_ERR_UNSUPPORTED;
         }
-        if (prec != DEAC_PRECISION_FP64 && h->models[h->model_of[i]]->mterm) {
+        const deac::Model &mi = h->models[h->model_of[i]]->m;
+        if (prec != DEAC_PRECISION_FP64 && (mi.use_first || mi.use_third)) {
             // the moments fold the FP64 trials, which the mixed kernel keeps
             // only as their FP32 / BF16 change from the parent
             deac::last_message() = "instance " + std::to_string(i) + ": mixed K.S in the batch does not take moment "
                               "constraints (run FP64, or the instance through deac_run)";
+            return DEAC_ERR_UNSUPPORTED;
+        }
+        if (prec != DEAC_PRECISION_FP64 && mi.normalize && mi.renormalize) {
+            // the scale of a trial needs all of it before its contraction,
+            // which the mixed kernel's delta form does not take
+            deac::last_message() = "instance " + std::to_string(i) + ": mixed K.S in the batch does not renormalize "
+                              "each generation (run FP64, or the instance through deac_run)";
             return DEAC_ERR_UNSUPPORTED;
         }
     }
@@ -762,6 +771,8 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.first_moment = m.first_moment;
                 d.third_moment = m.third_moment;
                 d.third_moment_error = m.third_moment_error;
+                d.renorm = (m.normalize && m.renormalize) ? 1 : 0;
+                d.zeroth = m.zeroth_moment;
                 d.walk_rows = b.walk_rows;
                 d.sub_tab = b.sub_tab;
                 size_t pop_lo = h->pop_off[i];

@@ -75,7 +75,7 @@ constexpr int kOffTrial = 4 * kCD;     // 2 buffers: trial genomes in MFMA B ord
 constexpr int kMaskWords = 256;        // mask words per chunk: 2 generations x Ppad x 2 candidates x C/32
 constexpr int kMaxT = 4 * 4 * 16 + kBatchMaxRem;  // timeslices: 16 full tiles (4 per chain wave) + remainder rows
 
-enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3 };
+enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3, kPassNorm = 4 };
 enum { kPrecFp64 = 0, kPrecFp32 = 1, kPrecBf16 = 2 };  // DEAC_PRECISION_*
 constexpr int kEk = (kMaxT * kBatchMixedMaxP + kBatchThreads - 1) / kBatchThreads;  // chi^2 terms per thread (mixed)
 
@@ -114,6 +114,8 @@ struct BatchShared {
     alignas(16) double f_trial[2][kBatchMaxP][2];   // read as pairs
     double y0t[kBatchMaxP], y0r[kBatchMaxP];  // trial value at position 0 (zero-frequency match)
     double mom[kBatchMaxP][2];                // first and third moments of the pass's genomes (FP64 kernel)
+    double norm[kBatchMaxP];                  // zeroth moment of each trial (renormalization)
+    double rscale[2][kBatchMaxP];             // renormalization factor of each trial, by generation parity
     int accepted[3][kBatchMaxP];
     int flag;                                 // 0 evolve, 1 stop test fired, 2 end of the launch
     int qstop[kBatchPackMax];                 // instances whose stop test fired
@@ -231,6 +233,15 @@ __device__ __forceinline__ int bf_trial_offset(int m, int k)  // byte offset, k 
 {
     constexpr int Ppad = 16 * NM;
     return ((k >> 5) * Ppad + m) * 64 + ((((k >> 3) & 3) + (m >> 2)) & 3) * 16 + (k & 7) * 2;
+}
+
+// Lane l's double, read by every lane (l wave-uniform): two v_readlane_b32.
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(b), l);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(b >> 32), l);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
 // u < cr for u = (x >> 11) * 2^-53  <=>  (x >> 11) < ceil(cr * 2^53).
@@ -685,9 +696,12 @@ __device__ __forceinline__ void mixed_chains(const BatchDesc &d, BatchShared &sh
 
 // One pass over the genome in chunks.  kind: kPassInit (model ISF of the
 // stored population), kPassEvolve (generation g), kPassApply (write back the
-// genomes accepted in generation g), kPassPre (masks only).  mgen: the
-// generation whose masks the workers draw (-1: none); wgen: the generation
-// the walk wave walks (-1: none).
+// genomes accepted in generation g), kPassPre (masks only), kPassNorm
+// (per-generation renormalization, FP64 kernel: the write-back of generation
+// g-1's acceptances and the zeroth moment of each trial of generation g,
+// before its kPassEvolve, which then forms the trials again, scaled, and
+// rebuilds nothing).  mgen: the generation whose masks the workers draw (-1:
+// none); wgen: the generation the walk wave walks (-1: none).
 template <int PREC, int NT, int NM, int REM>
 __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, double *lds, int kind, long g, long mgen,
                                          long wgen)
@@ -864,8 +878,10 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         // fabs of a DE value, or not (allow_negative): an and of the high word
         const uint32_t absmask = __builtin_amdgcn_readfirstlane(d.allow_neg) ? 0xffffffffu : 0x7fffffffu;
         constexpr int kWpc = C / 32;  // mask words per member and chunk
-        const bool rebuild = kind == kPassEvolve || kind == kPassApply;
-        const long pg = kind == kPassEvolve ? g - 1 : g;      // generation whose accepted trials are rebuilt
+        const bool renorm = PREC == kPrecFp64 && d.renorm;
+        const bool trials = kind == kPassEvolve || kind == kPassNorm;
+        const bool rebuild = kind == kPassApply || kind == kPassNorm || (kind == kPassEvolve && !renorm);
+        const long pg = trials ? g - 1 : g;                   // generation whose accepted trials are rebuilt
         const int pslot = static_cast<int>((pg + 3) % 3);
         const int pprev = static_cast<int>((pg + 2) % 3);     // generation pg-1
         const int gslot = static_cast<int>((g + 3) % 3);
@@ -892,7 +908,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             od[u][0] = od[u][1] = od[u][2] = m * C + p;
             odb[u][0] = odb[u][1] = odb[u][2] = m * C + p;
             mo[u] = mob[u] = 0;
-            if (kind == kPassEvolve) {
+            if (trials) {
                 const WalkRec &r = sh.rec[g & 3][cm];
 #pragma unroll
                 for (int k = 0; k < 3; k++) od[u][k] = static_cast<int>(r.mutant[k] & (kBatchMaxP - 1)) * C + p;
@@ -1054,7 +1070,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             // trial of chunk c-3 (or, for the initial evaluation, the stored
             // genomes) into the trial buffer in MFMA B order
             const int ct = c - 3;
-            if (chains && ct >= 0 && ct < nchunk) {
+            if ((chains || kind == kPassNorm) && ct >= 0 && ct < nchunk) {
                 const double *buf = lds + kOffPop + (ct % 4) * kCD;
                 double *tb = lds + kOffTrial + (ct & 1) * kCD;
                 double2 v[kT];
@@ -1069,6 +1085,15 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     }
                 } else {
                     de_stage(buf, sh.mring[ct % 4], mo, od, g, ct, sh.y0t, v, PREC != kPrecFp64);
+                    if (renorm && kind == kPassEvolve) {
+                        // the trial times zeroth_moment / norm (deac.cpp:1353-1360)
+#pragma unroll
+                        for (int u = 0; u < kT; u++) {
+                            const double sc = sh.rscale[g & 1][fm(u)];
+                            v[u].x *= sc;
+                            v[u].y *= sc;
+                        }
+                    }
                 }
                 if constexpr (PREC == kPrecFp64) {
 #pragma unroll
@@ -1103,8 +1128,17 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             // rebuild of chunk c-1 (generation pg), both rows when either
             // holds an accepted member
             const int cb = c - 1;
-            if (rebuild && cb >= 0 && cb < nchunk && (bm[0] | bm[kT - 1]))
+            if (rebuild && cb >= 0 && cb < nchunk && (bm[0] | bm[kT - 1])) {
                 de_stage(lds + kOffPop + (cb % 4) * kCD, sh.mring[cb % 4], mob, odb, pg, cb, sh.y0r, rb, false);
+                if (renorm) {
+#pragma unroll
+                    for (int u = 0; u < kT; u++) {
+                        const double sc = sh.rscale[pg & 1][fm(u)];
+                        rb[u].x *= sc;
+                        rb[u].y *= sc;
+                    }
+                }
+            }
             // load(c) lands
             if (ld) {
 #pragma unroll
@@ -1154,27 +1188,75 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         int cq = 0;
         uint64_t ws[4] = {sh.walk_state[0][0], sh.walk_state[0][1], sh.walk_state[0][2], sh.walk_state[0][3]};
         int next = 0;
-        const bool moments = PREC == kPrecFp64 && chains && (d.use_first || d.use_third) && lane < d.P;
-        const double *const mt = uniform_ptr(d.mterm);
+        // Moment rows (first, third) or the normalization row (kPassNorm):
+        // the weights of chunk c are the same for every member; lane l loads
+        // positions l, l + 64 of the chunk a step ahead, and the fold reads
+        // them back with v_readlane.
+        const bool moments = PREC == kPrecFp64 && chains && (d.use_first || d.use_third);
+        const bool norms = PREC == kPrecFp64 && kind == kPassNorm;
+        const bool folds = moments || norms;
+        const gcdouble *const mt = (const gcdouble *)uniform_ptr(d.mterm);
         const int nL = __builtin_amdgcn_readfirstlane(d.L), nG = __builtin_amdgcn_readfirstlane(d.G);
         const int nLpad = __builtin_amdgcn_readfirstlane(d.Lpad);
         const bool godd = (nG & 1) != 0;
+        const int row0 = norms ? 2 : 0;  // the weight rows folded: row0 (and row0 + 1 for the moments)
+        constexpr int WPL = (C + kWave - 1) / kWave;
+        double wn[2][WPL], wc[2][WPL];
+        auto load_w = [&](int chunk) {
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+#pragma unroll
+                for (int j = 0; j < WPL; j++) {
+                    const int sl = j * kWave + lane, k = chunk * C + sl;
+                    wn[r][j] = (sl < C && k < nL && (r == 0 || moments)) ? mt[(row0 + r) * nLpad + k] : 0.0;
+                }
+        };
+        if (folds && nchunk > 0) load_w(0);
         double m1 = 0.0, m3 = 0.0;
         for (int c = 0; c < nsteps; c++) {
             const int cm = c - 4;
-            if (moments && cm >= 0 && cm < nchunk) {
+            if (folds && cm >= 0 && cm < nchunk) {
+#pragma unroll
+                for (int r = 0; r < 2; r++)
+#pragma unroll
+                    for (int j = 0; j < WPL; j++) wc[r][j] = wn[r][j];
+                if (cm + 1 < nchunk) load_w(cm + 1);  // lands during this step and the next barrier
                 const double *tb = lds + kOffTrial + (cm & 1) * kCD;
                 const int k0 = cm * C, n = min(C, nL - k0);
-                for (int sl = 0; sl < n; sl++) {
-                    const int k = k0 + sl;
-                    const double v = tb[trial_index<NM>(lane, sl)];
-                    const double a = mt[k], b = mt[nLpad + k];
-                    if (godd && (k == nG - 1 || k == nL - 1)) {
-                        m1 = fma(v, a, m1);
-                        m3 = fma(v, b, m3);
-                    } else {
-                        m1 = m1 + v * a;
-                        m3 = m3 + v * b;
+                const int own = lane < d.P ? lane : 0;  // lanes past P fold member 0's values, discarded
+                auto weight = [&](int r, int sl) {
+                    return readlane_f64(WPL == 1 || sl < kWave ? wc[r][0] : wc[r][WPL - 1], sl & (kWave - 1));
+                };
+                const bool fused = godd && ((nG - 1 >= k0 && nG - 1 < k0 + n) || (nL - 1 >= k0 && nL - 1 < k0 + n));
+                if (n == C && !fused) {
+                    // a whole chunk without a fused position: eight trial
+                    // values read together, then their products added in order
+                    for (int s0 = 0; s0 < C; s0 += 8) {
+                        double v[8];
+#pragma unroll
+                        for (int j = 0; j < 8; j++) v[j] = tb[trial_index<NM>(own, s0 + j)];
+                        if (norms) {
+#pragma unroll
+                            for (int j = 0; j < 8; j++) m1 = m1 + v[j] * weight(0, s0 + j);
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < 8; j++) {
+                                m1 = m1 + v[j] * weight(0, s0 + j);
+                                m3 = m3 + v[j] * weight(1, s0 + j);
+                            }
+                        }
+                    }
+                } else {
+                    for (int sl = 0; sl < n; sl++) {
+                        const int k = k0 + sl;
+                        const double v = tb[trial_index<NM>(own, sl)];
+                        const double a = weight(0, sl);
+                        const bool f = godd && (k == nG - 1 || k == nL - 1);
+                        m1 = f ? fma(v, a, m1) : m1 + v * a;
+                        if (!norms) {
+                            const double b = weight(1, sl);
+                            m3 = f ? fma(v, b, m3) : m3 + v * b;
+                        }
                     }
                 }
             }
@@ -1191,10 +1273,11 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             }
             step_barrier();
         }
-        if (moments) {
+        if (moments && lane < d.P) {
             sh.mom[lane][0] = m1;
             sh.mom[lane][1] = m3;
         }
+        if (norms && lane < d.P) sh.norm[lane] = m1;
         if (lane == 0)
             for (int w = 0; w < 4; w++) sh.walk_state[cq][w] = ws[w];
         __syncthreads();
@@ -1267,11 +1350,24 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
     long last = -1;   // last generation evolved in this launch
     const int tlast = (d.T & 1) ? d.T - 1 : -1;  // fused element of the chi^2 sum
     gdouble *const epar = (gdouble *)d.epar;     // [T][Ppad] parent model ISF, FP64
+    bool normed = false;  // this generation's renormalization pass is done
     for (;;) {
-        const long pg = kind == kPassEvolve ? g : kind == kPassApply ? last : g0 - 1;
-        const long mgen = evaluate ? -1 : kind == kPassEvolve ? g + 1 : kind == kPassApply ? -1 : g0;
-        const long wgen = evaluate ? -1 : kind == kPassEvolve ? g + 2 : kind == kPassApply ? -1 : g0 + 1;
-        run_pass<PREC, NT, NM, REM>(d, sh, lds, kind, pg, mgen, wgen);
+        // per-generation renormalization (deac.cpp:1342-1364): the trials'
+        // zeroth moments first, then the pass that scales and contracts them
+        // (one call site of the pass, so its code appears once)
+        const bool norm = PREC == kPrecFp64 && d.renorm && kind == kPassEvolve && !normed;
+        const int pk = norm ? kPassNorm : kind;
+        const long pg = pk == kPassEvolve || pk == kPassNorm ? g : kind == kPassApply ? last : g0 - 1;
+        const long mgen = evaluate || norm ? -1 : kind == kPassEvolve ? g + 1 : kind == kPassApply ? -1 : g0;
+        const long wgen = evaluate || norm ? -1 : kind == kPassEvolve ? g + 2 : kind == kPassApply ? -1 : g0 + 1;
+        run_pass<PREC, NT, NM, REM>(d, sh, lds, pk, pg, mgen, wgen);
+        if (norm) {
+            if (threadIdx.x < P) sh.rscale[g & 1][threadIdx.x] = d.zeroth / sh.norm[threadIdx.x];
+            __syncthreads();
+            normed = true;
+            continue;
+        }
+        normed = false;
         if (kind == kPassApply) break;
 
         if (kind == kPassInit || kind == kPassEvolve) {

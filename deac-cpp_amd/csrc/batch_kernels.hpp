@@ -33,7 +33,8 @@ struct BatchDesc {
                                // bfloat16 (BF16X3), rows scaled by 2^-ex[t], zero past T and L
     const double *kscale;      // mixed K.S: [T] 2^ex[t]
     const double *obs, *sig;   // [T] ISF and its error
-    const double *mterm;       // [2][Lpad] first- and third-moment weights (zero past L), or null (FP64 kernel)
+    const double *mterm;       // [3][Lpad] first-moment, third-moment and normalization weights (zero past L),
+                               // or null (FP64 kernel)
     const uint64_t *walk_rows; // [256][4] rows of T^L, T the xoshiro transition
     const uint64_t *sub_tab;   // [S-1][64][16][4] nibble tables of T^(k*Lsub), k = 1..S-1
     // state of Q instances of one model stacked along the members: member m
@@ -55,6 +56,7 @@ struct BatchDesc {
     double stop[kBatchPackMax];    // stop_minimum_fitness per instance
     double tau_cr, tau_f;
     double first_moment, third_moment, third_moment_error;  // used when use_first / use_third
+    double zeroth;             // zeroth moment the trials are renormalized to (renorm)
     uint64_t mod_magic;        // floor((2^64-1)/Pi)
     int init;                  // evaluate the initial population first (gen0 == 0; FP64 kernel only)
     int mode;                  // 0: evolve; 1: evaluation only (FP64 kernel, init set): fitness and epar of the
@@ -63,6 +65,7 @@ struct BatchDesc {
     int P;                     // members in the workgroup, Q Pi
     int Q, Pi;                 // instances and members per instance
     int use_first, use_third;  // moment constraints (deac.cpp:1420-1477, 1550-1555), FP64 kernel only
+    int renorm;                // renormalize each generation's trials (deac.cpp:1342-1364), FP64 kernel only
 };
 
 // Exact FP64 kernel.  NT: 16-row kernel tiles per chain wavefront (1, 2 or 4); NM: 16-member

@@ -130,7 +130,6 @@ def _batch_problem(T=17, **kw):
     (dict(population_size=70), 5, "at most 64 members"),
     (dict(population_size=40, T=140), 5, "at most 132 timeslices"),
     (dict(spectra_type="bdsf"), 5, "detailed-balance spectra (bdsf)"),
-    (dict(normalize=True, renormalize=True), 5, "renormalize each generation"),
     (dict(population_size=2), 1, "population_size"),
 ])
 def test_batch_create_rejects_before_touching_a_device(kw, status, text):

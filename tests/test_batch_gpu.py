@@ -23,7 +23,7 @@ def batch_compatible(case):
     # the complete 200000-generation config-2 run is the plain engine's case
     # (one instance is a single workgroup here); config 4's long seeds run
     # inside a 256-seed batch below
-    return (a["P"] <= 32 and m["variant"] != "standard_sp" and a["N"] <= 20000 and m["spectra"] != "bdsf")
+    return a["P"] <= 32 and a["N"] <= 20000 and m["spectra"] != "bdsf"
 
 
 def _engine_problem(case):
@@ -409,3 +409,46 @@ def test_batch_64_members_refuses_mixed_precision():
             b.run([1], 10, 0.0, precision="fp32")
         assert e.value.status == 5
         b.run([1], 10, 0.0)  # FP64 runs
+
+
+# Per-generation renormalization (the reference's single-particle build,
+# deac.cpp:1342-1364): each generation's trials are scaled to the zeroth
+# moment before their contraction, in a pass of their own.
+RENORM = [
+    (33, 200, 8, dict(spectra_type="spfsf"), 1.0),
+    (65, 301, 32, dict(spectra_type="spbsf", model="hyperbolic"), 1.0),
+    (17, 90, 12, dict(spectra_type="spfsf", zero_temperature=True, first_moment=0.4), 0.0),
+    (51, 128, 48, dict(spectra_type="bfull", model="normalization", third_moment=2.0, third_moment_error=0.5), 1.0),
+]
+
+
+@pytest.mark.parametrize("T,G,P,kw,temp", RENORM)
+@pytest.mark.parametrize("slice_len", [None, "11"])
+def test_batch_renormalization_matches_oracle(T, G, P, kw, temp, slice_len, monkeypatch):
+    print(f"[gpu] batch with renormalization T={T} G={G} P={P} {kw} slice={slice_len}", flush=True)
+    if slice_len:
+        monkeypatch.setenv("DEAC_BATCH_SLICE", slice_len)
+    t, f, s = _synthetic_arrays(T)
+    if kw["spectra_type"].startswith("sp"):
+        f, s = -f, s  # single-particle ISFs are negative
+    N, seeds = 120, [7, 8, 9]
+    prob = da.make_problem(t, f, s, da.frequency_grid(G, 32.0), temp, normalize=True, renormalize=True,
+                           population_size=P, **kw)
+    with da.Batch([prob] * len(seeds)) as b:
+        outs = b.run(seeds, [N, N - 29, N], [0.0, 0.0, 0.0], True)
+    for sd, n_gen, o in zip(seeds, (N, N - 29, N), outs):
+        r = ob.run(ob.make_problem(t, f, s, ob.frequency_grid(G, 32.0), temp, normalize=True, renormalize=True,
+                                   population_size=P, number_of_generations=n_gen, stop_minimum_fitness=0.0,
+                                   **kw), sd)
+        assert not _compare(o, r), (sd, _compare(o, r))
+        assert o["rng"] == r["rng"] and o["generation"] == r["generation"]
+
+
+def test_batch_renormalization_refuses_mixed_precision():
+    t, f, s = _synthetic_arrays(33)
+    prob = da.make_problem(t, -f, s, da.frequency_grid(64, 32.0), 1.0, normalize=True, renormalize=True,
+                           spectra_type="spfsf")
+    with da.Batch([prob]) as b:
+        with pytest.raises(da.DeacError, match="renormalize") as e:
+            b.run([1], 10, 0.0, precision="bf16x3")
+        assert e.value.status == 5

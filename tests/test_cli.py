@@ -308,10 +308,16 @@ def test_cli_mixed_precision_seeds_with_48_members_run_seed_by_seed(tmp_path):
 
 
 @pytest.mark.gpu
-def test_cli_seed_batch_unbatchable_runs_seed_by_seed(tmp_path):
-    # per-generation renormalization runs through the single-instance engine
+def test_cli_seed_batch_with_renormalization_matches_single_runs(tmp_path):
+    # per-generation renormalization (single-particle spectra) in the batch kernel
     _seed_batch_vs_single_runs(tmp_path, ["--normalize", "--single_particle_fermionic_spectral_function"], [3, 4],
-                               expect_batched=False)
+                               expect_batched=True)
+
+
+@pytest.mark.gpu
+def test_cli_seed_batch_unbatchable_runs_seed_by_seed(tmp_path):
+    # more than 64 members run through the single-instance engine
+    _seed_batch_vs_single_runs(tmp_path, ["--normalize"], [3, 4], expect_batched=False, P="70")
 
 
 @pytest.mark.gpu
