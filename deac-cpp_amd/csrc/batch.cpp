@@ -101,7 +101,13 @@ struct deac_batch {
     uint32_t *masks = nullptr;
     uint64_t *rng = nullptr;
     long *status = nullptr;                    // per instance: stopped, generation reached
+    int *accp = nullptr;                       // per instance [kBatchMaxP]: acceptances of the last generation
+                                               // a launch evolved
+    std::vector<long> masks_at;                // per instance: generation whose masks (and the acceptances
+                                               // before it) its last launch left on the device, or -1
     deac::BatchDesc *descs = nullptr;          // device copy of the launch descriptors
+    deac::BatchDesc *hdescs = nullptr;         // their host side, pinned (3 per instance)
+    long *hstatus = nullptr;                   // pinned host copy of status
     double *scratch = nullptr;                 // this device's statistics scratch (deac_batch_run_noalloc)
     std::vector<long> reached;                 // per instance: generation reached by the last run (absolute)
     std::vector<char> epar_valid;              // per instance: epar holds the current population's model ISF
@@ -115,6 +121,8 @@ struct deac_batch {
         for (hipStream_t g : gstream)
             if (g) (void)hipStreamSynchronize(g);
         for (void *p : allocations) (void)hipFree(p);
+        if (hdescs) (void)hipHostFree(hdescs);
+        if (hstatus) (void)hipHostFree(hstatus);
         for (hipEvent_t e : {ev0, ev1, kev0, kev1, kev2, kev3, fork})
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : gdone)
@@ -465,9 +473,16 @@ deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_b
     A(h->f_old, count * deac::kBatchMaxP * 2);
     A(h->rng, count * 4);
     A(h->status, count * 2);
+    A(h->accp, count * deac::kBatchMaxP);
     A(h->epar, nepar);
     A(h->descs, 3 * count);  // evolution launches, and the evaluation launches of mixed K.S before and after
     if (st != DEAC_OK) return st;
+    // pinned host sides of the per-round descriptor upload and status read
+    if (hipHostMalloc(reinterpret_cast<void **>(&h->hdescs), 3 * count * sizeof(deac::BatchDesc)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&h->hstatus), 2 * count * sizeof(long)) != hipSuccess) {
+        (void)hipGetLastError();
+        return DEAC_ERR_OUT_OF_MEMORY;
+    }
     *out = h.release();
     return DEAC_OK;
 }
@@ -556,6 +571,11 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
     if (resume) base = h->reached;
     std::vector<char> epar_ok = h->epar_valid;
     epar_ok.resize(n, 0);
+    if (!resume) h->masks_at.assign(n, -1);
+    h->masks_at.resize(n, -1);
+    // a run may fail part way: until it completes, nothing it left counts
+    std::vector<long> masks_at = h->masks_at;
+    h->masks_at.assign(n, -1);
     h->reached.clear();  // set again when this call completes
     std::vector<uint64_t> rng_host(4 * n);
     if (!resume) {
@@ -625,9 +645,10 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
     std::vector<long> next_gen(base), stopped_at(n, -1), n_abs(n);
     for (size_t i = 0; i < n; i++) n_abs[i] = base[i] + static_cast<long>(options[i].number_of_generations);
     std::vector<char> active(n, 1);
-    std::vector<deac::BatchDesc> descs(3 * n);
+    deac::BatchDesc *const descs = h->hdescs;  // pinned: the upload of one round completes before the next
     deac_run_profile prof{};
     BATCH_HIP(hipEventRecord(h->ev0, s));
+    BATCH_HIP(hipEventRecord(h->ev1, s));  // re-recorded after every round's status copy
     bool first = true;
     // Launch key: (precision, NT or NQ, NM, remainder rows).
     typedef std::tuple<int, int, int, int> Key;
@@ -793,6 +814,7 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                     d.rng[q] = h->rng + 4 * j;
                     d.stats[q] = stats[j];
                     d.status[q] = h->status + 2 * j;
+                    d.accp[q] = h->accp + j * deac::kBatchMaxP;
                     d.stats_len[q] = static_cast<long>(options[j].number_of_generations);
                     d.stats_base[q] = base[j];
                     d.stop[q] = options[j].stop_minimum_fitness;
@@ -806,6 +828,9 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.P = d.Q * d.Pi;
                 d.mod_magic = ~UINT64_C(0) / m.P;
                 d.init = (first && !resume && prec == DEAC_PRECISION_FP64) ? 1 : 0;
+                // an instance alone in its workgroup whose last launch ended here
+                // resumes its masks instead of drawing them again
+                d.resume = (!d.init && pk.size() == 1 && masks_at[i] == next_gen[i]) ? 1 : 0;
                 d.mode = 0;
                 d.T = static_cast<int>(m.T);
                 d.G = static_cast<int>(m.G);
@@ -849,7 +874,7 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                     (grp == &eb ? evals_before : evals_after).push_back({key, {begin, slot - begin}});
                 }
         }
-        BATCH_HIP(hipMemcpyAsync(h->descs, descs.data(), slot * sizeof(deac::BatchDesc), hipMemcpyHostToDevice, s));
+        BATCH_HIP(hipMemcpyAsync(h->descs, descs, slot * sizeof(deac::BatchDesc), hipMemcpyHostToDevice, s));
         // events: kev0 | evaluations before | kev1 | evolution | kev2 | evaluations after | kev3
         BATCH_HIP(hipEventRecord(h->kev0, s));
         for (auto &[key, range] : evals_before)
@@ -876,8 +901,10 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
         for (auto &[key, range] : evals_after)
             BATCH_HIP(launch(key, h->descs + range.first, static_cast<int>(range.second), s));
         BATCH_HIP(hipEventRecord(h->kev3, s));
-        std::vector<long> st(2 * n);
-        BATCH_HIP(hipMemcpyAsync(st.data(), h->status, st.size() * sizeof(long), hipMemcpyDeviceToHost, s));
+        long *const st = h->hstatus;
+        BATCH_HIP(hipMemcpyAsync(st, h->status, 2 * n * sizeof(long), hipMemcpyDeviceToHost, s));
+        // the device work of the run ends here when this round is the last
+        BATCH_HIP(hipEventRecord(h->ev1, s));
         BATCH_HIP(hipStreamSynchronize(s));
         float ms_before = 0.0f, ms_evolve = 0.0f, ms_after = 0.0f;
         BATCH_HIP(hipEventElapsedTime(&ms_before, h->kev0, h->kev1));
@@ -895,6 +922,9 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
             if (reached < descs[k].gen0 || reached > descs[k].gen_end || (st[2 * i] != 0 && st[2 * i] != 1))
                 return DEAC_ERR_DEVICE;
             prof.generations += static_cast<size_t>(reached - next_gen[i]);
+            // a launch that evolved a generation alone in its workgroup drew the
+            // masks of the generation it ended at
+            masks_at[i] = (descs[k].Q == 1 && reached > descs[k].gen0) ? reached : -1;
             next_gen[i] = reached;
             if (st[2 * i]) {
                 stopped_at[i] = reached;
@@ -905,7 +935,6 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
         }
         first = false;
     }
-    BATCH_HIP(hipEventRecord(h->ev1, s));
     BATCH_HIP(hipEventSynchronize(h->ev1));
     float loop = 0.0f;
     BATCH_HIP(hipEventElapsedTime(&loop, h->ev0, h->ev1));
@@ -958,6 +987,7 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
         std::memcpy(&rng_states[4 * i], &rng_host[4 * i], 4 * sizeof(uint64_t));
     }
     h->reached = next_gen;
+    h->masks_at = masks_at;
     h->epar_valid.assign(n, 0);
     for (size_t i = 0; i < n; i++) h->epar_valid[i] = options[i].ks_precision != DEAC_PRECISION_FP64;
     return DEAC_OK;

@@ -914,10 +914,12 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                 for (int k = 0; k < 3; k++) od[u][k] = static_cast<int>(r.mutant[k] & (kBatchMaxP - 1)) * C + p;
                 mo[u] = (cm * 2 + (sh.accepted[aslot][cm] & 1)) * kWpc + (p >> 5);
             }
-            // The walk record of pg is read only for accepted members: the
-            // record of the generation before a launch's first one was never
-            // written in that launch (and nothing is accepted there).
-            const bool acc = rebuild && pg >= 0 && m < nP && sh.accepted[pslot][cm];
+            // The walk record of pg is read only for accepted members, and
+            // only for generations evolved in this launch: the record of the
+            // generation before a launch's first one was never written in it
+            // (a resumed launch holds that generation's acceptances, for the
+            // crossover candidates, but its genomes were written back).
+            const bool acc = rebuild && pg >= d.gen0 && m < nP && sh.accepted[pslot][cm];
             if (acc) {
                 const WalkRec &r = sh.rec[pg & 3][cm];
 #pragma unroll
@@ -1321,6 +1323,8 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
             sh.f_old[m][sg] = m < P ? fw[i * 2 + sg] : 0.0;
         }
         for (int k = 0; k < 3; k++) sh.accepted[k][m] = 0;
+        if (d.resume && !evaluate && m < P)
+            sh.accepted[(d.gen0 + 2) % 3][m] = ((const __attribute__((address_space(1))) int *)d.accp[q])[i];
     }
     __syncthreads();
 
@@ -1334,7 +1338,13 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         for (int q = 0; q < d.Q; q++) {
             const gcu64 *rng = (const gcu64 *)d.rng[q];
             uint64_t ws[4] = {rng[0], rng[1], rng[2], rng[3]};
-            for (int item = 0; item < 2 * d.Pi; item++) walk_item(d, sh, ws, g0, q, item, lane);
+            // a resumed launch walks the next generation too (the mask pass
+            // that would walk it is skipped); one loop, so one copy of the walk
+            const int items = (d.resume ? 4 : 2) * d.Pi;
+            for (int it = 0; it < items; it++) {
+                const int gi = it >= 2 * d.Pi ? 1 : 0;
+                walk_item(d, sh, ws, g0 + gi, q, it - 2 * d.Pi * gi, lane);
+            }
             if (lane == 0)
                 for (int w = 0; w < 4; w++) sh.walk_state[q][w] = ws[w];
         }
@@ -1360,7 +1370,9 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         const long pg = pk == kPassEvolve || pk == kPassNorm ? g : kind == kPassApply ? last : g0 - 1;
         const long mgen = evaluate || norm ? -1 : kind == kPassEvolve ? g + 1 : kind == kPassApply ? -1 : g0;
         const long wgen = evaluate || norm ? -1 : kind == kPassEvolve ? g + 2 : kind == kPassApply ? -1 : g0 + 1;
-        run_pass<PREC, NT, NM, REM>(d, sh, lds, pk, pg, mgen, wgen);
+        // a resumed launch has gen0's masks on the device and walked gen0 + 1
+        // in the prologue: no mask pass
+        if (!(kind == kPassPre && d.resume && !evaluate)) run_pass<PREC, NT, NM, REM>(d, sh, lds, pk, pg, mgen, wgen);
         if (norm) {
             if (threadIdx.x < P) sh.rscale[g & 1][threadIdx.x] = d.zeroth / sh.norm[threadIdx.x];
             __syncthreads();
@@ -1510,6 +1522,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         const int m = threadIdx.x, q = m / d.Pi, i = m - q * d.Pi;
         gdouble *fit = (gdouble *)d.fitness[q], *cr = (gdouble *)d.cr_old[q], *fw = (gdouble *)d.f_old[q];
         fit[i] = sh.fitness[m];
+        if (last >= 0) ((__attribute__((address_space(1))) int *)d.accp[q])[i] = sh.accepted[last % 3][m];
         for (int sg = 0; sg < 2; sg++) {
             cr[i * 2 + sg] = sh.cr_old[m][sg];
             fw[i * 2 + sg] = sh.f_old[m][sg];

@@ -51,6 +51,8 @@ struct BatchDesc {
                                // evaluation launches), or null
     double *stats[kBatchPackMax];  // [3][stats_len] mean, minimum, squared mean per generation, or null
     long *status[kBatchPackMax];   // [0] stopped, [1] generation reached
+    int *accp[kBatchPackMax];      // [Pi] acceptances of the last generation a launch evolved (written at
+                                   // its end; read at the start of a resume launch)
     long stats_len[kBatchPackMax], stats_base[kBatchPackMax];  // entry for generation g: stats[g - stats_base]
     long gen0, gen_end;        // evolve generations gen0 .. gen_end-1 unless the stop test fires
     double stop[kBatchPackMax];    // stop_minimum_fitness per instance
@@ -59,6 +61,9 @@ struct BatchDesc {
     double zeroth;             // zeroth moment the trials are renormalized to (renorm)
     uint64_t mod_magic;        // floor((2^64-1)/Pi)
     int init;                  // evaluate the initial population first (gen0 == 0; FP64 kernel only)
+    int resume;                // the previous launch of this instance (Q = 1) ended at gen0 and left the
+                               // masks of gen0 and the acceptances of gen0 - 1: walk gen0 and gen0 + 1 and
+                               // skip the mask pass
     int mode;                  // 0: evolve; 1: evaluation only (FP64 kernel, init set): fitness and epar of the
                                // stored population, nothing else written
     int T, G, nsign, L, Lpad, Lw, nchunk, ntt, nrem, ntm, S, Lsub, allow_neg;
