@@ -75,7 +75,7 @@ constexpr int kOffTrial = 4 * kCD;     // 2 buffers: trial genomes in MFMA B ord
 constexpr int kMaskWords = 256;        // mask words per chunk: 2 generations x Ppad x 2 candidates x C/32
 constexpr int kMaxT = 4 * 4 * 16 + kBatchMaxRem;  // timeslices: 16 full tiles (4 per chain wave) + remainder rows
 
-enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3, kPassNorm = 4 };
+enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3, kPassNorm = 4, kPassNone = 5 };
 enum { kPrecFp64 = 0, kPrecFp32 = 1, kPrecBf16 = 2 };  // DEAC_PRECISION_*
 constexpr int kEk = (kMaxT * kBatchMixedMaxP + kBatchThreads - 1) / kBatchThreads;  // chi^2 terms per thread (mixed)
 
@@ -532,7 +532,7 @@ struct Stager {
         const int w = e % kWpc, cand = (e / kWpc) & 1, m = (e / (2 * kWpc)) % Ppad;
         const int slot = e / (2 * kWpc * Ppad) == 0 ? gslot : pslot;
         moff = static_cast<uint32_t>(((slot * Ppad + m) * 2 + cand) * nLw + w) * 4u;
-        nload = kind != kPassPre ? nchunk : 0;
+        nload = kind != kPassPre && kind != kPassNone ? nchunk : 0;
         mw = 0;
     }
     __device__ __forceinline__ void issue(int c)
@@ -700,8 +700,9 @@ __device__ __forceinline__ void mixed_chains(const BatchDesc &d, BatchShared &sh
 // (per-generation renormalization, FP64 kernel: the write-back of generation
 // g-1's acceptances and the zeroth moment of each trial of generation g,
 // before its kPassEvolve, which then forms the trials again, scaled, and
-// rebuilds nothing).  mgen: the generation whose masks the workers draw (-1:
-// none); wgen: the generation the walk wave walks (-1: none).
+// rebuilds nothing), kPassNone (no steps: the mask pass of a resumed
+// launch).  mgen: the generation whose masks the workers draw (-1: none);
+// wgen: the generation the walk wave walks (-1: none).
 template <int PREC, int NT, int NM, int REM>
 __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, double *lds, int kind, long g, long mgen,
                                          long wgen)
@@ -712,7 +713,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     const int nchunk = d.nchunk;
     const bool chains = kind == kPassInit || kind == kPassEvolve;
-    const int nsteps = kind == kPassApply ? nchunk + 2 : nchunk + 4;
+    const int nsteps = kind == kPassApply ? nchunk + 2 : kind == kPassNone ? 0 : nchunk + 4;
 
     if (wave < kWorker0) {
         if constexpr (PREC != kPrecFp64) {
@@ -1366,13 +1367,14 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         // zeroth moments first, then the pass that scales and contracts them
         // (one call site of the pass, so its code appears once)
         const bool norm = PREC == kPrecFp64 && d.renorm && kind == kPassEvolve && !normed;
-        const int pk = norm ? kPassNorm : kind;
-        const long pg = pk == kPassEvolve || pk == kPassNorm ? g : kind == kPassApply ? last : g0 - 1;
-        const long mgen = evaluate || norm ? -1 : kind == kPassEvolve ? g + 1 : kind == kPassApply ? -1 : g0;
-        const long wgen = evaluate || norm ? -1 : kind == kPassEvolve ? g + 2 : kind == kPassApply ? -1 : g0 + 1;
         // a resumed launch has gen0's masks on the device and walked gen0 + 1
-        // in the prologue: no mask pass
-        if (!(kind == kPassPre && d.resume && !evaluate)) run_pass<PREC, NT, NM, REM>(d, sh, lds, pk, pg, mgen, wgen);
+        // in the prologue: its mask pass has no steps
+        const bool none = kind == kPassPre && d.resume && !evaluate;
+        const int pk = norm ? kPassNorm : none ? kPassNone : kind;
+        const long pg = pk == kPassEvolve || pk == kPassNorm ? g : kind == kPassApply ? last : g0 - 1;
+        const long mgen = evaluate || norm || none ? -1 : kind == kPassEvolve ? g + 1 : kind == kPassApply ? -1 : g0;
+        const long wgen = evaluate || norm || none ? -1 : kind == kPassEvolve ? g + 2 : kind == kPassApply ? -1 : g0 + 1;
+        run_pass<PREC, NT, NM, REM>(d, sh, lds, pk, pg, mgen, wgen);
         if (norm) {
             if (threadIdx.x < P) sh.rscale[g & 1][threadIdx.x] = d.zeroth / sh.norm[threadIdx.x];
             __syncthreads();
