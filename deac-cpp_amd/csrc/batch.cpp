@@ -43,6 +43,9 @@ struct BatchModel {
     int Ppad = 0, NM = 0, NT = 0, C = 0, nchunk = 0, Lpad = 0, Lw = 0, ntt = 0, nrem = 0, S = 0, Lsub = 0;
     int ntm = 0, NQ = 0;                   // mixed K.S: kernel tiles (rows past T zero), pairs per chain wave
     double *ktile = nullptr, *krem = nullptr, *obs = nullptr, *sig = nullptr;
+    double *kfold = nullptr;               // bdsf: K transposed [Lpad][T] for the fold kernel
+    double *nfm_dt = nullptr;              // bdsf: -1st-moment weights [T], when used
+    int fold_groups = 0, fold_members = 0, MF = 0;  // bdsf fold kernel layout (BatchDesc)
     double *mterm = nullptr;               // [3][Lpad] first-moment, third-moment and normalization weights,
                                            // when moments or per-generation renormalization are used
     uint64_t *walk_rows = nullptr, *sub_tab = nullptr;
@@ -165,13 +168,20 @@ deac_status plan_model(BatchModel &b, std::string &why)
         why = "batched runs take at most 64 members per population";
         return DEAC_ERR_UNSUPPORTED;
     }
+    const int T = static_cast<int>(m.T), L = static_cast<int>(m.Kc);
     if (m.bdsf) {
         // the detailed-balance build sums the trials' model ISF as a fold of
-        // rounded products, which the batch kernel's MFMA chains do not
-        why = "batched runs do not take the detailed-balance spectra (bdsf)";
-        return DEAC_ERR_UNSUPPORTED;
+        // rounded products, which the fold kernel forms on the VALU: one
+        // lane per (timeslice, member group) of the 256 chain lanes, at most
+        // 16 members per group
+        b.fold_groups = T <= 256 ? 256 / T : 0;
+        b.fold_members = b.fold_groups ? (static_cast<int>(m.P) + b.fold_groups - 1) / b.fold_groups : 0;
+        b.MF = b.fold_members == 0 ? 0 : b.fold_members <= 4 ? 4 : b.fold_members <= 8 ? 8 : b.fold_members <= 16 ? 16 : 0;
+        if (b.MF == 0) {
+            why = "batched detailed-balance (bdsf) runs take at most 16 * floor(256 / T) members and T <= 256";
+            return DEAC_ERR_UNSUPPORTED;
+        }
     }
-    const int T = static_cast<int>(m.T), L = static_cast<int>(m.Kc);
     b.Ppad = m.P <= 16 ? 16 : m.P <= 32 ? 32 : 64;
     b.NM = b.Ppad / 16;
     b.C = deac::kBatchChunkDoubles / b.Ppad;
@@ -344,6 +354,15 @@ deac_status upload_model(deac_batch *h, BatchModel &b)
     up(b.walk_rows, rows.data(), rows.size() * sizeof(uint64_t));
     up(b.sub_tab, tabs.data(), tabs.size() * sizeof(uint64_t));
     if (b.mterm) up(b.mterm, mt.data(), mt.size() * sizeof(double));
+    if (m.bdsf && st == DEAC_OK) {
+        std::vector<double> kf(static_cast<size_t>(b.Lpad) * T, 0.0);
+        for (int t = 0; t < T; t++)
+            for (int k = 0; k < L; k++) kf[static_cast<size_t>(k) * T + t] = m.kernel[static_cast<size_t>(t) * L + k];
+        A(b.kfold, kf.size());
+        if (m.use_nfm) A(b.nfm_dt, m.T);
+        up(b.kfold, kf.data(), kf.size() * sizeof(double));
+        if (b.nfm_dt) up(b.nfm_dt, m.nfm_dt.data(), m.T * sizeof(double));
+    }
     return st;
 }
 
@@ -541,6 +560,11 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
             return DEAC_ERR_UNSUPPORTED;
         }
         const deac::Model &mi = h->models[h->model_of[i]]->m;
+        if (prec != DEAC_PRECISION_FP64 && mi.bdsf) {
+            deac::last_message() = "instance " + std::to_string(i) + ": mixed K.S in the batch does not take the "
+                              "detailed-balance spectra (run FP64, or the instance through deac_run)";
+            return DEAC_ERR_UNSUPPORTED;
+        }
         if (prec != DEAC_PRECISION_FP64 && (mi.use_first || mi.use_third)) {
             // the moments fold the FP64 trials, which the mixed kernel keeps
             // only as their FP32 / BF16 change from the parent
@@ -652,15 +676,21 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
     bool first = true;
     // Launch key: (precision, NT or NQ, NM, remainder rows).
     typedef std::tuple<int, int, int, int> Key;
+    // bdsf instances evolve on the fold kernel (precision kFoldKey); the FP64
+    // kernel evaluates their initial population (the reference's initial
+    // contraction is the FMA chain, deac.cpp:860)
+    constexpr int kFoldKey = 3;
     auto key_of = [&](size_t i, bool eval) -> Key {
         const BatchModel &b = *h->models[h->model_of[i]];
         const int prec = options[i].ks_precision;
+        if (b.m.bdsf && !eval) return Key{kFoldKey, b.MF, b.NM, 0};
         if (prec == DEAC_PRECISION_FP64 || eval) return Key{DEAC_PRECISION_FP64, b.NT, b.NM, b.nrem > 0 ? 1 : 0};
         return Key{prec, b.NQ, b.NM, 0};
     };
     auto launch = [&](const Key &k, const deac::BatchDesc *dd, int cnt, hipStream_t st) {
         if (std::get<0>(k) == DEAC_PRECISION_FP64)
             return deac::launch_batch(dd, cnt, std::get<1>(k), std::get<2>(k), std::get<3>(k), st);
+        if (std::get<0>(k) == kFoldKey) return deac::launch_batch_fold(dd, cnt, std::get<1>(k), std::get<2>(k), st);
         return deac::launch_batch_mixed(dd, cnt, std::get<0>(k), std::get<1>(k), std::get<2>(k), st);
     };
     // Instances of instance i's problem that may share its workgroup (FP64;
@@ -677,7 +707,7 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
     const size_t n_cu = static_cast<size_t>(std::max(h->cu_count, 1));
     auto pack_limit = [&](size_t i, size_t n_active) {
         const BatchModel &b = *h->models[h->model_of[i]];
-        if (options[i].ks_precision != DEAC_PRECISION_FP64 || h->sched.pack == 1 || b.Ppad != 16) return 1;
+        if (options[i].ks_precision != DEAC_PRECISION_FP64 || h->sched.pack == 1 || b.Ppad != 16 || b.m.bdsf) return 1;
         int q = std::min<int>(deac::kBatchPackMax, 16 / static_cast<int>(b.m.P));
         if (h->sched.pack > 1) q = std::min(q, h->sched.pack);
         else {
@@ -751,7 +781,7 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
         std::vector<std::pair<Key, std::pair<size_t, size_t>>> launches, evals_before, evals_after;
         std::vector<size_t> order;                       // lead instance of each slot
         std::vector<std::pair<size_t, size_t>> placed;   // (instance, slot) of this round's evolution launches
-        std::vector<size_t> mixed_slots;  // evolution slots of mixed instances
+        std::vector<size_t> mixed_slots;  // evolution slots of instances not on the FP64 kernel (mixed, bdsf)
         for (auto &[key, mp] : glist) {
             const std::vector<size_t> &members = *mp;
             // Workgroups: FP64 instances of one problem with small populations
@@ -792,6 +822,13 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.first_moment = m.first_moment;
                 d.third_moment = m.third_moment;
                 d.third_moment_error = m.third_moment_error;
+                d.kfold = b.kfold;
+                d.fold_groups = b.fold_groups;
+                d.fold_members = b.fold_members;
+                d.use_nfm = m.use_nfm ? 1 : 0;
+                d.nfm_dt = b.nfm_dt;
+                d.nfm_value = m.nfm_value;
+                d.nfm_error = m.nfm_error;
                 d.renorm = (m.normalize && m.renormalize) ? 1 : 0;
                 d.zeroth = m.zeroth_moment;
                 d.walk_rows = b.walk_rows;
@@ -827,7 +864,7 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.Pi = static_cast<int>(m.P);
                 d.P = d.Q * d.Pi;
                 d.mod_magic = ~UINT64_C(0) / m.P;
-                d.init = (first && !resume && prec == DEAC_PRECISION_FP64) ? 1 : 0;
+                d.init = (first && !resume && std::get<0>(key) == DEAC_PRECISION_FP64) ? 1 : 0;
                 // an instance alone in its workgroup whose last launch ended here
                 // resumes its masks instead of drawing them again
                 d.resume = (!d.init && pk.size() == 1 && masks_at[i] == next_gen[i]) ? 1 : 0;
@@ -845,7 +882,7 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.S = b.S;
                 d.Lsub = b.Lsub;
                 d.allow_neg = m.allow_negative ? 1 : 0;
-                if (prec != DEAC_PRECISION_FP64) mixed_slots.push_back(slot);
+                if (std::get<0>(key) != DEAC_PRECISION_FP64) mixed_slots.push_back(slot);
                 order.push_back(i);
                 slot++;
             }
@@ -859,6 +896,11 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
             std::map<Key, std::vector<size_t>> eb, ea;
             for (size_t k : mixed_slots) {
                 const size_t i = order[k];
+                if (h->models[h->model_of[i]]->m.bdsf) {
+                    // exact folds: only the initial population's evaluation
+                    if (first && !resume) eb[key_of(i, true)].push_back(k);
+                    continue;
+                }
                 ea[key_of(i, true)].push_back(k);
                 if (first && !(resume && epar_ok[i])) eb[key_of(i, true)].push_back(k);
             }
@@ -1092,8 +1134,11 @@ deac_status deac_batch_tune(deac_batch *h, const uint64_t *rng_states, const dea
     std::map<std::tuple<int, int, int, int>, int> shapes;
     for (size_t i = 0; i < n; i++) {
         const BatchModel &b = *h->models[h->model_of[i]];
-        shapes[{options[i].ks_precision, options[i].ks_precision == DEAC_PRECISION_FP64 ? b.NT : b.NQ, b.NM,
-                options[i].ks_precision == DEAC_PRECISION_FP64 && b.nrem > 0 ? 1 : 0}] = 1;
+        if (b.m.bdsf)
+            shapes[{3, b.MF, b.NM, 0}] = 1;  // the fold kernel
+        else
+            shapes[{options[i].ks_precision, options[i].ks_precision == DEAC_PRECISION_FP64 ? b.NT : b.NQ, b.NM,
+                    options[i].ks_precision == DEAC_PRECISION_FP64 && b.nrem > 0 ? 1 : 0}] = 1;
     }
     const bool several_groups = shapes.size() > 1, several_models = h->models.size() > 1;
     std::vector<deac_batch_schedule> cand;

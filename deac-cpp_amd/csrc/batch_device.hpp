@@ -77,6 +77,7 @@ constexpr int kMaxT = 4 * 4 * 16 + kBatchMaxRem;  // timeslices: 16 full tiles (
 
 enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3, kPassNorm = 4, kPassNone = 5 };
 enum { kPrecFp64 = 0, kPrecFp32 = 1, kPrecBf16 = 2 };  // DEAC_PRECISION_*
+enum { kPrecFold = 3 };  // FP64 throughout, the trials' contraction a fold of rounded products (bdsf)
 constexpr int kEk = (kMaxT * kBatchMixedMaxP + kBatchThreads - 1) / kBatchThreads;  // chi^2 terms per thread (mixed)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -115,6 +116,7 @@ struct BatchShared {
     double y0t[kBatchMaxP], y0r[kBatchMaxP];  // trial value at position 0 (zero-frequency match)
     double mom[kBatchMaxP][2];                // first and third moments of the pass's genomes (FP64 kernel)
     double norm[kBatchMaxP];                  // zeroth moment of each trial (renormalization)
+    double nfm[kBatchMaxP];                   // -1st moment of each trial's model ISF (bdsf)
     double rscale[2][kBatchMaxP];             // renormalization factor of each trial, by generation parity
     int accepted[3][kBatchMaxP];
     int flag;                                 // 0 evolve, 1 stop test fired, 2 end of the launch
@@ -694,6 +696,89 @@ __device__ __forceinline__ void mixed_chains(const BatchDesc &d, BatchShared &sh
     }
 }
 
+// Chain waves of the detailed-balance (bdsf) kernel: the trials' model ISF
+// as the reference's detailed-balance build sums it (its loop inlines
+// matrix_multiply_LxM_by_MxN, deac.cpp:110-118, 1389-1390): per (tau,
+// member) a fold of rounded products in position order, the last position
+// fused when the genome length is odd.  The 256 chain lanes are slots s =
+// 64 wave + lane: timeslice s % T of member group s / T, which takes members
+// q MPG .. q MPG + MPG - 1 (MF accumulators, the ones past MPG or P fold a
+// clamped member and are discarded).  Per position a lane reads K^T[k][tau]
+// (consecutive lanes, consecutive addresses) and each of its members' trial
+// value from the trial buffer (one address per group); the next eight K
+// values load while eight positions are folded.
+template <int MF, int NM>
+__device__ __forceinline__ void fold_chains(const BatchDesc &d, double *lds, bool chains, int nchunk, int nsteps,
+                                            int wave, int lane)
+{
+    constexpr int C = 128 / NM, Ppad = 16 * NM;
+    const int T = d.T, L = d.L, mpg = d.fold_members;
+    const int slot = wave * kWave + lane;
+    const int grp = slot / T, t = slot - grp * T;
+    const bool on = chains && grp < d.fold_groups;
+    int mr[MF];
+#pragma unroll
+    for (int j = 0; j < MF; j++) mr[j] = min(grp * mpg + j, d.P - 1);
+    double acc[MF];
+#pragma unroll
+    for (int j = 0; j < MF; j++) acc[j] = 0.0;
+    const gcdouble *kcol = (const gcdouble *)d.kfold + (on ? t : 0);  // K^T[k][t] at kcol[k T]
+    const bool lodd = (L & 1) != 0;
+    const int nwork = chains ? nchunk : 0;
+    for (int c = 0; c < min(4, nsteps); c++) step_barrier();
+    for (int cm = 0; cm < nwork; cm++) {
+        if (on) {
+            const double *tb = lds + kOffTrial + (cm & 1) * kCD;
+            const int k0 = cm * C, n = min(C, L - k0);
+            if (n == C && !(lodd && k0 + C == L)) {
+                double kn[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) kn[u] = kcol[static_cast<size_t>(k0 + u) * T];
+                for (int s0 = 0; s0 < C; s0 += 8) {
+                    double kv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) kv[u] = kn[u];
+                    if (s0 + 8 < C) {
+#pragma unroll
+                        for (int u = 0; u < 8; u++) kn[u] = kcol[static_cast<size_t>(k0 + s0 + 8 + u) * T];
+                    }
+#pragma unroll
+                    for (int j = 0; j < MF; j++) {
+                        double sv[8];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) sv[u] = tb[trial_index<NM>(mr[j], s0 + u)];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) acc[j] = acc[j] + sv[u] * kv[u];
+                    }
+                }
+            } else {
+                // the last chunk: partial, or holding the fused last position
+                for (int sl = 0; sl < n; sl++) {
+                    const int k = k0 + sl;
+                    const double kv = kcol[static_cast<size_t>(k) * T];
+                    const bool f = lodd && k == L - 1;
+#pragma unroll
+                    for (int j = 0; j < MF; j++) {
+                        const double sv = tb[trial_index<NM>(mr[j], sl)];
+                        acc[j] = f ? fma(sv, kv, acc[j]) : acc[j] + sv * kv;
+                    }
+                }
+            }
+        }
+        step_barrier();
+    }
+    for (int c = min(4, nsteps) + nwork; c < nsteps; c++) step_barrier();
+    __syncthreads();
+    // E[t][m] into LDS (the chunk buffers are free now)
+    if (on) {
+#pragma unroll
+        for (int j = 0; j < MF; j++) {
+            const int m = grp * mpg + j;
+            if (j < mpg && m < d.P) lds[t * Ppad + m] = acc[j];
+        }
+    }
+}
+
 // One pass over the genome in chunks.  kind: kPassInit (model ISF of the
 // stored population), kPassEvolve (generation g), kPassApply (write back the
 // genomes accepted in generation g), kPassPre (masks only), kPassNorm
@@ -712,11 +797,14 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
     constexpr int KD = NT == 1 ? (REM ? 4 : 8) : (NT == 2 ? 4 : 2);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     const int nchunk = d.nchunk;
+    constexpr bool F64 = PREC == kPrecFp64 || PREC == kPrecFold;  // FP64 trial values
     const bool chains = kind == kPassInit || kind == kPassEvolve;
     const int nsteps = kind == kPassApply ? nchunk + 2 : kind == kPassNone ? 0 : nchunk + 4;
 
     if (wave < kWorker0) {
-        if constexpr (PREC != kPrecFp64) {
+        if constexpr (PREC == kPrecFold) {
+            fold_chains<NT, NM>(d, lds, chains, nchunk, nsteps, wave, lane);  // NT: accumulators per lane
+        } else if constexpr (PREC != kPrecFp64) {
             mixed_chains<PREC, NT, NM>(d, sh, lds, kind, g, chains, nchunk, nsteps, wave, lane);
         } else {
             // ------------------------------------------------ chain waves
@@ -879,7 +967,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         // fabs of a DE value, or not (allow_negative): an and of the high word
         const uint32_t absmask = __builtin_amdgcn_readfirstlane(d.allow_neg) ? 0xffffffffu : 0x7fffffffu;
         constexpr int kWpc = C / 32;  // mask words per member and chunk
-        const bool renorm = PREC == kPrecFp64 && d.renorm;
+        const bool renorm = F64 && d.renorm;
         const bool trials = kind == kPassEvolve || kind == kPassNorm;
         const bool rebuild = kind == kPassApply || kind == kPassNorm || (kind == kPassEvolve && !renorm);
         const long pg = trials ? g - 1 : g;                   // generation whose accepted trials are rebuilt
@@ -1048,7 +1136,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         for (int c = 0; c < nsteps; c++) {
             // (the mixed kernel's chain waves stage the chunks instead: Stager;
             // in the FP64 kernel their registers hold the kernel-tile ring)
-            const bool ld = PREC == kPrecFp64 && kind != kPassPre && c < nchunk;
+            const bool ld = F64 && kind != kPassPre && c < nchunk;
             if (ld) {
                 const gchar *pb = (const gchar *)popp + static_cast<size_t>(c) * C * 8;
 #pragma unroll
@@ -1087,7 +1175,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                         v[u].y = sp + 1 < nL ? x.y : 0.0;
                     }
                 } else {
-                    de_stage(buf, sh.mring[ct % 4], mo, od, g, ct, sh.y0t, v, PREC != kPrecFp64);
+                    de_stage(buf, sh.mring[ct % 4], mo, od, g, ct, sh.y0t, v, !F64);
                     if (renorm && kind == kPassEvolve) {
                         // the trial times zeroth_moment / norm (deac.cpp:1353-1360)
 #pragma unroll
@@ -1098,7 +1186,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                         }
                     }
                 }
-                if constexpr (PREC == kPrecFp64) {
+                if constexpr (F64) {
 #pragma unroll
                     for (int u = 0; u < kT; u++) {
                         const int t0 = NM >= 2 ? td0 + 32 * NM * u : td0 ^ (8 * u);  // position p
@@ -1195,8 +1283,8 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         // the weights of chunk c are the same for every member; lane l loads
         // positions l, l + 64 of the chunk a step ahead, and the fold reads
         // them back with v_readlane.
-        const bool moments = PREC == kPrecFp64 && chains && (d.use_first || d.use_third);
-        const bool norms = PREC == kPrecFp64 && kind == kPassNorm;
+        const bool moments = F64 && chains && (d.use_first || d.use_third);
+        const bool norms = F64 && kind == kPassNorm;
         const bool folds = moments || norms;
         const gcdouble *const mt = (const gcdouble *)uniform_ptr(d.mterm);
         const int nL = __builtin_amdgcn_readfirstlane(d.L), nG = __builtin_amdgcn_readfirstlane(d.G);
@@ -1294,6 +1382,7 @@ template <int PREC, int NT, int NM, int REM>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *__restrict__ descs)
 {
     constexpr int Ppad = 16 * NM;
+    constexpr bool F64 = PREC == kPrecFp64 || PREC == kPrecFold;  // FP64 trial values
     extern __shared__ __attribute__((aligned(16))) double lds[];  // 16-byte aligned: ds_*_b128 at full rate
     __shared__ BatchShared sh;
     const BatchDesc &d = descs[blockIdx.x];
@@ -1309,7 +1398,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
     for (int k = threadIdx.x; k < d.T; k += kBatchThreads) {
         sh.obs[k] = obs[k];
         sh.sig[k] = sig[k];
-        sh.kscale[k] = PREC == kPrecFp64 ? 1.0 : ((const gcdouble *)d.kscale)[k];
+        sh.kscale[k] = F64 ? 1.0 : ((const gcdouble *)d.kscale)[k];
     }
     const bool evaluate = d.mode == 1;  // fitness (and epar) of the stored population only
     for (int k = threadIdx.x; k < 2 * kCD; k += kBatchThreads) lds[kOffTrial + k] = 0.0;  // padded members stay 0
@@ -1366,7 +1455,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         // per-generation renormalization (deac.cpp:1342-1364): the trials'
         // zeroth moments first, then the pass that scales and contracts them
         // (one call site of the pass, so its code appears once)
-        const bool norm = PREC == kPrecFp64 && d.renorm && kind == kPassEvolve && !normed;
+        const bool norm = F64 && d.renorm && kind == kPassEvolve && !normed;
         // a resumed launch has gen0's masks on the device and walked gen0 + 1
         // in the prologue: its mask pass has no steps
         const bool none = kind == kPassPre && d.resume && !evaluate;
@@ -1393,7 +1482,22 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
             // kept in registers until the selection decides which replace
             // their parent's.
             double ek[kEk];
-            if constexpr (PREC == kPrecFp64) {
+            if (F64 && d.use_nfm) {
+                // -1st-moment term from the model ISF, before the residuals
+                // replace it (deac.cpp:961-965, 1543-1547): a fold of rounded
+                // products over tau, the last fused when T is odd
+                if (wave == kWorker0 && lane < P) {
+                    const gcdouble *dt = (const gcdouble *)d.nfm_dt;
+                    double mm = 0.0;
+                    for (int t = 0; t < d.T; t++) {
+                        const double e = lds[t * Ppad + lane];
+                        mm = ((d.T & 1) && t == d.T - 1) ? fma(e, dt[t], mm) : mm + e * dt[t];
+                    }
+                    sh.nfm[lane] = mm;
+                }
+                __syncthreads();
+            }
+            if constexpr (F64) {
                 for (int k = threadIdx.x; k < d.T * Ppad; k += kBatchThreads) {
                     const int t = k / Ppad;
                     if (k % Ppad < P) {
@@ -1427,6 +1531,10 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
                         chi = t == tlast ? fma(v, v, chi) : chi + v;
                     }
                     double fit = chi / static_cast<double>(d.T);
+                    if (F64 && d.use_nfm) {  // deac.cpp:961-965, 1543-1547
+                        const double q = (d.nfm_value - sh.nfm[lane]) / d.nfm_error;
+                        fit = fma(q, q, fit);
+                    }
                     if (d.use_first) {  // deac.cpp:969, 1551
                         const double dm = sh.mom[lane][0] - d.first_moment;
                         fit += (dm * dm) / d.first_moment;
@@ -1461,7 +1569,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
                 }
                 return;
             }
-            if constexpr (PREC != kPrecFp64) {
+            if constexpr (!F64) {
                 // accepted trials' model ISF replaces their parent's
 #pragma unroll
                 for (int j = 0; j < kEk; j++) {

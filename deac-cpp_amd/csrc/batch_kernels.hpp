@@ -35,6 +35,8 @@ struct BatchDesc {
     const double *obs, *sig;   // [T] ISF and its error
     const double *mterm;       // [3][Lpad] first-moment, third-moment and normalization weights (zero past L),
                                // or null (FP64 kernel)
+    const double *kfold;       // bdsf fold kernel: K transposed, [Lpad][T] (zero past L)
+    const double *nfm_dt;      // [T] trapezoid weights of the -1st-moment term (use_nfm)
     const uint64_t *walk_rows; // [256][4] rows of T^L, T the xoshiro transition
     const uint64_t *sub_tab;   // [S-1][64][16][4] nibble tables of T^(k*Lsub), k = 1..S-1
     // state of Q instances of one model stacked along the members: member m
@@ -59,6 +61,7 @@ struct BatchDesc {
     double tau_cr, tau_f;
     double first_moment, third_moment, third_moment_error;  // used when use_first / use_third
     double zeroth;             // zeroth moment the trials are renormalized to (renorm)
+    double nfm_value, nfm_error;  // -1st-moment term (use_nfm): target and error
     uint64_t mod_magic;        // floor((2^64-1)/Pi)
     int init;                  // evaluate the initial population first (gen0 == 0; FP64 kernel only)
     int resume;                // the previous launch of this instance (Q = 1) ended at gen0 and left the
@@ -71,11 +74,22 @@ struct BatchDesc {
     int Q, Pi;                 // instances and members per instance
     int use_first, use_third;  // moment constraints (deac.cpp:1420-1477, 1550-1555), FP64 kernel only
     int renorm;                // renormalize each generation's trials (deac.cpp:1342-1364), FP64 kernel only
+    int use_nfm;               // -1st-moment term in the fitness (bdsf above zero temperature, deac.cpp:961-965)
+    int fold_groups, fold_members;  // bdsf fold kernel: member groups over the 256 chain lanes (lane slot s:
+                                    // group s / T, timeslice s % T) and members per group
 };
 
 // Exact FP64 kernel.  NT: 16-row kernel tiles per chain wavefront (1, 2 or 4); NM: 16-member
 // tiles (1, 2, or 4 with NT <= 2); rem: 1 when the kernel has 1..4 rows past its last full tile.
 hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, int rem, hipStream_t s);
+
+// The detailed-balance (bdsf) kernel: the trials' model ISF as a fold of rounded products
+// on the FP64 VALU (the reference build's loop contraction), everything else as the FP64
+// kernel; never evaluates a population from scratch (the FP64 kernel's evaluation launch
+// gives the initial fitness).  mf: accumulators per lane, one of kBatchFoldMf, at least
+// fold_members; nm as above.
+constexpr int kBatchFoldMf[] = {4, 8, 16};
+hipError_t launch_batch_fold(const BatchDesc *descs, int count, int mf, int nm, hipStream_t s);
 
 // Device scratch of deac_batch_run_noalloc: a file-scope array of kBatchScratchDoubles
 // doubles per GPU (32 MiB), its address looked up once per device with that device

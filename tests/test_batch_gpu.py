@@ -23,7 +23,7 @@ def batch_compatible(case):
     # the complete 200000-generation config-2 run is the plain engine's case
     # (one instance is a single workgroup here); config 4's long seeds run
     # inside a 256-seed batch below
-    return a["P"] <= 32 and a["N"] <= 20000 and m["spectra"] != "bdsf"
+    return a["P"] <= 32 and a["N"] <= 20000
 
 
 def _engine_problem(case):

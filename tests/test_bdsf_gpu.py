@@ -7,8 +7,9 @@ reference CPU build -- the trial population's model ISF summed as a fold of
 rounded products.  Its zero-temperature build is the only zero-temperature
 build that compiles.  The bdsf_* and zerot_* goldens come from those builds;
 tests/test_gpu_parity.py and tests/test_split_gpu.py run them through
-deac_run bit for bit.  Here: the command line against the same goldens, the
-batch's refusal and the per-seed fallback, and a mixed-precision run.
+deac_run bit for bit.  Here: the command line against the same goldens, seed
+batches on the batch kernel's fold contraction against the oracle, and a
+mixed-precision run.
 """
 import os
 import subprocess
@@ -72,20 +73,17 @@ def test_cli_bdsf_matches_reference_golden(tmp_path, name):
     assert not errors, f"{name}: " + "; ".join(errors)
 
 
-def test_bdsf_seed_batch_runs_seed_by_seed(tmp_path):
-    # the batch kernel refuses bdsf; deac.e runs the seeds one after another,
-    # each equal to its own reference run
-    print("[gpu] bdsf refused by the batch; deac.e seeds one after another", flush=True)
-    case = gc.load("bdsf_normalization_odd_T7_P5_G9")  # no moment constraints: refused for bdsf itself
+def test_bdsf_seed_batch_on_the_fold_kernel(tmp_path):
+    # deac.e batches bdsf seeds (the batch kernel's fold contraction), each
+    # equal to its own reference run
+    print("[gpu] bdsf seeds batched by deac.e", flush=True)
+    case = gc.load("bdsf_normalization_odd_T7_P5_G9")
     a = case["meta"]["args"]
     p = da.make_problem(case["tau"], case["isf"], case["err"], gc.frequency_of(case, da.frequency_grid), a["T"],
                         **gc.problem_kwargs(case))
-    with pytest.raises(da.DeacError, match="bdsf"):
-        da.Batch([p, p])
     r = _cli(case, tmp_path, a["seed"], ["--number_of_seeds", "2"])
     assert r.returncode == 0, r.stdout
-    assert "seeds run one after another" in r.stdout
-    assert "batched runs do not take the detailed-balance spectra (bdsf)" in r.stdout
+    assert "seeds run one after another" not in r.stdout, r.stdout
     errors = gc.compare_outputs(_files(tmp_path, "deac-bdsf", a["seed"]), case)
     assert not errors, "; ".join(errors)
     with da.Engine(p) as e:
@@ -114,3 +112,67 @@ def test_bdsf_mixed_precision_split_loop(prec):
     assert chi[0] == got["minimum_fitness"]
     tol = {"fp32": 1e-4, "bf16x3": 1e-3}[prec]
     assert abs(got["minimum"][0] - ref["minimum"][0]) <= tol * ref["minimum"][0]
+
+
+# bdsf batches against the oracle at shapes beyond the goldens: the fold
+# kernel's lane layout (256 / T member groups), odd genome lengths (the fused
+# last position), the -1st-moment term, moments, zero temperature, launch
+# slices, uneven generation counts and a stop inside the batch
+FOLD = {
+    "standard_T65_G513_P32": dict(T=65, G=513, temp=1.0, kw=dict(population_size=32, use_negative_first_moment=True)),
+    "hyperbolic_T33_G300_P12": dict(T=33, G=300, temp=1.2, kw=dict(model="hyperbolic", population_size=12,
+                                                                    use_negative_first_moment=True,
+                                                                    first_moment=0.6, third_moment=3.0,
+                                                                    third_moment_error=0.5)),
+    "zeroT_normalization_T17_G97_P8": dict(T=17, G=97, temp=0.0, kw=dict(model="normalization", population_size=8,
+                                                                          zero_temperature=True, normalize=True)),
+    "normalization_T100_G64_P20": dict(T=100, G=64, temp=0.8, kw=dict(model="normalization", population_size=20,
+                                                                      use_negative_first_moment=True)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FOLD))
+@pytest.mark.parametrize("slice_len", [None, "9"])
+def test_bdsf_batch_matches_oracle(name, slice_len, monkeypatch):
+    import oracle_binding as ob
+    spec = FOLD[name]
+    print(f"[gpu] bdsf batch {name} slice={slice_len}", flush=True)
+    if slice_len:
+        monkeypatch.setenv("DEAC_BATCH_SLICE", slice_len)
+    T = spec["T"]
+    beta = 1.0 / spec["temp"] if spec["temp"] > 0 else 1.0
+    tau = np.linspace(0.0, beta / 2, T)
+    isf = 0.7 * np.exp(-1.3 * tau) + 0.3 * np.exp(-0.2 * tau)
+    err = 2e-3 * isf
+    freq = da.frequency_grid(spec["G"], 16.0)
+    prob = da.make_problem(tau, isf, err, freq, spec["temp"], spectra_type="bdsf", **spec["kw"])
+    seeds, Ns = [3, 4, 5], [90, 61, 90]
+
+    def oracle(sd, N, stop):
+        return ob.run(ob.make_problem(tau, isf, err, ob.frequency_grid(spec["G"], 16.0), spec["temp"],
+                                      spectra_type="bdsf", number_of_generations=N, stop_minimum_fitness=stop,
+                                      **spec["kw"]), sd)
+    stops = [0.0, 0.0, float(oracle(5, 90, 0.0)["minimum"][40])]
+    with da.Batch([prob] * 3) as b:
+        outs = b.run(seeds, Ns, stops, True)
+    for sd, N, st, o in zip(seeds, Ns, stops, outs):
+        r = oracle(sd, N, st)
+        for k in ("minimum", "mean", "squared_mean", "dsf", "frequency"):
+            assert gc.bit_diff(o[k], r[k]) is None, (sd, k, gc.bit_diff(o[k], r[k]))
+        assert o["rng"] == r["rng"] and o["generation"] == r["generation"], sd
+
+
+def test_bdsf_batch_continues_and_refuses_mixed_precision():
+    case = gc.load("bdsf_standard")
+    a = case["meta"]["args"]
+    p = da.make_problem(case["tau"], case["isf"], case["err"], gc.frequency_of(case, da.frequency_grid), a["T"],
+                        **gc.problem_kwargs(case))
+    with da.Batch([p, p]) as b:
+        whole = b.run([7, 8], 80, 0.0, True)
+        first = b.run([7, 8], 31, 0.0, True)
+        rest = b.continue_run(50, 0.0, True)
+        with pytest.raises(da.DeacError, match="detailed-balance"):
+            b.run([7, 8], 10, 0.0, precision="fp32")
+    for w, f, r in zip(whole, first, rest):
+        assert gc.bit_diff(np.concatenate([f["minimum"], r["minimum"][1:]]), w["minimum"]) is None
+        assert gc.bit_diff(r["dsf"], w["dsf"]) is None and r["rng"] == w["rng"]
