@@ -45,7 +45,7 @@ struct BatchModel {
     double *ktile = nullptr, *krem = nullptr, *obs = nullptr, *sig = nullptr;
     double *kfold = nullptr;               // bdsf: K transposed [Lpad][T] for the fold kernel
     double *nfm_dt = nullptr;              // bdsf: -1st-moment weights [T], when used
-    int fold_groups = 0, fold_members = 0, MF = 0;  // bdsf fold kernel layout (BatchDesc)
+    int fold_rows = 0, MF = 0;             // bdsf fold kernel: timeslices per chain wave, accumulators
     double *mterm = nullptr;               // [3][Lpad] first-moment, third-moment and normalization weights,
                                            // when moments or per-generation renormalization are used
     uint64_t *walk_rows = nullptr, *sub_tab = nullptr;
@@ -171,14 +171,15 @@ deac_status plan_model(BatchModel &b, std::string &why)
     const int T = static_cast<int>(m.T), L = static_cast<int>(m.Kc);
     if (m.bdsf) {
         // the detailed-balance build sums the trials' model ISF as a fold of
-        // rounded products, which the fold kernel forms on the VALU: one
-        // lane per (timeslice, member group) of the 256 chain lanes, at most
-        // 16 members per group
-        b.fold_groups = T <= 256 ? 256 / T : 0;
-        b.fold_members = b.fold_groups ? (static_cast<int>(m.P) + b.fold_groups - 1) / b.fold_groups : 0;
-        b.MF = b.fold_members == 0 ? 0 : b.fold_members <= 4 ? 4 : b.fold_members <= 8 ? 8 : b.fold_members <= 16 ? 16 : 0;
+        // rounded products, which the fold kernel forms on the VALU: lane m
+        // folds member m, each chain wave a quarter of the timeslices, one
+        // accumulator each
+        b.fold_rows = (T + 3) / 4;
+        b.MF = 0;
+        for (int mf : deac::kBatchFoldMf)
+            if (b.MF == 0 && mf >= b.fold_rows) b.MF = mf;
         if (b.MF == 0) {
-            why = "batched detailed-balance (bdsf) runs take at most 16 * floor(256 / T) members and T <= 256";
+            why = "batched detailed-balance (bdsf) runs take at most 128 timeslices";
             return DEAC_ERR_UNSUPPORTED;
         }
     }
@@ -355,10 +356,13 @@ deac_status upload_model(deac_batch *h, BatchModel &b)
     up(b.sub_tab, tabs.data(), tabs.size() * sizeof(uint64_t));
     if (b.mterm) up(b.mterm, mt.data(), mt.size() * sizeof(double));
     if (m.bdsf && st == DEAC_OK) {
-        std::vector<double> kf(static_cast<size_t>(b.Lpad) * T, 0.0);
+        // [Lpad][4 chain waves][fold_rows]: row r of wave w is timeslice w + 4 r
+        const int rw = b.fold_rows;
+        std::vector<double> kf(static_cast<size_t>(b.Lpad) * 4 * rw, 0.0);
         for (int t = 0; t < T; t++)
-            for (int k = 0; k < L; k++) kf[static_cast<size_t>(k) * T + t] = m.kernel[static_cast<size_t>(t) * L + k];
-        A(b.kfold, kf.size());
+            for (int k = 0; k < L; k++)
+                kf[(static_cast<size_t>(k) * 4 + t % 4) * rw + t / 4] = m.kernel[static_cast<size_t>(t) * L + k];
+        A(b.kfold, kf.size() + 32);  // the fold kernel reads 32 rows from the last position's
         if (m.use_nfm) A(b.nfm_dt, m.T);
         up(b.kfold, kf.data(), kf.size() * sizeof(double));
         if (b.nfm_dt) up(b.nfm_dt, m.nfm_dt.data(), m.T * sizeof(double));
@@ -823,8 +827,7 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                 d.third_moment = m.third_moment;
                 d.third_moment_error = m.third_moment_error;
                 d.kfold = b.kfold;
-                d.fold_groups = b.fold_groups;
-                d.fold_members = b.fold_members;
+                d.fold_rows = b.fold_rows;
                 d.use_nfm = m.use_nfm ? 1 : 0;
                 d.nfm_dt = b.nfm_dt;
                 d.nfm_value = m.nfm_value;

@@ -700,68 +700,74 @@ __device__ __forceinline__ void mixed_chains(const BatchDesc &d, BatchShared &sh
 // as the reference's detailed-balance build sums it (its loop inlines
 // matrix_multiply_LxM_by_MxN, deac.cpp:110-118, 1389-1390): per (tau,
 // member) a fold of rounded products in position order, the last position
-// fused when the genome length is odd.  The 256 chain lanes are slots s =
-// 64 wave + lane: timeslice s % T of member group s / T, which takes members
-// q MPG .. q MPG + MPG - 1 (MF accumulators, the ones past MPG or P fold a
-// clamped member and are discarded).  Per position a lane reads K^T[k][tau]
-// (consecutive lanes, consecutive addresses) and each of its members' trial
-// value from the trial buffer (one address per group); the next eight K
-// values load while eight positions are folded.
+// fused when the genome length is odd.  Lane m folds member m; chain wave w
+// the timeslices w, w + 4, .. (row r: tau = w + 4 r, MF accumulators, rows
+// past T discarded).  K comes in blocks of 8 positions, lane r holding row
+// r's values (kfold: [Lpad][4 waves][fold_rows]), loaded a block ahead; a
+// position's rows are read back with v_readlane into scalar registers, all
+// of them before the position's products, so that a lane's products take
+// the trial value of its own member (one LDS read per position) and a
+// wave-uniform K value.
 template <int MF, int NM>
 __device__ __forceinline__ void fold_chains(const BatchDesc &d, double *lds, bool chains, int nchunk, int nsteps,
                                             int wave, int lane)
 {
     constexpr int C = 128 / NM, Ppad = 16 * NM;
-    const int T = d.T, L = d.L, mpg = d.fold_members;
-    const int slot = wave * kWave + lane;
-    const int grp = slot / T, t = slot - grp * T;
-    const bool on = chains && grp < d.fold_groups;
-    int mr[MF];
-#pragma unroll
-    for (int j = 0; j < MF; j++) mr[j] = min(grp * mpg + j, d.P - 1);
+    const int T = d.T, L = d.L, rw = d.fold_rows;
+    const int m = min(lane, d.P - 1);  // lanes past P fold a clamped member, discarded
     double acc[MF];
 #pragma unroll
-    for (int j = 0; j < MF; j++) acc[j] = 0.0;
-    const gcdouble *kcol = (const gcdouble *)d.kfold + (on ? t : 0);  // K^T[k][t] at kcol[k T]
+    for (int r = 0; r < MF; r++) acc[r] = 0.0;
+    // lane r's column of this wave's rows: K[w + 4 r][k] at kw[k * 4 * rw]
+    const gcdouble *kw = (const gcdouble *)d.kfold + wave * rw + min(lane, rw - 1);
+    const size_t kstride = static_cast<size_t>(4) * rw;
     const bool lodd = (L & 1) != 0;
     const int nwork = chains ? nchunk : 0;
     for (int c = 0; c < min(4, nsteps); c++) step_barrier();
     for (int cm = 0; cm < nwork; cm++) {
-        if (on) {
-            const double *tb = lds + kOffTrial + (cm & 1) * kCD;
-            const int k0 = cm * C, n = min(C, L - k0);
-            if (n == C && !(lodd && k0 + C == L)) {
-                double kn[8];
+        const double *tb = lds + kOffTrial + (cm & 1) * kCD;
+        const int k0 = cm * C, n = min(C, L - k0);
+        if (n == C && !(lodd && k0 + C == L)) {
+            double kn[8], sn[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) kn[u] = kcol[static_cast<size_t>(k0 + u) * T];
-                for (int s0 = 0; s0 < C; s0 += 8) {
-                    double kv[8];
+            for (int u = 0; u < 8; u++) {
+                kn[u] = kw[(k0 + u) * kstride];
+                sn[u] = tb[trial_index<NM>(m, u)];
+            }
+            for (int s0 = 0; s0 < C; s0 += 8) {
+                double kb[8], sv[8];
 #pragma unroll
-                    for (int u = 0; u < 8; u++) kv[u] = kn[u];
-                    if (s0 + 8 < C) {
+                for (int u = 0; u < 8; u++) {
+                    kb[u] = kn[u];
+                    sv[u] = sn[u];
+                }
+                if (s0 + 8 < C) {
 #pragma unroll
-                        for (int u = 0; u < 8; u++) kn[u] = kcol[static_cast<size_t>(k0 + s0 + 8 + u) * T];
-                    }
-#pragma unroll
-                    for (int j = 0; j < MF; j++) {
-                        double sv[8];
-#pragma unroll
-                        for (int u = 0; u < 8; u++) sv[u] = tb[trial_index<NM>(mr[j], s0 + u)];
-#pragma unroll
-                        for (int u = 0; u < 8; u++) acc[j] = acc[j] + sv[u] * kv[u];
+                    for (int u = 0; u < 8; u++) {
+                        kn[u] = kw[(k0 + s0 + 8 + u) * kstride];
+                        sn[u] = tb[trial_index<NM>(m, s0 + 8 + u)];
                     }
                 }
-            } else {
-                // the last chunk: partial, or holding the fused last position
-                for (int sl = 0; sl < n; sl++) {
-                    const int k = k0 + sl;
-                    const double kv = kcol[static_cast<size_t>(k) * T];
-                    const bool f = lodd && k == L - 1;
 #pragma unroll
-                    for (int j = 0; j < MF; j++) {
-                        const double sv = tb[trial_index<NM>(mr[j], sl)];
-                        acc[j] = f ? fma(sv, kv, acc[j]) : acc[j] + sv * kv;
-                    }
+                for (int u = 0; u < 8; u++) {
+                    double kr[MF];
+#pragma unroll
+                    for (int r = 0; r < MF; r++) kr[r] = readlane_f64(kb[u], r);
+#pragma unroll
+                    for (int r = 0; r < MF; r++) acc[r] = acc[r] + sv[u] * kr[r];
+                }
+            }
+        } else {
+            // the last chunk: partial, or holding the fused last position
+            for (int sl = 0; sl < n; sl++) {
+                const int k = k0 + sl;
+                const double kv = kw[k * kstride];
+                const double sv = tb[trial_index<NM>(m, sl)];
+                const bool f = lodd && k == L - 1;
+#pragma unroll
+                for (int r = 0; r < MF; r++) {
+                    const double a = readlane_f64(kv, r);
+                    acc[r] = f ? fma(sv, a, acc[r]) : acc[r] + sv * a;
                 }
             }
         }
@@ -770,11 +776,11 @@ __device__ __forceinline__ void fold_chains(const BatchDesc &d, double *lds, boo
     for (int c = min(4, nsteps) + nwork; c < nsteps; c++) step_barrier();
     __syncthreads();
     // E[t][m] into LDS (the chunk buffers are free now)
-    if (on) {
+    if (chains && lane < d.P) {
 #pragma unroll
-        for (int j = 0; j < MF; j++) {
-            const int m = grp * mpg + j;
-            if (j < mpg && m < d.P) lds[t * Ppad + m] = acc[j];
+        for (int r = 0; r < MF; r++) {
+            const int t = wave + 4 * r;
+            if (t < T) lds[t * Ppad + lane] = acc[r];
         }
     }
 }

@@ -25,9 +25,11 @@ hipError_t launch_fold_one(const BatchDesc *descs, int count, hipStream_t s)
 template <int NM>
 hipError_t launch_fold_mf(const BatchDesc *descs, int count, int mf, hipStream_t s)
 {
-    if (mf == 4) return launch_fold_one<4, NM>(descs, count, s);
     if (mf == 8) return launch_fold_one<8, NM>(descs, count, s);
     if (mf == 16) return launch_fold_one<16, NM>(descs, count, s);
+    if (mf == 17) return launch_fold_one<17, NM>(descs, count, s);
+    if (mf == 24) return launch_fold_one<24, NM>(descs, count, s);
+    if (mf == 32) return launch_fold_one<32, NM>(descs, count, s);
     return hipErrorInvalidValue;
 }
 

@@ -35,7 +35,8 @@ struct BatchDesc {
     const double *obs, *sig;   // [T] ISF and its error
     const double *mterm;       // [3][Lpad] first-moment, third-moment and normalization weights (zero past L),
                                // or null (FP64 kernel)
-    const double *kfold;       // bdsf fold kernel: K transposed, [Lpad][T] (zero past L)
+    const double *kfold;       // bdsf fold kernel: [Lpad][4 chain waves][fold_rows], entry (k, w, r) = K row
+                               // w + 4 r at position k (zero past T and L)
     const double *nfm_dt;      // [T] trapezoid weights of the -1st-moment term (use_nfm)
     const uint64_t *walk_rows; // [256][4] rows of T^L, T the xoshiro transition
     const uint64_t *sub_tab;   // [S-1][64][16][4] nibble tables of T^(k*Lsub), k = 1..S-1
@@ -75,8 +76,7 @@ struct BatchDesc {
     int use_first, use_third;  // moment constraints (deac.cpp:1420-1477, 1550-1555), FP64 kernel only
     int renorm;                // renormalize each generation's trials (deac.cpp:1342-1364), FP64 kernel only
     int use_nfm;               // -1st-moment term in the fitness (bdsf above zero temperature, deac.cpp:961-965)
-    int fold_groups, fold_members;  // bdsf fold kernel: member groups over the 256 chain lanes (lane slot s:
-                                    // group s / T, timeslice s % T) and members per group
+    int fold_rows;             // bdsf fold kernel: timeslices per chain wave, ceil(T / 4)
 };
 
 // Exact FP64 kernel.  NT: 16-row kernel tiles per chain wavefront (1, 2 or 4); NM: 16-member
@@ -87,8 +87,8 @@ hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, int r
 // on the FP64 VALU (the reference build's loop contraction), everything else as the FP64
 // kernel; never evaluates a population from scratch (the FP64 kernel's evaluation launch
 // gives the initial fitness).  mf: accumulators per lane, one of kBatchFoldMf, at least
-// fold_members; nm as above.
-constexpr int kBatchFoldMf[] = {4, 8, 16};
+// fold_rows; nm as above.
+constexpr int kBatchFoldMf[] = {8, 16, 17, 24, 32};
 hipError_t launch_batch_fold(const BatchDesc *descs, int count, int mf, int nm, hipStream_t s);
 
 // Device scratch of deac_batch_run_noalloc: a file-scope array of kBatchScratchDoubles

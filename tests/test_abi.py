@@ -129,7 +129,7 @@ def _batch_problem(T=17, **kw):
 @pytest.mark.parametrize("kw,status,text", [
     (dict(population_size=70), 5, "at most 64 members"),
     (dict(population_size=40, T=140), 5, "at most 132 timeslices"),
-    (dict(spectra_type="bdsf", T=130, population_size=20), 5, "detailed-balance (bdsf) runs take at most"),
+    (dict(spectra_type="bdsf", T=130, population_size=20), 5, "detailed-balance (bdsf) runs take at most 128"),
     (dict(population_size=2), 1, "population_size"),
 ])
 def test_batch_create_rejects_before_touching_a_device(kw, status, text):
