@@ -259,6 +259,29 @@ def small_population_batch(da, gens):
     return out
 
 
+def detailed_balance_batch(da, gens):
+    """The detailed-balance variant (bdsf, with the -1st-moment term) on the
+    seed-batch shape: 256 seeds of G=4096, T=65, P=32 for `gens` generations
+    on the batch's fold kernel, against one seed on the plain path."""
+    t, f, s = load_isf()
+    prob = da.make_problem(t, f, s, da.frequency_grid(G, OMEGA_MAX), TEMPERATURE, population_size=32,
+                           normalize=True, spectra_type="bdsf", use_negative_first_moment=True)
+    n = 256
+    seeds = list(range(1, n + 1))
+    with da.Batch([prob] * n) as b:
+        b.run(seeds, 11, 0.0, track_stats=False)
+        b.continue_run(gens + 1, 0.0, track_stats=False)
+        prof = b.profile
+    batch_ms = prof["loop_ms"] / gens
+    with da.Engine(prob) as e:
+        e.run(1, 11, 0.0, track_stats=False)
+        out = e.run(1, gens + 1, 0.0, track_stats=False)
+    plain_ms = out["profile"]["loop_ms"] / out["profile"]["generations"]
+    return {"workload": f"{n} seeds x (G=4096, T=65, P=32, bdsf, --normalize, -1st moment), FP64, {gens} generations",
+            "batch_ms_per_generation": batch_ms, "batch_genome_evals_per_s": n * 32 / (batch_ms / 1e3),
+            "plain_ms_per_generation_one_seed": plain_ms, "batch_over_seed_by_seed": n * plain_ms / batch_ms}
+
+
 def role_cycles(da, args):
     """Per-wavefront cycles of the batch kernels on the seed batch, one JSON
     line per precision: the first `--warmup` generations of a run, then
@@ -403,6 +426,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--uneven-gens", type=int, default=200, help="timed generations of the uneven batch (0: skip)")
     ap.add_argument("--no-precision-sweep", action="store_true")
+    ap.add_argument("--bdsf-gens", type=int, default=100,
+                    help="timed generations of the detailed-balance seed batch (0: skip)")
     ap.add_argument("--small-pop-gens", type=int, default=300,
                     help="timed generations of the 512-seed P=8 batch, stacked and not (0: skip)")
     ap.add_argument("--accuracy-gens", type=int, default=2000,
@@ -506,6 +531,7 @@ def main():
     accuracy = batch_precision_accuracy(da, prob, args.accuracy_gens) if args.accuracy_gens > 0 else None
     uneven = uneven_batch(da, args.uneven_gens) if args.uneven_gens > 0 else None
     small = small_population_batch(da, args.small_pop_gens) if args.small_pop_gens > 0 else None
+    bdsf = detailed_balance_batch(da, args.bdsf_gens) if args.bdsf_gens > 0 else None
     sweep = None if args.no_precision_sweep else precision_sweep(da)
     trajectories = precision_trajectories(da, args.trajectory_gens) if args.trajectory_gens > 0 else None
 
@@ -549,6 +575,7 @@ def main():
                    "batch_precision_accuracy": accuracy,
                    "uneven_batch": uneven,
                    "small_population_batch": small,
+                   "detailed_balance_batch": bdsf,
                    "precision_sweep": sweep,
                    "precision_trajectories": trajectories},
         "roofline": {"bound": "hbm", "achieved": evals_per_launch * bytes_per_eval / kern_s / 1e9,
