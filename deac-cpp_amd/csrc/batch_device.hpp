@@ -1,6 +1,9 @@
-// Batched DEAC on gfx950: one persistent workgroup per reconstruction.
-// Device code shared by the kernel's instantiations: batch_kernels.hip
-// (exact FP64) and batch_mixed_kernels.hip (mixed-precision K.S).
+// Batched DEAC on gfx950: one persistent workgroup per reconstruction (or
+// per up to four small-population reconstructions of one problem, stacked
+// along the member rows).  Device code shared by the kernel's
+// instantiations: batch_kernels.hip (exact FP64), batch_mixed_kernels.hip
+// (mixed-precision K.S) and batch_fold_kernels.hip (the detailed-balance
+// variant, whose trial contraction is a fold of rounded products).
 //
 // Each instance runs the reference generation loop (deac.cpp:1138-1573) with
 // the reference's random stream and FP64 evaluation order, so its results are
@@ -29,7 +32,8 @@
 // step and store the data into LDS at the bottom, so the loads fly while the
 // step computes.
 //
-// Population storage: [P][Lpad] member-major, one buffer.  A generation's
+// Population storage: [P][Lpad] member-major per instance (stacked
+// instances at byte offsets pop_row0[q] from one base).  A generation's
 // accepted trials are not written when they are selected; the next
 // generation recomputes them from the same inputs (stored genomes, masks, F,
 // donors) while its chunks pass through LDS and writes them over the chunk in
