@@ -43,9 +43,9 @@ struct BatchModel {
     int Ppad = 0, NM = 0, NT = 0, C = 0, nchunk = 0, Lpad = 0, Lw = 0, ntt = 0, nrem = 0, S = 0, Lsub = 0;
     int ntm = 0, NQ = 0;                   // mixed K.S: kernel tiles (rows past T zero), pairs per chain wave
     double *ktile = nullptr, *krem = nullptr, *obs = nullptr, *sig = nullptr;
-    double *kfold = nullptr;               // bdsf: K transposed [Lpad][T] for the fold kernel
+    double *kfold = nullptr;               // bdsf: K transposed [Lpad][fold_rows] for the fold kernel
     double *nfm_dt = nullptr;              // bdsf: -1st-moment weights [T], when used
-    int fold_rows = 0, MF = 0;             // bdsf fold kernel: timeslices per chain wave, accumulators
+    int fold_rows = 0, MF = 0;             // bdsf fold kernel: kfold row stride, shape 2 RL + XR
     double *mterm = nullptr;               // [3][Lpad] first-moment, third-moment and normalization weights,
                                            // when moments or per-generation renormalization are used
     uint64_t *walk_rows = nullptr, *sub_tab = nullptr;
@@ -169,22 +169,32 @@ deac_status plan_model(BatchModel &b, std::string &why)
         return DEAC_ERR_UNSUPPORTED;
     }
     const int T = static_cast<int>(m.T), L = static_cast<int>(m.Kc);
+    b.Ppad = m.P <= 16 ? 16 : m.P <= 32 ? 32 : 64;
+    b.NM = b.Ppad / 16;
     if (m.bdsf) {
         // the detailed-balance build sums the trials' model ISF as a fold of
-        // rounded products, which the fold kernel forms on the VALU: lane m
-        // folds member m, each chain wave a quarter of the timeslices, one
-        // accumulator each
-        b.fold_rows = (T + 3) / 4;
+        // rounded products, which the fold kernel forms on the VALU in
+        // register tiles: RL timeslices of one of 64 / NM row blocks per
+        // lane (R = 64 RL / NM), the smallest RL whose blocks hold T, or
+        // all but at most 4 timeslices, which the four chain waves then
+        // fold one each
         b.MF = 0;
-        for (int mf : deac::kBatchFoldMf)
-            if (b.MF == 0 && mf >= b.fold_rows) b.MF = mf;
+        for (int rl = 1; rl <= 8 && b.MF == 0 && T <= 128; rl *= 2) {
+            const int R = 64 / b.NM * rl;
+            if (R > 128) break;
+            if (T <= R) {
+                b.MF = 2 * rl;
+                b.fold_rows = R;
+            } else if (T - R <= 4) {
+                b.MF = 2 * rl + 1;
+                b.fold_rows = R + 4;
+            }
+        }
         if (b.MF == 0) {
             why = "batched detailed-balance (bdsf) runs take at most 128 timeslices";
             return DEAC_ERR_UNSUPPORTED;
         }
     }
-    b.Ppad = m.P <= 16 ? 16 : m.P <= 32 ? 32 : 64;
-    b.NM = b.Ppad / 16;
     b.C = deac::kBatchChunkDoubles / b.Ppad;
     b.nchunk = (L + b.C - 1) / b.C;
     b.Lpad = b.nchunk * b.C;
@@ -192,6 +202,12 @@ deac_status plan_model(BatchModel &b, std::string &why)
     // resource with 32-bit byte offsets
     if (static_cast<long>(b.Lpad) * 128 > INT32_MAX) {
         why = "batched runs take genomes of at most 16 million positions";
+        return DEAC_ERR_UNSUPPORTED;
+    }
+    // the fold kernel reads kfold ([Lpad][fold_rows] doubles) the same way
+    if (m.bdsf && static_cast<long>(b.Lpad) * b.fold_rows * 8 > INT32_MAX) {
+        why = "batched detailed-balance (bdsf) runs take genomes of at most " +
+              std::to_string(INT32_MAX / (8L * b.fold_rows)) + " positions at this number of timeslices";
         return DEAC_ERR_UNSUPPORTED;
     }
     b.Lw = b.Lpad / 32;
@@ -356,13 +372,13 @@ deac_status upload_model(deac_batch *h, BatchModel &b)
     up(b.sub_tab, tabs.data(), tabs.size() * sizeof(uint64_t));
     if (b.mterm) up(b.mterm, mt.data(), mt.size() * sizeof(double));
     if (m.bdsf && st == DEAC_OK) {
-        // [Lpad][4 chain waves][fold_rows]: row r of wave w is timeslice w + 4 r
+        // [Lpad][fold_rows]: the timeslices of a position contiguous
         const int rw = b.fold_rows;
-        std::vector<double> kf(static_cast<size_t>(b.Lpad) * 4 * rw, 0.0);
+        std::vector<double> kf(static_cast<size_t>(b.Lpad) * rw, 0.0);
         for (int t = 0; t < T; t++)
             for (int k = 0; k < L; k++)
-                kf[(static_cast<size_t>(k) * 4 + t % 4) * rw + t / 4] = m.kernel[static_cast<size_t>(t) * L + k];
-        A(b.kfold, kf.size() + 32);  // the fold kernel reads 32 rows from the last position's
+                kf[static_cast<size_t>(k) * rw + t] = m.kernel[static_cast<size_t>(t) * L + k];
+        A(b.kfold, kf.size());
         if (m.use_nfm) A(b.nfm_dt, m.T);
         up(b.kfold, kf.data(), kf.size() * sizeof(double));
         if (b.nfm_dt) up(b.nfm_dt, m.nfm_dt.data(), m.T * sizeof(double));

@@ -704,74 +704,143 @@ __device__ __forceinline__ void mixed_chains(const BatchDesc &d, BatchShared &sh
 // as the reference's detailed-balance build sums it (its loop inlines
 // matrix_multiply_LxM_by_MxN, deac.cpp:110-118, 1389-1390): per (tau,
 // member) a fold of rounded products in position order, the last position
-// fused when the genome length is odd.  Lane m folds member m; chain wave w
-// the timeslices w, w + 4, .. (row r: tau = w + 4 r, MF accumulators, rows
-// past T discarded).  K comes in blocks of 8 positions, lane r holding row
-// r's values (kfold: [Lpad][4 waves][fold_rows]), loaded a block ahead; a
-// position's rows are read back with v_readlane into scalar registers, all
-// of them before the position's products, so that a lane's products take
-// the trial value of its own member (one LDS read per position) and a
-// wave-uniform K value.
-template <int MF, int NM>
+// fused when the genome length is odd.  Register tiles of the outer product
+// K[t][k] S[m][k] per position: chain wave w takes members 4 NM w ..
+// 4 NM w + 4 NM - 1; lane l takes RL consecutive timeslices of row block
+// l % (64 / NM) and the 4 consecutive members of slot l / (64 / NM), RL x 4
+// accumulators.  Per position a lane loads its RL values of K (kfold
+// [Lpad][fold_rows], timeslices contiguous, D positions ahead, through a
+// buffer resource whose wave-uniform offset carries the position) and its
+// four trial values (two 16-byte LDS reads, the same address for every lane
+// of a slot).  XR: the timeslices past the row blocks, R .. T - 1 (at most
+// 4): wave w folds timeslice R + w, lane m member m, K loaded at one address
+// by all lanes.
+template <int RL, int XR, int NM>
 __device__ __forceinline__ void fold_chains(const BatchDesc &d, double *lds, bool chains, int nchunk, int nsteps,
                                             int wave, int lane)
 {
     constexpr int C = 128 / NM, Ppad = 16 * NM;
-    const int T = d.T, L = d.L, rw = d.fold_rows;
-    const int m = min(lane, d.P - 1);  // lanes past P fold a clamped member, discarded
-    double acc[MF];
+    constexpr int RQ = 64 / NM;               // row blocks per wave (lanes per member slot)
+    constexpr int R = RQ * RL;                // timeslices of the row blocks
+    constexpr int D = RL <= 2 ? 8 : 16 / RL;  // K prefetch distance in positions (divides 8)
+    constexpr int NB = (RL + 1) / 2;          // 16-byte K loads per position
+    const int T = d.T, L = d.L, P = d.P, tp = d.fold_rows;
+    const int rq = lane % RQ, ms = lane / RQ;
+    const int mb = wave * 4 * NM + 4 * ms;   // the lane's first member
+    double acc[RL][4];
 #pragma unroll
-    for (int r = 0; r < MF; r++) acc[r] = 0.0;
-    // lane r's column of this wave's rows: K[w + 4 r][k] at kw[k * 4 * rw]
-    const gcdouble *kw = (const gcdouble *)d.kfold + wave * rw + min(lane, rw - 1);
-    const size_t kstride = static_cast<size_t>(4) * rw;
+    for (int i = 0; i < RL; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    const rsrc_t rk = make_rsrc(d.kfold, d.Lpad * tp * 8);  // positions past Lpad read zeros
+    const uint32_t vo = rq * RL * 8;
+    const int pstride = tp * 8;               // bytes per position
+    // the extra timeslice R + wave: its K value at byte offset xo of a
+    // position, the same address in every lane
+    const int xo = (R + wave) * 8;
+    auto load_x = [&](int pos) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rk, 0u, pos * pstride + xo, 0));
+    };
+    const bool xv = XR && R + wave < T && lane < P;
+    const int xm = lane < Ppad ? lane : 0;
+    double accx = 0.0;
     const bool lodd = (L & 1) != 0;
     const int nwork = chains ? nchunk : 0;
+    // K of positions p .. p + D - 1 in flight (slot p % D)
+    double kb[D][RL], xb[D];
+    auto load_k = [&](double (&kv)[RL], int pos) {
+        if constexpr (RL == 1) {
+            kv[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rk, vo, pos * pstride, 0));
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const double2 v = buffer_load2(rk, vo + 16 * b, pos * pstride);
+                kv[2 * b] = v.x;
+                kv[2 * b + 1] = v.y;
+            }
+        }
+    };
+    if (nwork > 0) {
+#pragma unroll
+        for (int u = 0; u < D; u++) {
+            load_k(kb[u], u);
+            if constexpr (XR) xb[u] = load_x(u);
+        }
+    }
     for (int c = 0; c < min(4, nsteps); c++) step_barrier();
     for (int cm = 0; cm < nwork; cm++) {
         const double *tb = lds + kOffTrial + (cm & 1) * kCD;
         const int k0 = cm * C, n = min(C, L - k0);
         if (n == C && !(lodd && k0 + C == L)) {
-            double kn[8], sn[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                kn[u] = kw[(k0 + u) * kstride];
-                sn[u] = tb[trial_index<NM>(m, u)];
-            }
             for (int s0 = 0; s0 < C; s0 += 8) {
-                double kb[8], sv[8];
+                // the slot's four members sit in one aligned quad of each
+                // position's 16-member group: member mb + j at quad entry
+                // j ^ ((sl >> 1) & 3); the quad at (mb & 15) ^ ((sl >> 1) & 12),
+                // fixed across the 8 positions.  The extra timeslice's member
+                // xm at entry (xm & 15) ^ ((sl >> 1) & 15) of its group.
+                const double *qb = tb + (s0 >> 2) * NM * 64 + (mb >> 4) * 64 + ((mb & 15) ^ ((s0 >> 1) & 12));
+                const double *xq = tb + (s0 >> 2) * NM * 64 + (xm >> 4) * 64;
+                const int xs0 = (xm & 15) ^ ((s0 >> 1) & 12);
+                auto quad = [&](int u, double2 &a, double2 &b) {
+                    const double2 *q = reinterpret_cast<const double2 *>(qb + (u >> 2) * NM * 64 + (u & 3) * 16);
+                    a = q[0];
+                    b = q[1];
+                };
+                auto xval = [&](int u) { return xq[(u >> 2) * NM * 64 + (u & 3) * 16 + (xs0 ^ ((u >> 1) & 3))]; };
+                // the trial values of position u + 1 are read while position
+                // u's products run
+                double2 na, nb;
+                double nx = 0.0;
+                quad(0, na, nb);
+                if constexpr (XR) nx = xval(0);
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
-                    kb[u] = kn[u];
-                    sv[u] = sn[u];
-                }
-                if (s0 + 8 < C) {
-#pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        kn[u] = kw[(k0 + s0 + 8 + u) * kstride];
-                        sn[u] = tb[trial_index<NM>(m, s0 + 8 + u)];
+                    const int sl = s0 + u;
+                    const double2 a = na, b = nb;
+                    const double xs = nx;
+                    if (u < 7) {
+                        quad(u + 1, na, nb);
+                        if constexpr (XR) nx = xval(u + 1);
                     }
-                }
+                    const double v[4] = {a.x, a.y, b.x, b.y};
+                    const int sw = (u >> 1) & 3;
+                    double kc[RL];
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    double kr[MF];
+                    for (int i = 0; i < RL; i++) kc[i] = kb[u % D][i];
 #pragma unroll
-                    for (int r = 0; r < MF; r++) kr[r] = readlane_f64(kb[u], r);
+                    for (int i = 0; i < RL; i++)
 #pragma unroll
-                    for (int r = 0; r < MF; r++) acc[r] = acc[r] + sv[u] * kr[r];
+                        for (int j = 0; j < 4; j++) acc[i][j] = acc[i][j] + v[j ^ sw] * kc[i];
+                    if constexpr (XR) accx = accx + xs * xb[u % D];
+                    __builtin_amdgcn_sched_barrier(0);
+                    const int pn = k0 + sl + D;
+                    load_k(kb[u % D], pn);
+                    if constexpr (XR) xb[u % D] = load_x(pn);
+                    // keep each position's loads where they are, after its
+                    // products: the scheduler would otherwise sink the K
+                    // loads to the end of the 8 positions, or start one
+                    // before the products that read the slot's old values,
+                    // and wait for it there
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
         } else {
             // the last chunk: partial, or holding the fused last position
             for (int sl = 0; sl < n; sl++) {
                 const int k = k0 + sl;
-                const double kv = kw[k * kstride];
-                const double sv = tb[trial_index<NM>(m, sl)];
                 const bool f = lodd && k == L - 1;
+                double v[4], kv[RL];
 #pragma unroll
-                for (int r = 0; r < MF; r++) {
-                    const double a = readlane_f64(kv, r);
-                    acc[r] = f ? fma(sv, a, acc[r]) : acc[r] + sv * a;
+                for (int j = 0; j < 4; j++) v[j] = tb[trial_index<NM>(mb + j, sl)];
+                load_k(kv, k);
+#pragma unroll
+                for (int i = 0; i < RL; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) acc[i][j] = f ? fma(v[j], kv[i], acc[i][j]) : acc[i][j] + v[j] * kv[i];
+                if constexpr (XR) {
+                    const double xs = tb[trial_index<NM>(xm, sl)], xk = load_x(k);
+                    accx = f ? fma(xs, xk, accx) : accx + xs * xk;
                 }
             }
         }
@@ -780,12 +849,15 @@ __device__ __forceinline__ void fold_chains(const BatchDesc &d, double *lds, boo
     for (int c = min(4, nsteps) + nwork; c < nsteps; c++) step_barrier();
     __syncthreads();
     // E[t][m] into LDS (the chunk buffers are free now)
-    if (chains && lane < d.P) {
+    if (chains) {
 #pragma unroll
-        for (int r = 0; r < MF; r++) {
-            const int t = wave + 4 * r;
-            if (t < T) lds[t * Ppad + lane] = acc[r];
+        for (int i = 0; i < RL; i++) {
+            const int t = rq * RL + i;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (t < T && mb + j < P) lds[t * Ppad + mb + j] = acc[i][j];
         }
+        if (xv) lds[(R + wave) * Ppad + lane] = accx;
     }
 }
 
@@ -813,7 +885,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
 
     if (wave < kWorker0) {
         if constexpr (PREC == kPrecFold) {
-            fold_chains<NT, NM>(d, lds, chains, nchunk, nsteps, wave, lane);  // NT: accumulators per lane
+            fold_chains<NT / 2, NT % 2, NM>(d, lds, chains, nchunk, nsteps, wave, lane);  // NT: fold shape 2 RL + XR
         } else if constexpr (PREC != kPrecFp64) {
             mixed_chains<PREC, NT, NM>(d, sh, lds, kind, g, chains, nchunk, nsteps, wave, lane);
         } else {

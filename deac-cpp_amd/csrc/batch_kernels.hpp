@@ -35,8 +35,8 @@ struct BatchDesc {
     const double *obs, *sig;   // [T] ISF and its error
     const double *mterm;       // [3][Lpad] first-moment, third-moment and normalization weights (zero past L),
                                // or null (FP64 kernel)
-    const double *kfold;       // bdsf fold kernel: [Lpad][4 chain waves][fold_rows], entry (k, w, r) = K row
-                               // w + 4 r at position k (zero past T and L)
+    const double *kfold;       // bdsf fold kernel: [Lpad][fold_rows], entry (k, t) = K row t at position k
+                               // (zero past T and L)
     const double *nfm_dt;      // [T] trapezoid weights of the -1st-moment term (use_nfm)
     const uint64_t *walk_rows; // [256][4] rows of T^L, T the xoshiro transition
     const uint64_t *sub_tab;   // [S-1][64][16][4] nibble tables of T^(k*Lsub), k = 1..S-1
@@ -76,7 +76,7 @@ struct BatchDesc {
     int use_first, use_third;  // moment constraints (deac.cpp:1420-1477, 1550-1555), FP64 kernel only
     int renorm;                // renormalize each generation's trials (deac.cpp:1342-1364), FP64 kernel only
     int use_nfm;               // -1st-moment term in the fitness (bdsf above zero temperature, deac.cpp:961-965)
-    int fold_rows;             // bdsf fold kernel: timeslices per chain wave, ceil(T / 4)
+    int fold_rows;             // bdsf fold kernel: row stride of kfold, R (+ 4 with extra timeslices)
 };
 
 // Exact FP64 kernel.  NT: 16-row kernel tiles per chain wavefront (1, 2 or 4); NM: 16-member
@@ -86,10 +86,11 @@ hipError_t launch_batch(const BatchDesc *descs, int count, int nt, int nm, int r
 // The detailed-balance (bdsf) kernel: the trials' model ISF as a fold of rounded products
 // on the FP64 VALU (the reference build's loop contraction), everything else as the FP64
 // kernel; never evaluates a population from scratch (the FP64 kernel's evaluation launch
-// gives the initial fitness).  mf: accumulators per lane, one of kBatchFoldMf, at least
-// fold_rows; nm as above.
-constexpr int kBatchFoldMf[] = {8, 16, 17, 24, 32};
-hipError_t launch_batch_fold(const BatchDesc *descs, int count, int mf, int nm, hipStream_t s);
+// gives the initial fitness).  shape = 2 RL + XR: a lane folds RL consecutive timeslices of
+// one of 64 / nm row blocks (R = 64 RL / nm timeslices) for 4 members; XR = 1 folds timeslices
+// R .. T - 1 (at most 4), one per chain wave.  RL: 1 or 2 (nm 1), 1, 2 or 4 (nm 2), 1, 2, 4
+// or 8 (nm 4); nm as above.
+hipError_t launch_batch_fold(const BatchDesc *descs, int count, int shape, int nm, hipStream_t s);
 
 // Device scratch of deac_batch_run_noalloc: a file-scope array of kBatchScratchDoubles
 // doubles per GPU (32 MiB), its address looked up once per device with that device
