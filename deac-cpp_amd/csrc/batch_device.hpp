@@ -875,8 +875,11 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                                          long wgen)
 {
     constexpr int C = 128 / NM, Ppad = 16 * NM;
-    // kernel-tile prefetch depth in pairs of 4-position blocks (divides C/8)
-    constexpr int KD = NT == 1 ? (REM ? 4 : 8) : (NT == 2 ? 4 : 2);
+    // kernel-tile prefetch depth in pairs of 4-position blocks; it must
+    // divide C/8 (4 block pairs per chunk at NM = 4), since the chain loop
+    // keeps block pair g in slot (g mod C/8) mod KD
+    constexpr int KD = std::min(NT == 1 ? (REM ? 4 : 8) : (NT == 2 ? 4 : 2), C / 8);
+    static_assert((C / 8) % KD == 0, "the K prefetch depth must divide the block pairs of a chunk");
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     const int nchunk = d.nchunk;
     constexpr bool F64 = PREC == kPrecFp64 || PREC == kPrecFold;  // FP64 trial values

@@ -382,6 +382,11 @@ WIDE = [
     (51, 200, 40, dict(model="normalization", spectra_type="ffull", zero_temperature=True, normalize=True)),
     (132, 120, 64, dict(model="hyperbolic", allow_negative=True)),
     (128, 97, 33, dict(first_moment=30.0, third_moment=1e9, third_moment_error=1e8, normalize=True)),
+    # whole 16-row tiles only and one tile per chain wave (4 block pairs of
+    # K per chunk at this member count)
+    (16, 64, 40, dict(normalize=True)),
+    (32, 33, 48, dict(model="hyperbolic")),
+    (64, 130, 64, dict(normalize=True)),
 ]
 
 
