@@ -23,7 +23,8 @@ hipError_t launch_fold_one(const BatchDesc *descs, int count, hipStream_t s)
     return hipGetLastError();
 }
 
-// shape = 2 RL + XR; RL at most 128 / (64 / NM) timeslices per lane
+// shape = 2 RL + XR, the shapes of T <= 128: R = 64 RL / NM <= 128, and
+// extra timeslices only past R < 128
 template <int NM>
 hipError_t launch_fold_shape(const BatchDesc *descs, int count, int shape, hipStream_t s)
 {
@@ -31,15 +32,14 @@ hipError_t launch_fold_shape(const BatchDesc *descs, int count, int shape, hipSt
     case 2: return launch_fold_one<2, NM>(descs, count, s);
     case 3: return launch_fold_one<3, NM>(descs, count, s);
     case 4: return launch_fold_one<4, NM>(descs, count, s);
-    case 5: return launch_fold_one<5, NM>(descs, count, s);
     }
     if constexpr (NM >= 2) {
+        if (shape == 5) return launch_fold_one<5, NM>(descs, count, s);
         if (shape == 8) return launch_fold_one<8, NM>(descs, count, s);
-        if (shape == 9) return launch_fold_one<9, NM>(descs, count, s);
     }
     if constexpr (NM >= 4) {
+        if (shape == 9) return launch_fold_one<9, NM>(descs, count, s);
         if (shape == 16) return launch_fold_one<16, NM>(descs, count, s);
-        if (shape == 17) return launch_fold_one<17, NM>(descs, count, s);
     }
     return hipErrorInvalidValue;
 }

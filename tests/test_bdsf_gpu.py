@@ -162,6 +162,48 @@ def test_bdsf_batch_matches_oracle(name, slice_len, monkeypatch):
         assert o["rng"] == r["rng"] and o["generation"] == r["generation"], sd
 
 
+# One case per register-tile shape of the fold kernel (2 RL + XR: RL
+# timeslices per lane, R = 64 RL / NM of them in the row blocks, XR extra
+# timeslices past R), for each member-tile count NM (P <= 16, 32, 64), with
+# odd and even genome lengths (the fused last position, partial chunks)
+FOLD_SHAPES = {
+    "nm1_rl1_x_T66_P16": dict(T=66, G=97, P=16),
+    "nm1_rl2_T100_P10": dict(T=100, G=64, P=10),
+    "nm2_rl1_T30_P24": dict(T=30, G=130, P=24),
+    "nm2_rl1_x_T34_P24": dict(T=34, G=75, P=24),
+    "nm2_rl2_T64_P20": dict(T=64, G=200, P=20),
+    "nm4_rl1_T16_P40": dict(T=16, G=33, P=40),
+    "nm4_rl1_x_T19_P36": dict(T=19, G=64, P=36),
+    "nm4_rl2_T32_P48": dict(T=32, G=65, P=48),
+    "nm4_rl2_x_T35_P64": dict(T=35, G=96, P=64),
+    "nm4_rl4_T50_P40": dict(T=50, G=47, P=40),
+    "nm4_rl4_x_T68_P48": dict(T=68, G=120, P=48),
+    "nm4_rl8_T128_P64": dict(T=128, G=71, P=64),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FOLD_SHAPES))
+def test_bdsf_fold_shapes_match_oracle(name):
+    import oracle_binding as ob
+    spec = FOLD_SHAPES[name]
+    print(f"[gpu] bdsf fold shape {name}", flush=True)
+    T, G, P = spec["T"], spec["G"], spec["P"]
+    tau = np.linspace(0.0, 0.4, T)
+    isf = 0.6 * np.exp(-1.1 * tau) + 0.4 * np.exp(-0.3 * tau)
+    err = 2e-3 * isf
+    kw = dict(population_size=P, spectra_type="bdsf", use_negative_first_moment=True)
+    prob = da.make_problem(tau, isf, err, da.frequency_grid(G, 16.0), 1.25, **kw)
+    seeds, N = [11, 12], 70
+    with da.Batch([prob] * 2) as b:
+        outs = b.run(seeds, N, 0.0, True)
+    for sd, o in zip(seeds, outs):
+        r = ob.run(ob.make_problem(tau, isf, err, ob.frequency_grid(G, 16.0), 1.25, number_of_generations=N,
+                                   stop_minimum_fitness=0.0, **kw), sd)
+        for k in ("minimum", "mean", "squared_mean", "dsf", "frequency"):
+            assert gc.bit_diff(o[k], r[k]) is None, (sd, k, gc.bit_diff(o[k], r[k]))
+        assert o["rng"] == r["rng"] and o["generation"] == r["generation"], sd
+
+
 def test_bdsf_batch_continues_and_refuses_mixed_precision():
     case = gc.load("bdsf_standard")
     a = case["meta"]["args"]
