@@ -406,6 +406,30 @@ def test_batch_up_to_64_members_matches_oracle(T, G, P, kw):
         assert o["rng"] == r["rng"] and o["generation"] == r["generation"]
 
 
+# Every FP64 kernel shape (NT 16-row kernel tiles per chain wave, NM
+# 16-member tiles, remainder rows REM) against the oracle: T picks NT and
+# REM (64: NT 1; 35: NT 1 + 3 rows; 90: NT 2, 10 rows padded to a tile;
+# 114: NT 2 + 2 rows; 160: NT 4; 196: NT 4 + 4 rows), P picks NM (12, 28,
+# 50 -> 1, 2, 4; 4 member tiles take at most 2 kernel tiles per wave)
+SHAPE_SWEEP = [(T, P) for T in (64, 35, 90, 114, 160, 196) for P in (12, 28, 50) if not (T > 132 and P > 32)]
+
+
+@pytest.mark.parametrize("T,P", SHAPE_SWEEP)
+def test_batch_kernel_shapes_match_oracle(T, P):
+    print(f"[gpu] batch kernel shape T={T} P={P}", flush=True)
+    t, f, s = _synthetic_arrays(T)
+    G, N, seeds = 45 + (T % 7) * 11, 60, [9, 10]
+    kw = dict(normalize=(T % 2 == 0), model="hyperbolic" if P == 28 else "standard")
+    prob = da.make_problem(t, f, s, da.frequency_grid(G, 48.0), 1.35, population_size=P, **kw)
+    with da.Batch([prob] * len(seeds)) as b:
+        outs = b.run(seeds, N, 0.0, True)
+    for sd, o in zip(seeds, outs):
+        r = ob.run(ob.make_problem(t, f, s, ob.frequency_grid(G, 48.0), 1.35, population_size=P,
+                                   number_of_generations=N, stop_minimum_fitness=0.0, **kw), sd)
+        assert not _compare(o, r), (sd, _compare(o, r))
+        assert o["rng"] == r["rng"] and o["generation"] == r["generation"]
+
+
 def test_batch_64_members_refuses_mixed_precision():
     t, f, s = _synthetic_arrays(65)
     prob = da.make_problem(t, f, s, da.frequency_grid(128, 64.0), 1.35, population_size=48)

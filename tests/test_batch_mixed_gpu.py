@@ -74,12 +74,14 @@ def test_mixed_seed_batch_against_fp64(prec, monkeypatch):
 def test_mixed_shapes_against_fp64(prec):
     # one member tile and two; odd T (rows past the last tile are zero rows);
     # zero temperature and the normalization model; genomes not a multiple of
-    # the chunk; the largest T
+    # the chunk; the largest T.  Tile pairs per chain wave: 1 (T = 18, 33),
+    # 2 (50, 100), 3 (65, 180), 5 (129, 257), 9 (260)
     print(f"[gpu] mixed batch shapes {prec}", flush=True)
     cases = []
     for T, G, P, kw in ((18, 300, 8, {}), (33, 512, 16, dict(model="normalization")),
                         (65, 700, 24, dict(zero_temperature=True)), (129, 1000, 32, {}),
-                        (257, 2048, 16, {}), (260, 96, 32, dict(model="hyperbolic"))):
+                        (257, 2048, 16, {}), (260, 96, 32, dict(model="hyperbolic")),
+                        (50, 333, 30, {}), (100, 256, 12, dict(model="hyperbolic")), (180, 129, 10, {})):
         beta = 1.0 / 1.35
         t = np.linspace(0.0, beta / 2, T)
         f = 0.6 * np.exp(-2.0 * t) + 0.4 * np.exp(-9.0 * t)
