@@ -29,6 +29,9 @@ KW = {
     "negw_P4": dict(T=21, G=150, temp=1.0, kw=dict(population_size=4, allow_negative=True, model="hyperbolic")),
     "moments_P6": dict(T=25, G=120, temp=1.0, kw=dict(population_size=6, first_moment=1.0, third_moment=3.0,
                                                       third_moment_error=0.5)),
+    "renorm_P8": dict(T=33, G=150, temp=1.0, kw=dict(population_size=8, spectra_type="bfull", renormalize=True,
+                                                     normalize=True)),
+    "bigT_P4": dict(T=150, G=80, temp=1.35, kw=dict(population_size=4, model="hyperbolic")),
 }
 
 
