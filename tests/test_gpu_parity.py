@@ -54,6 +54,29 @@ def test_engine_matches_oracle_config1_other_seeds(seed):
     assert g["rng"] == o["rng"]
 
 
+@pytest.mark.parametrize("P,T,G,kw", [
+    (512, 65, 200, dict(normalize=True)),  # the reference CLI's default population (deac.cpp:1935-1938)
+    (257, 33, 97, dict(model="hyperbolic")),
+    (100, 129, 64, dict(spectra_type="ffull", model="normalization", normalize=True)),
+    (200, 40, 150, dict(spectra_type="bdsf", use_negative_first_moment=True)),
+])
+def test_engine_large_populations_match_oracle(P, T, G, kw):
+    # populations past the batch kernel's 64 members run on the plain engine
+    print(f"[gpu] plain engine P={P} T={T} G={G} {kw}", flush=True)
+    tau = np.linspace(0.0, 0.37, T)
+    isf = 0.55 * np.exp(-1.7 * tau) + 0.45 * np.exp(-0.4 * tau)
+    err = 2e-3 * isf
+    freq = ob.frequency_grid(G, 48.0)
+    N, seed = 40, 7
+    o = ob.run(ob.make_problem(tau, isf, err, freq, 1.35, number_of_generations=N, population_size=P,
+                               stop_minimum_fitness=0.0, **kw), seed)
+    with da.Engine(da.make_problem(tau, isf, err, freq, 1.35, population_size=P, **kw)) as e:
+        g = e.run(seed, N, 0.0)
+    for k in ("minimum", "mean", "squared_mean", "dsf", "frequency"):
+        assert gc.bit_diff(g[k], o[k]) is None, (k, gc.bit_diff(g[k], o[k]))
+    assert g["rng"] == o["rng"]
+
+
 @pytest.mark.parametrize("model,spectra", [("standard", "bfull"), ("normalization", "ffull"), ("hyperbolic", "bfull")])
 def test_zero_temperature_matches_oracle(model, spectra):
     # the reference's zero-temperature build compiles only with the
