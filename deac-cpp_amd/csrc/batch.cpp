@@ -585,13 +585,6 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
                               "detailed-balance spectra (run FP64, or the instance through deac_run)";
             return DEAC_ERR_UNSUPPORTED;
         }
-        if (prec != DEAC_PRECISION_FP64 && (mi.use_first || mi.use_third)) {
-            // the moments fold the FP64 trials, which the mixed kernel keeps
-            // only as their FP32 / BF16 change from the parent
-            deac::last_message() = "instance " + std::to_string(i) + ": mixed K.S in the batch does not take moment "
-                              "constraints (run FP64, or the instance through deac_run)";
-            return DEAC_ERR_UNSUPPORTED;
-        }
         if (prec != DEAC_PRECISION_FP64 && mi.normalize && mi.renormalize) {
             // the scale of a trial needs all of it before its contraction,
             // which the mixed kernel's delta form does not take
@@ -694,7 +687,7 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
     BATCH_HIP(hipEventRecord(h->ev0, s));
     BATCH_HIP(hipEventRecord(h->ev1, s));  // re-recorded after every round's status copy
     bool first = true;
-    // Launch key: (precision, NT or NQ, NM, remainder rows).
+    // Launch key: (precision, NT or NQ, NM, remainder rows (FP64) or moment constraints (mixed)).
     typedef std::tuple<int, int, int, int> Key;
     // bdsf instances evolve on the fold kernel (precision kFoldKey); the FP64
     // kernel evaluates their initial population (the reference's initial
@@ -705,13 +698,13 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
         const int prec = options[i].ks_precision;
         if (b.m.bdsf && !eval) return Key{kFoldKey, b.MF, b.NM, 0};
         if (prec == DEAC_PRECISION_FP64 || eval) return Key{DEAC_PRECISION_FP64, b.NT, b.NM, b.nrem > 0 ? 1 : 0};
-        return Key{prec, b.NQ, b.NM, 0};
+        return Key{prec, b.NQ, b.NM, (b.m.use_first || b.m.use_third) ? 1 : 0};  // last: moment constraints
     };
     auto launch = [&](const Key &k, const deac::BatchDesc *dd, int cnt, hipStream_t st) {
         if (std::get<0>(k) == DEAC_PRECISION_FP64)
             return deac::launch_batch(dd, cnt, std::get<1>(k), std::get<2>(k), std::get<3>(k), st);
         if (std::get<0>(k) == kFoldKey) return deac::launch_batch_fold(dd, cnt, std::get<1>(k), std::get<2>(k), st);
-        return deac::launch_batch_mixed(dd, cnt, std::get<0>(k), std::get<1>(k), std::get<2>(k), st);
+        return deac::launch_batch_mixed(dd, cnt, std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k), st);
     };
     // Instances of instance i's problem that may share its workgroup (FP64;
     // populations that leave room in the 16 member rows of their kernel
@@ -1157,7 +1150,8 @@ deac_status deac_batch_tune(deac_batch *h, const uint64_t *rng_states, const dea
             shapes[{3, b.MF, b.NM, 0}] = 1;  // the fold kernel
         else
             shapes[{options[i].ks_precision, options[i].ks_precision == DEAC_PRECISION_FP64 ? b.NT : b.NQ, b.NM,
-                    options[i].ks_precision == DEAC_PRECISION_FP64 && b.nrem > 0 ? 1 : 0}] = 1;
+                    options[i].ks_precision == DEAC_PRECISION_FP64 ? (b.nrem > 0 ? 1 : 0)
+                                                                   : (b.m.use_first || b.m.use_third ? 1 : 0)}] = 1;
     }
     const bool several_groups = shapes.size() > 1, several_models = h->models.size() > 1;
     std::vector<deac_batch_schedule> cand;

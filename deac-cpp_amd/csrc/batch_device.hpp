@@ -76,6 +76,11 @@ constexpr int kRngWave0 = 13;          // 13-15: mask waves
 constexpr int kCD = kBatchChunkDoubles;
 constexpr int kOffPop = 0;             // 4 buffers: genomes of chunks in flight, [Ppad][C]
 constexpr int kOffTrial = 4 * kCD;     // 2 buffers: trial genomes in MFMA B order
+// Mixed kernel with moment constraints: the workers' partial moment sums of
+// a pass, [member][C/2 lanes][2], at the end of the chunk region (past the
+// model ISF [T][Ppad] that the chain waves write there after the pass)
+constexpr int kPartDoubles = 2048;
+constexpr int kOffPart = 6 * kCD - kPartDoubles;
 constexpr int kMaskWords = 256;        // mask words per chunk: 2 generations x Ppad x 2 candidates x C/32
 constexpr int kMaxT = 4 * 4 * 16 + kBatchMaxRem;  // timeslices: 16 full tiles (4 per chain wave) + remainder rows
 
@@ -883,6 +888,10 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     const int nchunk = d.nchunk;
     constexpr bool F64 = PREC == kPrecFp64 || PREC == kPrecFold;  // FP64 trial values
+    // mixed K.S instantiations with REM = 1 take moment constraints
+    constexpr bool MOM = (PREC == kPrecFp32 || PREC == kPrecBf16) && REM != 0;
+    static_assert(!MOM || (C / 2) * Ppad * 2 <= kPartDoubles, "the partial moment sums fit their region");
+    static_assert(kMaxT * kBatchMixedMaxP <= kOffPart, "the mixed model ISF stays below the partial sums");
     const bool chains = kind == kPassInit || kind == kPassEvolve;
     const int nsteps = kind == kPassApply ? nchunk + 2 : kind == kPassNone ? 0 : nchunk + 4;
 
@@ -1061,6 +1070,18 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         const int gslot = static_cast<int>((g + 3) % 3);
         const int aslot = static_cast<int>((g + 2) % 3);      // generation g-1
         const int p = (2 * lane) % C;
+        // Mixed K.S with moment constraints (MOM): the first and third
+        // moments of each trial, the lane's two positions per chunk summed in
+        // FP64 across the pass (the mixed fitness is an approximation: the
+        // FP64 evaluation after the slice recomputes every reported fitness
+        // in the reference's order); the weights of the trial chunk are
+        // loaded at the top of its step.
+        const bool wmom = MOM && kind == kPassEvolve;
+        const gcdouble *const mtp = (const gcdouble *)uniform_ptr(d.mterm);
+        double pm1[kT], pm3[kT];
+        double2 wm1 = make_double2(0.0, 0.0), wm3 = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < kT; u++) pm1[u] = pm3[u] = 0.0;
         // Per-row constants of the pass.  Padded members (m >= P) compute on
         // the last member's data: their trial columns only feed their own
         // (discarded) model values.
@@ -1222,6 +1243,13 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             // (the mixed kernel's chain waves stage the chunks instead: Stager;
             // in the FP64 kernel their registers hold the kernel-tile ring)
             const bool ld = F64 && kind != kPassPre && c < nchunk;
+            if constexpr (MOM) {
+                if (wmom && c >= 3 && c - 3 < nchunk) {
+                    const size_t k = static_cast<size_t>(c - 3) * C + p;
+                    wm1 = make_double2(mtp[k], mtp[k + 1]);
+                    wm3 = make_double2(mtp[nLpad + k], mtp[nLpad + k + 1]);
+                }
+            }
             if (ld) {
                 const gchar *pb = (const gchar *)popp + static_cast<size_t>(c) * C * 8;
 #pragma unroll
@@ -1260,7 +1288,18 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                         v[u].y = sp + 1 < nL ? x.y : 0.0;
                     }
                 } else {
-                    de_stage(buf, sh.mring[ct % 4], mo, od, g, ct, sh.y0t, v, !F64);
+                    de_stage(buf, sh.mring[ct % 4], mo, od, g, ct, sh.y0t, v, !F64 && !MOM);
+                    if constexpr (MOM) {
+                        // the trial's moment terms, then dS = trial - parent
+#pragma unroll
+                        for (int u = 0; u < kT; u++) {
+                            pm1[u] += v[u].x * wm1.x + v[u].y * wm1.y;
+                            pm3[u] += v[u].x * wm3.x + v[u].y * wm3.y;
+                            const double2 x0 = *reinterpret_cast<const double2 *>(buf + own + u * 1024);
+                            v[u].x -= x0.x;
+                            v[u].y -= x0.y;
+                        }
+                    }
                     if (renorm && kind == kPassEvolve) {
                         // the trial times zeroth_moment / norm (deac.cpp:1353-1360)
 #pragma unroll
@@ -1325,6 +1364,17 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             step_barrier();
         }
         __syncthreads();
+        if constexpr (MOM) {
+            if (wmom) {
+#pragma unroll
+                for (int u = 0; u < kT; u++)
+                    if (mrow(u) < nP) {
+                        double *q = lds + kOffPart + (mrow(u) * (C / 2) + p / 2) * 2;
+                        q[0] = pm1[u];
+                        q[1] = pm3[u];
+                    }
+            }
+        }
     } else if (wave >= kRngWave0) {
         // ------------------------------------------------ mask waves
         // The next generation's crossover draws, Lsub per lane per pass.
@@ -1468,6 +1518,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
 {
     constexpr int Ppad = 16 * NM;
     constexpr bool F64 = PREC == kPrecFp64 || PREC == kPrecFold;  // FP64 trial values
+    constexpr bool MOM = (PREC == kPrecFp32 || PREC == kPrecBf16) && REM != 0;  // mixed, moment constraints
     extern __shared__ __attribute__((aligned(16))) double lds[];  // 16-byte aligned: ds_*_b128 at full rate
     __shared__ BatchShared sh;
     const BatchDesc &d = descs[blockIdx.x];
@@ -1616,6 +1667,20 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
                         chi = t == tlast ? fma(v, v, chi) : chi + v;
                     }
                     double fit = chi / static_cast<double>(d.T);
+                    if constexpr (MOM) {
+                        // the trial's moments: the workers' partial sums, in lane order
+                        if (kind == kPassEvolve) {
+                            constexpr int H = 64 / NM;  // partial sums per member (C / 2)
+                            const double *q = lds + kOffPart + lane * H * 2;
+                            double a = 0.0, b = 0.0;
+                            for (int j = 0; j < H; j++) {
+                                a += q[2 * j];
+                                b += q[2 * j + 1];
+                            }
+                            sh.mom[lane][0] = a;
+                            sh.mom[lane][1] = b;
+                        }
+                    }
                     if (F64 && d.use_nfm) {  // deac.cpp:961-965, 1543-1547
                         const double q = (d.nfm_value - sh.nfm[lane]) / d.nfm_error;
                         fit = fma(q, q, fit);

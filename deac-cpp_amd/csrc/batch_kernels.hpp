@@ -106,9 +106,11 @@ hipError_t batch_role_cycles_fp64(unsigned long long *out, int reset);
 hipError_t batch_role_cycles_mixed(unsigned long long *out, int reset);
 
 // Mixed K.S kernel (precision DEAC_PRECISION_FP32 or DEAC_PRECISION_BF16X3).  nq: (kernel
-// tile, member tile) pairs per chain wavefront, one of kBatchMixedNq; nm as above.
+// tile, member tile) pairs per chain wavefront, one of kBatchMixedNq; nm as above; moments:
+// 1 for problems with moment constraints (the trials' moments summed by the workers).
 constexpr int kBatchMixedNq[] = {1, 2, 3, 5, 9};
 int batch_mixed_nq(int ntm, int nm);  // smallest instantiated nq for ntm kernel tiles (0: none)
-hipError_t launch_batch_mixed(const BatchDesc *descs, int count, int precision, int nq, int nm, hipStream_t s);
+hipError_t launch_batch_mixed(const BatchDesc *descs, int count, int precision, int nq, int nm, int moments,
+                              hipStream_t s);
 
 }  // namespace deac

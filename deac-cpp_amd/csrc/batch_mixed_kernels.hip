@@ -6,32 +6,40 @@ namespace deac {
 
 namespace {
 
-template <int PREC, int NQ, int NM>
+// MOM = 1: the instantiation with moment constraints (the kernel's REM
+// parameter, which the mixed contraction does not otherwise use)
+template <int PREC, int NQ, int NM, int MOM>
 hipError_t launch_mixed_one(const BatchDesc *descs, int count, hipStream_t s)
 {
     constexpr size_t lds_bytes = static_cast<size_t>(kBatchRegionDoubles) * sizeof(double);
     static bool configured = false;
     if (!configured) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<PREC, NQ, NM, 0>),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<PREC, NQ, NM, MOM>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes));
         if (e != hipSuccess) return e;
         configured = true;
     }
-    hipLaunchKernelGGL((batch_kernel<PREC, NQ, NM, 0>), dim3(count), dim3(kBatchThreads), lds_bytes, s, descs);
+    hipLaunchKernelGGL((batch_kernel<PREC, NQ, NM, MOM>), dim3(count), dim3(kBatchThreads), lds_bytes, s, descs);
     return hipGetLastError();
 }
 
-template <int PREC, int NM>
+template <int PREC, int NM, int MOM>
 hipError_t launch_mixed_nq(const BatchDesc *descs, int count, int nq, hipStream_t s)
 {
     switch (nq) {
-    case 1: return launch_mixed_one<PREC, 1, NM>(descs, count, s);
-    case 2: return launch_mixed_one<PREC, 2, NM>(descs, count, s);
-    case 3: return launch_mixed_one<PREC, 3, NM>(descs, count, s);
-    case 5: return launch_mixed_one<PREC, 5, NM>(descs, count, s);
-    case 9: if constexpr (NM == 2) return launch_mixed_one<PREC, 9, NM>(descs, count, s);
+    case 1: return launch_mixed_one<PREC, 1, NM, MOM>(descs, count, s);
+    case 2: return launch_mixed_one<PREC, 2, NM, MOM>(descs, count, s);
+    case 3: return launch_mixed_one<PREC, 3, NM, MOM>(descs, count, s);
+    case 5: return launch_mixed_one<PREC, 5, NM, MOM>(descs, count, s);
+    case 9: if constexpr (NM == 2) return launch_mixed_one<PREC, 9, NM, MOM>(descs, count, s);
     }
     return hipErrorInvalidValue;
+}
+
+template <int PREC, int MOM>
+hipError_t launch_mixed_nm(const BatchDesc *descs, int count, int nq, int nm, hipStream_t s)
+{
+    return nm == 1 ? launch_mixed_nq<PREC, 1, MOM>(descs, count, nq, s) : launch_mixed_nq<PREC, 2, MOM>(descs, count, nq, s);
 }
 
 }  // namespace
@@ -47,15 +55,16 @@ int batch_mixed_nq(int ntm, int nm)
     return 0;
 }
 
-hipError_t launch_batch_mixed(const BatchDesc *descs, int count, int precision, int nq, int nm, hipStream_t s)
+hipError_t launch_batch_mixed(const BatchDesc *descs, int count, int precision, int nq, int nm, int moments,
+                              hipStream_t s)
 {
     if (nm != 1 && nm != 2) return hipErrorInvalidValue;
     if (precision == kPrecFp32)
-        return nm == 1 ? launch_mixed_nq<kPrecFp32, 1>(descs, count, nq, s)
-                       : launch_mixed_nq<kPrecFp32, 2>(descs, count, nq, s);
+        return moments ? launch_mixed_nm<kPrecFp32, 1>(descs, count, nq, nm, s)
+                       : launch_mixed_nm<kPrecFp32, 0>(descs, count, nq, nm, s);
     if (precision == kPrecBf16)
-        return nm == 1 ? launch_mixed_nq<kPrecBf16, 1>(descs, count, nq, s)
-                       : launch_mixed_nq<kPrecBf16, 2>(descs, count, nq, s);
+        return moments ? launch_mixed_nm<kPrecBf16, 1>(descs, count, nq, nm, s)
+                       : launch_mixed_nm<kPrecBf16, 0>(descs, count, nq, nm, s);
     return hipErrorInvalidValue;
 }
 

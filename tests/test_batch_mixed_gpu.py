@@ -130,14 +130,48 @@ def test_batch_refuses_the_split_loop():
         assert st == 5 and "split loop" in da._last_message()
 
 
-def test_mixed_batch_refuses_moment_constraints():
-    # the moments fold FP64 trials, which the mixed kernel keeps only as their
-    # change from the parent: refused before any launch, FP64 runs them
-    print("[gpu] mixed batch with moment constraints refused", flush=True)
-    t, f, s = np.fromfile(f"{gc.GOLDEN_DIR}/isf_c1_T33.bin").reshape(3, -1)
-    p = da.make_problem(t, f, s, da.frequency_grid(256, 64.0), 1.35, population_size=8, normalize=True,
-                        first_moment=3.0)
-    with da.Batch([p]) as b:
-        with pytest.raises(da.DeacError, match="moment"):
-            b.run([1], 20, 0.0, precision="fp32")
-        assert b.run([1], 20, 0.0)[0]["generation"] == 19
+# (T, G, P, problem options, ISF error / ISF): errors large enough that the
+# first-moment term alone moves the fitness
+MOMENTS = [
+    (65, 700, 24, dict(first_moment=3.0, third_moment=40.0, third_moment_error=2.0, normalize=True), 1e-3),
+    (33, 256, 8, dict(first_moment=3.0), 0.5),
+    (129, 333, 32, dict(third_moment=40.0, third_moment_error=0.5, model="hyperbolic"), 1e-3),
+    (40, 300, 16, dict(first_moment=0.2, zero_temperature=True, normalize=True), 1e-3),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("T,G,P,kw,e", MOMENTS)
+def test_mixed_batch_with_moment_constraints(prec, T, G, P, kw, e):
+    # the mixed kernel sums each trial's first and third moments in FP64 (its
+    # own order): generation 0's selections match the FP64 batch's to the
+    # precision of the contraction, the initial and final fitness are the
+    # FP64 evaluation's, and the runs repeat bit for bit
+    print(f"[gpu] mixed batch {prec} with moments T={T} G={G} P={P} {kw}", flush=True)
+    beta = 1.0 / 1.35
+    t = np.linspace(0.0, beta / 2, T)
+    f = 0.6 * np.exp(-2.0 * t) + 0.4 * np.exp(-9.0 * t)
+    temp = 0.0 if kw.get("zero_temperature") else 1.35
+    p = da.make_problem(t, f, e * f, da.frequency_grid(G, 40.0), temp, population_size=P, **kw)
+    seeds, N = [3, 4, 5, 6], 60
+    with da.Batch([p] * len(seeds)) as b:
+        ref = b.run(seeds, N, 0.0, True, precision="fp64")
+        out = b.run(seeds, N, 0.0, True, precision=prec)
+        again = b.run(seeds, N, 0.0, True, precision=prec)
+    # the moment terms matter at this tolerance: without them the fitness of
+    # the best trial after generation 0 moves by far more than 1e-5
+    plain_kw = {k: v for k, v in kw.items() if "moment" not in k}
+    p0 = da.make_problem(t, f, e * f, da.frequency_grid(G, 40.0), temp, population_size=P, **plain_kw)
+    with da.Batch([p0] * len(seeds)) as b:
+        bare = b.run(seeds, N, 0.0, True, precision="fp64")
+    assert all(abs(z["minimum"][1] - r["minimum"][1]) > 1e-3 * r["minimum"][1] for z, r in zip(bare, ref))
+    for k, (o, r, a) in enumerate(zip(out, ref, again)):
+        assert o["minimum"][0] == r["minimum"][0], k
+        assert o["rng"] == r["rng"], k
+        assert abs(o["minimum"][1] - r["minimum"][1]) <= 1e-5 * r["minimum"][1], (k, o["minimum"][1], r["minimum"][1])
+        assert gc.bit_diff(o["minimum"], a["minimum"]) is None and gc.bit_diff(o["dsf"], a["dsf"]) is None, k
+        m = o["minimum"]
+        assert np.all(m[1:] <= m[:-1] * (1 + 1e-5)), k
+    fin = np.median([o["minimum_fitness"] for o in out])
+    finr = np.median([r["minimum_fitness"] for r in ref])
+    assert 0.3 < fin / finr < 3.0, (fin, finr)

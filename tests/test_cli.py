@@ -321,32 +321,27 @@ def test_cli_seed_batch_unbatchable_runs_seed_by_seed(tmp_path):
 
 
 @pytest.mark.gpu
-def test_cli_mixed_precision_seeds_with_moments_run_seed_by_seed(tmp_path):
-    # the mixed batch refuses moment constraints at run time; deac.e falls
-    # back to single runs, each exactly its own --seed run
-    _seed_batch_vs_single_runs(tmp_path, ["--ks_precision", "fp32", "--first_moment", "0.5"], [5, 6],
-                               expect_batched=False)
-
-
-@pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
-def test_cli_mixed_precision_seed_batch(tmp_path, prec):
-    # fp32 / bf16x3 seeds run in the batch kernel (delta form): each seed's
-    # files are that seed's instance of deac_batch_run at the same precision,
-    # bit for bit, and its reported fitness is the FP64 chi^2 of its genome
+@pytest.mark.parametrize("prec,moments", [("fp32", False), ("bf16x3", False), ("fp32", True)])
+def test_cli_mixed_precision_seed_batch(tmp_path, prec, moments):
+    # fp32 / bf16x3 seeds run in the batch kernel (delta form), with moment
+    # constraints too: each seed's files are that seed's instance of
+    # deac_batch_run at the same precision, bit for bit, and its reported
+    # fitness is the FP64 fitness of its genome
     import deac_amd as da
     fixture = tmp_path / "tiny-isf.bin"
     write_fixture(fixture)
     seeds = [5, 6, 7]
+    mflags = ["--first_moment", "0.5", "--third_moment", "2.0", "--third_moment_error", "0.5"] if moments else []
     cmd = deac_command(tmp_path, fixture, N="300", seed=str(seeds[0]),
                        extra=["--normalize", "--ks_precision", prec, "--track_stats",
-                              "--number_of_seeds", str(len(seeds))])
+                              "--number_of_seeds", str(len(seeds))] + mflags)
     r = run(cmd, tmp_path)
     assert r.returncode == 0, r.stdout
     assert "seeds run one after another" not in r.stdout, r.stdout
     tau, isf, err = np.fromfile(fixture).reshape(3, -1)
+    mkw = dict(first_moment=0.5, third_moment=2.0, third_moment_error=0.5) if moments else {}
     p = da.make_problem(tau, isf, err, da.frequency_grid(8, 4.0), 1.0, population_size=8, normalize=True,
-                        spectra_type="spfsf")
+                        spectra_type="spfsf", **mkw)
     with da.Batch([p] * len(seeds)) as b:
         outs = b.run(seeds, 300, 1.0, track_stats=True, precision=prec)
     res = tmp_path / "results"
