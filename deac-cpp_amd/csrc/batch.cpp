@@ -159,6 +159,10 @@ struct deac_batch {
 
 namespace {
 
+// 1 when a model's fitness has moment terms (first, third or -1st moment),
+// which the mixed kernels carry in an instantiation of their own
+int mixed_moments(const deac::Model &m) { return (m.use_first || m.use_third || m.use_nfm) ? 1 : 0; }
+
 // Kernel-side layout of one model; DEAC_ERR_UNSUPPORTED for shapes the batch
 // kernel does not run (the single-instance engine runs them).
 deac_status plan_model(BatchModel &b, std::string &why)
@@ -580,11 +584,6 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
             return DEAC_ERR_UNSUPPORTED;
         }
         const deac::Model &mi = h->models[h->model_of[i]]->m;
-        if (prec != DEAC_PRECISION_FP64 && mi.bdsf) {
-            deac::last_message() = "instance " + std::to_string(i) + ": mixed K.S in the batch does not take the "
-                              "detailed-balance spectra (run FP64, or the instance through deac_run)";
-            return DEAC_ERR_UNSUPPORTED;
-        }
         if (prec != DEAC_PRECISION_FP64 && mi.normalize && mi.renormalize) {
             // the scale of a trial needs all of it before its contraction,
             // which the mixed kernel's delta form does not take
@@ -696,9 +695,9 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
     auto key_of = [&](size_t i, bool eval) -> Key {
         const BatchModel &b = *h->models[h->model_of[i]];
         const int prec = options[i].ks_precision;
-        if (b.m.bdsf && !eval) return Key{kFoldKey, b.MF, b.NM, 0};
+        if (b.m.bdsf && !eval && prec == DEAC_PRECISION_FP64) return Key{kFoldKey, b.MF, b.NM, 0};
         if (prec == DEAC_PRECISION_FP64 || eval) return Key{DEAC_PRECISION_FP64, b.NT, b.NM, b.nrem > 0 ? 1 : 0};
-        return Key{prec, b.NQ, b.NM, (b.m.use_first || b.m.use_third) ? 1 : 0};  // last: moment constraints
+        return Key{prec, b.NQ, b.NM, mixed_moments(b.m)};  // last: moment terms in the fitness
     };
     auto launch = [&](const Key &k, const deac::BatchDesc *dd, int cnt, hipStream_t st) {
         if (std::get<0>(k) == DEAC_PRECISION_FP64)
@@ -908,7 +907,7 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
             std::map<Key, std::vector<size_t>> eb, ea;
             for (size_t k : mixed_slots) {
                 const size_t i = order[k];
-                if (h->models[h->model_of[i]]->m.bdsf) {
+                if (h->models[h->model_of[i]]->m.bdsf && options[i].ks_precision == DEAC_PRECISION_FP64) {
                     // exact folds: only the initial population's evaluation
                     if (first && !resume) eb[key_of(i, true)].push_back(k);
                     continue;
@@ -1146,12 +1145,11 @@ deac_status deac_batch_tune(deac_batch *h, const uint64_t *rng_states, const dea
     std::map<std::tuple<int, int, int, int>, int> shapes;
     for (size_t i = 0; i < n; i++) {
         const BatchModel &b = *h->models[h->model_of[i]];
-        if (b.m.bdsf)
+        if (b.m.bdsf && options[i].ks_precision == DEAC_PRECISION_FP64)
             shapes[{3, b.MF, b.NM, 0}] = 1;  // the fold kernel
         else
             shapes[{options[i].ks_precision, options[i].ks_precision == DEAC_PRECISION_FP64 ? b.NT : b.NQ, b.NM,
-                    options[i].ks_precision == DEAC_PRECISION_FP64 ? (b.nrem > 0 ? 1 : 0)
-                                                                   : (b.m.use_first || b.m.use_third ? 1 : 0)}] = 1;
+                    options[i].ks_precision == DEAC_PRECISION_FP64 ? (b.nrem > 0 ? 1 : 0) : mixed_moments(b.m)}] = 1;
     }
     const bool several_groups = shapes.size() > 1, several_models = h->models.size() > 1;
     std::vector<deac_batch_schedule> cand;

@@ -888,7 +888,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
     const int nchunk = d.nchunk;
     constexpr bool F64 = PREC == kPrecFp64 || PREC == kPrecFold;  // FP64 trial values
-    // mixed K.S instantiations with REM = 1 take moment constraints
+    // mixed K.S instantiations with REM = 1 take moment terms (first, third, -1st)
     constexpr bool MOM = (PREC == kPrecFp32 || PREC == kPrecBf16) && REM != 0;
     static_assert(!MOM || (C / 2) * Ppad * 2 <= kPartDoubles, "the partial moment sums fit their region");
     static_assert(kMaxT * kBatchMixedMaxP <= kOffPart, "the mixed model ISF stays below the partial sums");
@@ -1076,7 +1076,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         // FP64 evaluation after the slice recomputes every reported fitness
         // in the reference's order); the weights of the trial chunk are
         // loaded at the top of its step.
-        const bool wmom = MOM && kind == kPassEvolve;
+        const bool wmom = MOM && kind == kPassEvolve && (d.use_first || d.use_third);
         const gcdouble *const mtp = (const gcdouble *)uniform_ptr(d.mterm);
         double pm1[kT], pm3[kT];
         double2 wm1 = make_double2(0.0, 0.0), wm3 = make_double2(0.0, 0.0);
@@ -1518,7 +1518,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
 {
     constexpr int Ppad = 16 * NM;
     constexpr bool F64 = PREC == kPrecFp64 || PREC == kPrecFold;  // FP64 trial values
-    constexpr bool MOM = (PREC == kPrecFp32 || PREC == kPrecBf16) && REM != 0;  // mixed, moment constraints
+    constexpr bool MOM = (PREC == kPrecFp32 || PREC == kPrecBf16) && REM != 0;  // mixed, moment terms
     extern __shared__ __attribute__((aligned(16))) double lds[];  // 16-byte aligned: ds_*_b128 at full rate
     __shared__ BatchShared sh;
     const BatchDesc &d = descs[blockIdx.x];
@@ -1633,6 +1633,22 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
                 }
                 __syncthreads();
             }
+            if constexpr (MOM) {
+                // the same term from the mixed model ISF E = epar + dE 2^ex[t]
+                // (each value formed as the residuals below form it)
+                if (d.use_nfm) {
+                    if (wave == kWorker0 && lane < P) {
+                        const gcdouble *dt = (const gcdouble *)d.nfm_dt;
+                        double mm = 0.0;
+                        for (int t = 0; t < d.T; t++) {
+                            const double e = epar[t * Ppad + lane] + lds[t * Ppad + lane] * sh.kscale[t];
+                            mm = ((d.T & 1) && t == d.T - 1) ? fma(e, dt[t], mm) : mm + e * dt[t];
+                        }
+                        sh.nfm[lane] = mm;
+                    }
+                    __syncthreads();
+                }
+            }
             if constexpr (F64) {
                 for (int k = threadIdx.x; k < d.T * Ppad; k += kBatchThreads) {
                     const int t = k / Ppad;
@@ -1669,7 +1685,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
                     double fit = chi / static_cast<double>(d.T);
                     if constexpr (MOM) {
                         // the trial's moments: the workers' partial sums, in lane order
-                        if (kind == kPassEvolve) {
+                        if (kind == kPassEvolve && (d.use_first || d.use_third)) {
                             constexpr int H = 64 / NM;  // partial sums per member (C / 2)
                             const double *q = lds + kOffPart + lane * H * 2;
                             double a = 0.0, b = 0.0;
@@ -1681,7 +1697,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
                             sh.mom[lane][1] = b;
                         }
                     }
-                    if (F64 && d.use_nfm) {  // deac.cpp:961-965, 1543-1547
+                    if ((F64 || MOM) && d.use_nfm) {  // deac.cpp:961-965, 1543-1547
                         const double q = (d.nfm_value - sh.nfm[lane]) / d.nfm_error;
                         fit = fma(q, q, fit);
                     }

@@ -204,7 +204,7 @@ def test_bdsf_fold_shapes_match_oracle(name):
         assert o["rng"] == r["rng"] and o["generation"] == r["generation"], sd
 
 
-def test_bdsf_batch_continues_and_refuses_mixed_precision():
+def test_bdsf_batch_continues():
     case = gc.load("bdsf_standard")
     a = case["meta"]["args"]
     p = da.make_problem(case["tau"], case["isf"], case["err"], gc.frequency_of(case, da.frequency_grid), a["T"],
@@ -213,8 +213,43 @@ def test_bdsf_batch_continues_and_refuses_mixed_precision():
         whole = b.run([7, 8], 80, 0.0, True)
         first = b.run([7, 8], 31, 0.0, True)
         rest = b.continue_run(50, 0.0, True)
-        with pytest.raises(da.DeacError, match="detailed-balance"):
-            b.run([7, 8], 10, 0.0, precision="fp32")
     for w, f, r in zip(whole, first, rest):
         assert gc.bit_diff(np.concatenate([f["minimum"], r["minimum"][1:]]), w["minimum"]) is None
         assert gc.bit_diff(r["dsf"], w["dsf"]) is None and r["rng"] == w["rng"]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("nfm", [True, False])
+def test_bdsf_mixed_batch(prec, nfm):
+    # detailed-balance instances at FP32 / BF16x3 K.S run on the mixed kernel
+    # (delta form; the -1st-moment term from the mixed model ISF): the
+    # initial fitness is the FP64 evaluation's, generation 0's best matches
+    # the FP64 (fold) batch to the precision of the contraction, where the
+    # -1st-moment term moves it by far more, and runs repeat bit for bit
+    print(f"[gpu] bdsf mixed batch {prec} nfm={nfm}", flush=True)
+    T, G, P = 65, 513, 24
+    tau = np.linspace(0.0, 0.4, T)
+    isf = 0.6 * np.exp(-1.1 * tau) + 0.4 * np.exp(-0.3 * tau)
+    err = 2e-3 * isf
+
+    def problem(use_nfm):
+        return da.make_problem(tau, isf, err, da.frequency_grid(G, 16.0), 1.25, population_size=P,
+                               spectra_type="bdsf", use_negative_first_moment=use_nfm, normalize=True)
+    seeds, N = [3, 4, 5, 6], 60
+    with da.Batch([problem(nfm)] * len(seeds)) as b:
+        ref = b.run(seeds, N, 0.0, True, precision="fp64")
+        out = b.run(seeds, N, 0.0, True, precision=prec)
+        again = b.run(seeds, N, 0.0, True, precision=prec)
+    if nfm:
+        with da.Batch([problem(False)] * len(seeds)) as b:
+            bare = b.run(seeds, N, 0.0, True, precision="fp64")
+        assert all(abs(z["minimum"][1] - r["minimum"][1]) > 1e-3 * r["minimum"][1] for z, r in zip(bare, ref))
+    for k, (o, r, a) in enumerate(zip(out, ref, again)):
+        assert o["minimum"][0] == r["minimum"][0], k
+        assert o["rng"] == r["rng"], k
+        assert abs(o["minimum"][1] - r["minimum"][1]) <= 1e-5 * r["minimum"][1], (k, o["minimum"][1], r["minimum"][1])
+        assert gc.bit_diff(o["minimum"], a["minimum"]) is None and gc.bit_diff(o["dsf"], a["dsf"]) is None, k
+        assert o["dsf"].shape == r["dsf"].shape
+    fin = np.median([o["minimum_fitness"] for o in out])
+    finr = np.median([r["minimum_fitness"] for r in ref])
+    assert 0.3 < fin / finr < 3.0, (fin, finr)
