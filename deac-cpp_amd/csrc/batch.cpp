@@ -159,9 +159,13 @@ struct deac_batch {
 
 namespace {
 
-// 1 when a model's fitness has moment terms (first, third or -1st moment),
-// which the mixed kernels carry in an instantiation of their own
-int mixed_moments(const deac::Model &m) { return (m.use_first || m.use_third || m.use_nfm) ? 1 : 0; }
+// 1 when a model has moment terms (first, third or -1st moment in the
+// fitness, or the zeroth moment of per-generation renormalization), which
+// the mixed kernels carry in an instantiation of their own
+int mixed_moments(const deac::Model &m)
+{
+    return (m.use_first || m.use_third || m.use_nfm || (m.normalize && m.renormalize)) ? 1 : 0;
+}
 
 // Kernel-side layout of one model; DEAC_ERR_UNSUPPORTED for shapes the batch
 // kernel does not run (the single-instance engine runs them).
@@ -581,14 +585,6 @@ deac_status batch_run_impl(deac_batch *h, uint64_t *rng_states, const deac_run_o
         }
         if (prec != DEAC_PRECISION_FP64 && h->models[h->model_of[i]]->NQ == 0) {
             deac::last_message() = "instance " + std::to_string(i) + ": no mixed K.S kernel for this shape";
-            return DEAC_ERR_UNSUPPORTED;
-        }
-        const deac::Model &mi = h->models[h->model_of[i]]->m;
-        if (prec != DEAC_PRECISION_FP64 && mi.normalize && mi.renormalize) {
-            // the scale of a trial needs all of it before its contraction,
-            // which the mixed kernel's delta form does not take
-            deac::last_message() = "instance " + std::to_string(i) + ": mixed K.S in the batch does not renormalize "
-                              "each generation (run FP64, or the instance through deac_run)";
             return DEAC_ERR_UNSUPPORTED;
         }
     }

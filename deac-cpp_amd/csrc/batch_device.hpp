@@ -1061,7 +1061,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         // fabs of a DE value, or not (allow_negative): an and of the high word
         const uint32_t absmask = __builtin_amdgcn_readfirstlane(d.allow_neg) ? 0xffffffffu : 0x7fffffffu;
         constexpr int kWpc = C / 32;  // mask words per member and chunk
-        const bool renorm = F64 && d.renorm;
+        const bool renorm = (F64 || MOM) && d.renorm;
         const bool trials = kind == kPassEvolve || kind == kPassNorm;
         const bool rebuild = kind == kPassApply || kind == kPassNorm || (kind == kPassEvolve && !renorm);
         const long pg = trials ? g - 1 : g;                   // generation whose accepted trials are rebuilt
@@ -1077,6 +1077,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         // in the reference's order); the weights of the trial chunk are
         // loaded at the top of its step.
         const bool wmom = MOM && kind == kPassEvolve && (d.use_first || d.use_third);
+        const bool wnorm = MOM && kind == kPassNorm;  // the trials' zeroth moments (renormalization)
         const gcdouble *const mtp = (const gcdouble *)uniform_ptr(d.mterm);
         double pm1[kT], pm3[kT];
         double2 wm1 = make_double2(0.0, 0.0), wm3 = make_double2(0.0, 0.0);
@@ -1244,10 +1245,10 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
             // in the FP64 kernel their registers hold the kernel-tile ring)
             const bool ld = F64 && kind != kPassPre && c < nchunk;
             if constexpr (MOM) {
-                if (wmom && c >= 3 && c - 3 < nchunk) {
-                    const size_t k = static_cast<size_t>(c - 3) * C + p;
+                if ((wmom || wnorm) && c >= 3 && c - 3 < nchunk) {
+                    const size_t k = static_cast<size_t>(c - 3) * C + p + (wnorm ? 2 * static_cast<size_t>(nLpad) : 0);
                     wm1 = make_double2(mtp[k], mtp[k + 1]);
-                    wm3 = make_double2(mtp[nLpad + k], mtp[nLpad + k + 1]);
+                    if (wmom) wm3 = make_double2(mtp[nLpad + k], mtp[nLpad + k + 1]);
                 }
             }
             if (ld) {
@@ -1289,17 +1290,6 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     }
                 } else {
                     de_stage(buf, sh.mring[ct % 4], mo, od, g, ct, sh.y0t, v, !F64 && !MOM);
-                    if constexpr (MOM) {
-                        // the trial's moment terms, then dS = trial - parent
-#pragma unroll
-                        for (int u = 0; u < kT; u++) {
-                            pm1[u] += v[u].x * wm1.x + v[u].y * wm1.y;
-                            pm3[u] += v[u].x * wm3.x + v[u].y * wm3.y;
-                            const double2 x0 = *reinterpret_cast<const double2 *>(buf + own + u * 1024);
-                            v[u].x -= x0.x;
-                            v[u].y -= x0.y;
-                        }
-                    }
                     if (renorm && kind == kPassEvolve) {
                         // the trial times zeroth_moment / norm (deac.cpp:1353-1360)
 #pragma unroll
@@ -1307,6 +1297,18 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                             const double sc = sh.rscale[g & 1][fm(u)];
                             v[u].x *= sc;
                             v[u].y *= sc;
+                        }
+                    }
+                    if constexpr (MOM) {
+                        // the trial's moment terms (or, in the renormalization
+                        // pass, its zeroth moment), then dS = trial - parent
+#pragma unroll
+                        for (int u = 0; u < kT; u++) {
+                            if (wmom || wnorm) pm1[u] += v[u].x * wm1.x + v[u].y * wm1.y;
+                            if (wmom) pm3[u] += v[u].x * wm3.x + v[u].y * wm3.y;
+                            const double2 x0 = *reinterpret_cast<const double2 *>(buf + own + u * 1024);
+                            v[u].x -= x0.x;
+                            v[u].y -= x0.y;
                         }
                     }
                 }
@@ -1365,7 +1367,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
         }
         __syncthreads();
         if constexpr (MOM) {
-            if (wmom) {
+            if (wmom || wnorm) {
 #pragma unroll
                 for (int u = 0; u < kT; u++)
                     if (mrow(u) < nP) {
@@ -1591,7 +1593,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         // per-generation renormalization (deac.cpp:1342-1364): the trials'
         // zeroth moments first, then the pass that scales and contracts them
         // (one call site of the pass, so its code appears once)
-        const bool norm = F64 && d.renorm && kind == kPassEvolve && !normed;
+        const bool norm = (F64 || MOM) && d.renorm && kind == kPassEvolve && !normed;
         // a resumed launch has gen0's masks on the device and walked gen0 + 1
         // in the prologue: its mask pass has no steps
         const bool none = kind == kPassPre && d.resume && !evaluate;
@@ -1601,7 +1603,16 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchDesc *_
         const long wgen = evaluate || norm || none ? -1 : kind == kPassEvolve ? g + 2 : kind == kPassApply ? -1 : g0 + 1;
         run_pass<PREC, NT, NM, REM>(d, sh, lds, pk, pg, mgen, wgen);
         if (norm) {
-            if (threadIdx.x < P) sh.rscale[g & 1][threadIdx.x] = d.zeroth / sh.norm[threadIdx.x];
+            if (threadIdx.x < P) {
+                double nrm = sh.norm[threadIdx.x];
+                if constexpr (MOM) {
+                    // the mixed kernel's workers left partial sums, in lane order
+                    const double *q = lds + kOffPart + threadIdx.x * (64 / NM) * 2;
+                    nrm = 0.0;
+                    for (int j = 0; j < 64 / NM; j++) nrm += q[2 * j];
+                }
+                sh.rscale[g & 1][threadIdx.x] = d.zeroth / nrm;
+            }
             __syncthreads();
             normed = true;
             continue;

@@ -473,11 +473,44 @@ def test_batch_renormalization_matches_oracle(T, G, P, kw, temp, slice_len, monk
         assert o["rng"] == r["rng"] and o["generation"] == r["generation"]
 
 
-def test_batch_renormalization_refuses_mixed_precision():
-    t, f, s = _synthetic_arrays(33)
-    prob = da.make_problem(t, -f, s, da.frequency_grid(64, 32.0), 1.0, normalize=True, renormalize=True,
-                           spectra_type="spfsf")
-    with da.Batch([prob]) as b:
-        with pytest.raises(da.DeacError, match="renormalize") as e:
-            b.run([1], 10, 0.0, precision="bf16x3")
-        assert e.value.status == 5
+# mixed-kernel shapes (at most 32 members) where renormalization moves
+# generation 0's best fitness by more than 1e-3 (it barely does for the
+# hyperbolic model); the zero-temperature case has a first moment too
+MIXED_RENORM = [
+    (33, 200, 8, dict(spectra_type="spfsf"), 1.0),
+    (65, 301, 32, dict(spectra_type="spbsf"), 1.0),
+    (17, 90, 12, dict(spectra_type="spfsf", zero_temperature=True, first_moment=0.4), 0.0),
+    (50, 150, 24, dict(spectra_type="spfsf"), 1.0),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("T,G,P,kw,temp", MIXED_RENORM)
+def test_batch_renormalization_mixed_precision(prec, T, G, P, kw, temp):
+    # the mixed kernels' renormalization pass: the workers sum each trial's
+    # zeroth moment in FP64 (their own order), and the evolution pass scales
+    # the FP64 trial before its change from the parent goes to the
+    # contraction.  Generation 0's best matches the FP64 batch to the
+    # precision of the contraction where renormalization moves it by far
+    # more; runs repeat bit for bit.
+    print(f"[gpu] mixed batch {prec} with renormalization T={T} G={G} P={P} {kw}", flush=True)
+    t, f, s = _synthetic_arrays(T)
+    if kw["spectra_type"].startswith("sp"):
+        f, s = -f, s
+
+    def problem(renorm):
+        return da.make_problem(t, f, s, da.frequency_grid(G, 32.0), temp, normalize=True, renormalize=renorm,
+                               population_size=P, **kw)
+    N, seeds = 60, [7, 8, 9]
+    with da.Batch([problem(True)] * len(seeds)) as b:
+        ref = b.run(seeds, N, 0.0, True, precision="fp64")
+        out = b.run(seeds, N, 0.0, True, precision=prec)
+        again = b.run(seeds, N, 0.0, True, precision=prec)
+    with da.Batch([problem(False)] * len(seeds)) as b:
+        bare = b.run(seeds, N, 0.0, True, precision="fp64")
+    assert all(abs(z["minimum"][1] - r["minimum"][1]) > 1e-3 * r["minimum"][1] for z, r in zip(bare, ref))
+    for k, (o, r, a) in enumerate(zip(out, ref, again)):
+        assert o["minimum"][0] == r["minimum"][0], k
+        assert o["rng"] == r["rng"], k
+        assert abs(o["minimum"][1] - r["minimum"][1]) <= 1e-5 * r["minimum"][1], (k, o["minimum"][1], r["minimum"][1])
+        assert gc.bit_diff(o["minimum"], a["minimum"]) is None and gc.bit_diff(o["dsf"], a["dsf"]) is None, k
