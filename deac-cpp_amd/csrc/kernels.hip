@@ -67,6 +67,32 @@ __device__ __forceinline__ bool grid_barrier(unsigned *bar, unsigned &target, in
     return timed_out == 0;
 }
 
+// The walk's stream is wave-uniform: its step as scalar (SALU) operations,
+// the rotate as two 64-bit shifts (the funnel-shift form of xnext would put
+// it on the VALU and bring it back with v_readfirstlane on every draw).
+__device__ __forceinline__ uint64_t xnext_uniform(uint64_t (&s)[4])  // rng.hpp:25-39
+{
+    const uint64_t s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
+    const uint64_t out = s0 + s3;
+    const uint64_t t = s1 << 17;
+    const uint64_t n3 = s3 ^ s1;
+    s[1] = s1 ^ s2 ^ s0;
+    s[0] = s0 ^ n3;
+    s[2] = s2 ^ s0 ^ t;
+    s[3] = (n3 << 45) | (n3 >> 19);
+    return out;
+}
+
+// mod_p for a wave-uniform draw: the remainder before the correction is
+// below 2P < 2^32, so the correction compares 32-bit words (SALU)
+__device__ __forceinline__ uint32_t mod_p_uniform(uint64_t x, uint64_t magic, uint32_t P)
+{
+    const uint64_t q = __umul64hi(x, magic);
+    uint32_t r = static_cast<uint32_t>(x - q * P);
+    if (r >= P) r -= P;
+    return r;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- walk
@@ -90,13 +116,13 @@ __global__ __launch_bounds__(64) void walk_kernel(WalkArgs a)
             uint32_t flags = 0;
             double crv[2] = {0.0, 0.0}, fv[2] = {0.0, 0.0};
             for (int sg = 0; sg < a.nsign; sg++) {
-                if (to_unit(xnext(s)) < a.tau_cr) {
+                if (to_unit(xnext_uniform(s)) < a.tau_cr) {
                     flags |= 1u << (2 * sg);
-                    crv[sg] = to_unit(xnext(s));
+                    crv[sg] = to_unit(xnext_uniform(s));
                 }
-                if (to_unit(xnext(s)) < a.tau_f) {
+                if (to_unit(xnext_uniform(s)) < a.tau_f) {
                     flags |= 2u << (2 * sg);
-                    fv[sg] = 2.0 * to_unit(xnext(s));
+                    fv[sg] = 2.0 * to_unit(xnext_uniform(s));
                 }
             }
             if (lane == 0) {
@@ -111,9 +137,9 @@ __global__ __launch_bounds__(64) void walk_kernel(WalkArgs a)
         for (int i = 0; i < a.P; i++) {
             const uint32_t i0 = static_cast<uint32_t>(i);
             uint32_t m0 = i0, m1 = i0, m2 = i0;
-            while (m0 == i0) m0 = mod_p(xnext(s), a.mod_magic, P);
-            while (m1 == i0 || m1 == m0) m1 = mod_p(xnext(s), a.mod_magic, P);
-            while (m2 == i0 || m2 == m0 || m2 == m1) m2 = mod_p(xnext(s), a.mod_magic, P);
+            while (m0 == i0) m0 = mod_p_uniform(xnext_uniform(s), a.mod_magic, P);
+            while (m1 == i0 || m1 == m0) m1 = mod_p_uniform(xnext_uniform(s), a.mod_magic, P);
+            while (m2 == i0 || m2 == m0 || m2 == m1) m2 = mod_p_uniform(xnext_uniform(s), a.mod_magic, P);
             if (lane == 0) {
                 ctl[i].mutant[0] = m0;
                 ctl[i].mutant[1] = m1;
@@ -122,11 +148,19 @@ __global__ __launch_bounds__(64) void walk_kernel(WalkArgs a)
             }
             // s <- T^(nsign*G) s: lane l forms output bits l, 64+l, 128+l,
             // 192+l as parities of (row & s); ballots assemble the words.
+            // (r & s) ^ acc is one v_bitop3 per 32-bit half (truth table
+            // 0x6a over the inputs 0xf0, 0xcc, 0xaa).
             uint64_t out[4];
             for (int w = 0; w < 4; w++) {
                 const uint64_t *r = rows + (w * 64 + lane) * 4;
-                const uint64_t x = (r[0] & s[0]) ^ (r[1] & s[1]) ^ (r[2] & s[2]) ^ (r[3] & s[3]);
-                out[w] = __ballot(parity64(x));
+                uint32_t lo = static_cast<uint32_t>(r[0]) & static_cast<uint32_t>(s[0]);
+                uint32_t hi = static_cast<uint32_t>(r[0] >> 32) & static_cast<uint32_t>(s[0] >> 32);
+                for (int k = 1; k < 4; k++) {
+                    lo = __builtin_amdgcn_bitop3_b32(static_cast<uint32_t>(r[k]), static_cast<uint32_t>(s[k]), lo, 0x6a);
+                    hi = __builtin_amdgcn_bitop3_b32(static_cast<uint32_t>(r[k] >> 32), static_cast<uint32_t>(s[k] >> 32),
+                                                     hi, 0x6a);
+                }
+                out[w] = __ballot(__popc(lo ^ hi) & 1);
             }
             for (int w = 0; w < 4; w++) s[w] = out[w];
         }
