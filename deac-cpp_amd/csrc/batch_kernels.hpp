@@ -34,7 +34,7 @@ struct BatchDesc {
     const double *kscale;      // mixed K.S: [T] 2^ex[t]
     const double *obs, *sig;   // [T] ISF and its error
     const double *mterm;       // [3][Lpad] first-moment, third-moment and normalization weights (zero past L),
-                               // or null (FP64 kernel)
+                               // or null when the model uses none of them
     const double *kfold;       // bdsf fold kernel: [Lpad][fold_rows], entry (k, t) = K row t at position k
                                // (zero past T and L)
     const double *nfm_dt;      // [T] trapezoid weights of the -1st-moment term (use_nfm)
@@ -73,8 +73,8 @@ struct BatchDesc {
     int T, G, nsign, L, Lpad, Lw, nchunk, ntt, nrem, ntm, S, Lsub, allow_neg;
     int P;                     // members in the workgroup, Q Pi
     int Q, Pi;                 // instances and members per instance
-    int use_first, use_third;  // moment constraints (deac.cpp:1420-1477, 1550-1555), FP64 kernel only
-    int renorm;                // renormalize each generation's trials (deac.cpp:1342-1364), FP64 kernel only
+    int use_first, use_third;  // moment constraints (deac.cpp:1420-1477, 1550-1555); mixed: the REM = 1 kernels
+    int renorm;                // renormalize each generation's trials (deac.cpp:1342-1364); mixed: REM = 1
     int use_nfm;               // -1st-moment term in the fitness (bdsf above zero temperature, deac.cpp:961-965)
     int fold_rows;             // bdsf fold kernel: row stride of kfold, R (+ 4 with extra timeslices)
 };
