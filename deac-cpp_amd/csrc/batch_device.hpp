@@ -87,7 +87,9 @@ constexpr int kMaxT = 4 * 4 * 16 + kBatchMaxRem;  // timeslices: 16 full tiles (
 enum { kPassInit = 0, kPassEvolve = 1, kPassApply = 2, kPassPre = 3, kPassNorm = 4, kPassNone = 5 };
 enum { kPrecFp64 = 0, kPrecFp32 = 1, kPrecBf16 = 2 };  // DEAC_PRECISION_*
 enum { kPrecFold = 3 };  // FP64 throughout, the trials' contraction a fold of rounded products (bdsf)
-constexpr int kEk = (kMaxT * kBatchMixedMaxP + kBatchThreads - 1) / kBatchThreads;  // chi^2 terms per thread (mixed)
+// model ISF values of a mixed-kernel workgroup: T <= 260 at 32 members, T <= 132 at 64
+constexpr int kMixedIsf = kMaxT * 32 > 132 * kBatchMaxP ? kMaxT * 32 : 132 * kBatchMaxP;
+constexpr int kEk = (kMixedIsf + kBatchThreads - 1) / kBatchThreads;  // chi^2 terms per thread (mixed)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -574,7 +576,9 @@ struct Stager {
 // tile, member tile), NQ of them.  With two member tiles, waves 0 and 1 take
 // the even kernel tiles 0, 2, 4, .. and waves 2 and 3 the odd ones, each wave
 // one member tile (its parity), so every wave reads one B operand per block;
-// with one member tile, wave w takes tiles w, w + 4, ..  The kernel rows past
+// with one member tile, wave w takes tiles w, w + 4, ..; with four (33 to 64
+// members, T <= 132), wave w takes member tile w and every kernel tile.  The
+// kernel rows past
 // T are zero rows of the last tile (no remainder chains), and pairs past the
 // last tile compute on it and are discarded.
 //
@@ -595,12 +599,12 @@ __device__ __forceinline__ void mixed_chains(const BatchDesc &d, BatchShared &sh
     constexpr int KD = KDW >= UPC ? UPC : KDW >= 4 ? 4 : KDW >= 2 ? 2 : 1;  // divides UPC
     static_assert(UPC % KD == 0, "the ring depth divides the units of a chunk");
     const int ntm = d.ntm;
-    const int mt = NM == 2 ? (wave & 1) : 0;
+    const int mt = NM == 4 ? wave : NM == 2 ? (wave & 1) : 0;
     int tq[NQ];
     bool tv[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-        const int t = NM == 2 ? (wave >> 1) + 2 * q : wave + 4 * q;
+        const int t = NM == 4 ? q : NM == 2 ? (wave >> 1) + 2 * q : wave + 4 * q;
         tv[q] = chains && t < ntm;
         tq[q] = min(t, ntm - 1);
     }
@@ -891,7 +895,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
     // mixed K.S instantiations with REM = 1 take moment terms (first, third, -1st)
     constexpr bool MOM = (PREC == kPrecFp32 || PREC == kPrecBf16) && REM != 0;
     static_assert(!MOM || (C / 2) * Ppad * 2 <= kPartDoubles, "the partial moment sums fit their region");
-    static_assert(kMaxT * kBatchMixedMaxP <= kOffPart, "the mixed model ISF stays below the partial sums");
+    static_assert(kMixedIsf <= kOffPart, "the mixed model ISF stays below the partial sums");
     const bool chains = kind == kPassInit || kind == kPassEvolve;
     const int nsteps = kind == kPassApply ? nchunk + 2 : kind == kPassNone ? 0 : nchunk + 4;
 
@@ -1324,7 +1328,7 @@ __device__ __forceinline__ void run_pass(const BatchDesc &d, BatchShared &sh, do
                     float *tf = reinterpret_cast<float *>(tb);
 #pragma unroll
                     for (int u = 0; u < kT; u++) {
-                        const int t0 = NM == 2 ? td0 + 64 * u : td0 ^ (8 * u);
+                        const int t0 = NM >= 2 ? td0 + 32 * NM * u : td0 ^ (8 * u);
                         tf[t0] = static_cast<float>(v[u].x);
                         tf[t0 + 16] = static_cast<float>(v[u].y);
                     }

@@ -16,8 +16,7 @@ constexpr int kBatchWaves = 16;             // 4 chain, 8 worker, 1 walk, 3 mask
 constexpr int kBatchThreads = kBatchWaves * 64;
 constexpr int kBatchMaskLanes = 192;        // 3 mask wavefronts
 constexpr int kBatchChunkDoubles = 2048;    // members (padded) x genome positions per chunk
-constexpr int kBatchMaxP = 64;             // members per workgroup (the mixed kernel takes 32)
-constexpr int kBatchMixedMaxP = 32;
+constexpr int kBatchMaxP = 64;             // members per workgroup (33 to 64: T <= 132)
 constexpr int kBatchRegionDoubles = 6 * kBatchChunkDoubles;  // 4 population + 2 trial buffers
 constexpr int kBatchMaxRem = 4;             // kernel rows past the last full tile run on 4x4x4 MFMAs
 constexpr int kBatchPackMax = 4;            // instances of one model stacked in one workgroup (FP64)

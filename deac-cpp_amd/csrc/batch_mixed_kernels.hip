@@ -31,7 +31,7 @@ hipError_t launch_mixed_nq(const BatchDesc *descs, int count, int nq, hipStream_
     case 2: return launch_mixed_one<PREC, 2, NM, MOM>(descs, count, s);
     case 3: return launch_mixed_one<PREC, 3, NM, MOM>(descs, count, s);
     case 5: return launch_mixed_one<PREC, 5, NM, MOM>(descs, count, s);
-    case 9: if constexpr (NM == 2) return launch_mixed_one<PREC, 9, NM, MOM>(descs, count, s);
+    case 9: if constexpr (NM >= 2) return launch_mixed_one<PREC, 9, NM, MOM>(descs, count, s);
     }
     return hipErrorInvalidValue;
 }
@@ -39,7 +39,9 @@ hipError_t launch_mixed_nq(const BatchDesc *descs, int count, int nq, hipStream_
 template <int PREC, int MOM>
 hipError_t launch_mixed_nm(const BatchDesc *descs, int count, int nq, int nm, hipStream_t s)
 {
-    return nm == 1 ? launch_mixed_nq<PREC, 1, MOM>(descs, count, nq, s) : launch_mixed_nq<PREC, 2, MOM>(descs, count, nq, s);
+    if (nm == 1) return launch_mixed_nq<PREC, 1, MOM>(descs, count, nq, s);
+    if (nm == 2) return launch_mixed_nq<PREC, 2, MOM>(descs, count, nq, s);
+    return launch_mixed_nq<PREC, 4, MOM>(descs, count, nq, s);
 }
 
 }  // namespace
@@ -47,18 +49,18 @@ hipError_t launch_mixed_nm(const BatchDesc *descs, int count, int nq, int nm, hi
 int batch_mixed_nq(int ntm, int nm)
 {
     // pairs per chain wave: ceil(ntm / 2) with two member tiles (a wave keeps
-    // one member tile), ceil(ntm / 4) with one
-    if (nm > 2) return 0;  // at most 32 members
-    const int need = nm == 2 ? (ntm + 1) / 2 : (ntm + 3) / 4;
+    // one member tile), ceil(ntm / 4) with one, ntm with four (a wave keeps
+    // one member tile and takes every kernel tile)
+    const int need = nm == 4 ? ntm : nm == 2 ? (ntm + 1) / 2 : (ntm + 3) / 4;
     for (int nq : kBatchMixedNq)
-        if (nq >= need && (nm == 2 || nq <= 5)) return nq;
+        if (nq >= need && (nm >= 2 || nq <= 5)) return nq;
     return 0;
 }
 
 hipError_t launch_batch_mixed(const BatchDesc *descs, int count, int precision, int nq, int nm, int moments,
                               hipStream_t s)
 {
-    if (nm != 1 && nm != 2) return hipErrorInvalidValue;
+    if (nm != 1 && nm != 2 && nm != 4) return hipErrorInvalidValue;
     if (precision == kPrecFp32)
         return moments ? launch_mixed_nm<kPrecFp32, 1>(descs, count, nq, nm, s)
                        : launch_mixed_nm<kPrecFp32, 0>(descs, count, nq, nm, s);

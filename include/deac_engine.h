@@ -212,11 +212,10 @@ deac_status deac_last_run_profile(const deac_handle *handle, deac_run_profile *p
  * Each instance has its own generation count, stop test and precision; an
  * instance that stops leaves the launch.  Identical problems share one
  * device copy of the kernel matrix.  Population size at most 64 (33 to 64:
- * FP64 and T <= 132; mixed K.S takes at most 32); moment constraints,
- * per-generation renormalization and DEAC_BDSF (T <= 128) run at every
- * precision (mixed: the trials' moment terms and zeroth moments summed in
- * FP64 in the kernel's own order, every reported fitness from the FP64
- * evaluation). */
+ * T <= 132); moment constraints, per-generation renormalization and
+ * DEAC_BDSF (T <= 128) run at every precision (mixed: the trials' moment
+ * terms and zeroth moments summed in FP64 in the kernel's own order, every
+ * reported fitness from the FP64 evaluation). */
 typedef struct deac_batch deac_batch;
 
 deac_status deac_batch_create(const deac_problem *problems, size_t count, deac_batch **out, const char **message);

@@ -302,12 +302,6 @@ def test_cli_seed_batch_64_members_matches_single_runs(tmp_path):
 
 
 @pytest.mark.gpu
-def test_cli_mixed_precision_seeds_with_48_members_run_seed_by_seed(tmp_path):
-    # the mixed batch kernel takes at most 32 members
-    _seed_batch_vs_single_runs(tmp_path, ["--ks_precision", "bf16x3"], [3, 4], expect_batched=False, P="48")
-
-
-@pytest.mark.gpu
 def test_cli_seed_batch_with_renormalization_matches_single_runs(tmp_path):
     # per-generation renormalization (single-particle spectra) in the batch kernel
     _seed_batch_vs_single_runs(tmp_path, ["--normalize", "--single_particle_fermionic_spectral_function"], [3, 4],
@@ -321,8 +315,9 @@ def test_cli_seed_batch_unbatchable_runs_seed_by_seed(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec,moments", [("fp32", False), ("bf16x3", False), ("fp32", True)])
-def test_cli_mixed_precision_seed_batch(tmp_path, prec, moments):
+@pytest.mark.parametrize("prec,moments,P", [("fp32", False, 8), ("bf16x3", False, 8), ("fp32", True, 8),
+                                             ("bf16x3", False, 48)])
+def test_cli_mixed_precision_seed_batch(tmp_path, prec, moments, P):
     # fp32 / bf16x3 seeds run in the batch kernel (delta form), with moment
     # constraints too: each seed's files are that seed's instance of
     # deac_batch_run at the same precision, bit for bit, and its reported
@@ -332,7 +327,7 @@ def test_cli_mixed_precision_seed_batch(tmp_path, prec, moments):
     write_fixture(fixture)
     seeds = [5, 6, 7]
     mflags = ["--first_moment", "0.5", "--third_moment", "2.0", "--third_moment_error", "0.5"] if moments else []
-    cmd = deac_command(tmp_path, fixture, N="300", seed=str(seeds[0]),
+    cmd = deac_command(tmp_path, fixture, N="300", P=str(P), seed=str(seeds[0]),
                        extra=["--normalize", "--ks_precision", prec, "--track_stats",
                               "--number_of_seeds", str(len(seeds))] + mflags)
     r = run(cmd, tmp_path)
@@ -340,7 +335,7 @@ def test_cli_mixed_precision_seed_batch(tmp_path, prec, moments):
     assert "seeds run one after another" not in r.stdout, r.stdout
     tau, isf, err = np.fromfile(fixture).reshape(3, -1)
     mkw = dict(first_moment=0.5, third_moment=2.0, third_moment_error=0.5) if moments else {}
-    p = da.make_problem(tau, isf, err, da.frequency_grid(8, 4.0), 1.0, population_size=8, normalize=True,
+    p = da.make_problem(tau, isf, err, da.frequency_grid(8, 4.0), 1.0, population_size=P, normalize=True,
                         spectra_type="spfsf", **mkw)
     with da.Batch([p] * len(seeds)) as b:
         outs = b.run(seeds, 300, 1.0, track_stats=True, precision=prec)
