@@ -259,6 +259,44 @@ def small_population_batch(da, gens):
     return out
 
 
+def wide_population_batch(da, gens):
+    """Seeds with 64-member populations (four member tiles per workgroup):
+    256 seeds of the G=4096, T=65 workload at P=64 for `gens` timed
+    generations after a 100-generation run, at FP64, FP32 and BF16x3 K.S;
+    kernel time of the continued launch."""
+    t, f, s = load_isf()
+    n, P = 256, 64
+    prob = da.make_problem(t, f, s, da.frequency_grid(G, OMEGA_MAX), TEMPERATURE, population_size=P, normalize=True,
+                           spectra_type="bfull")
+    seeds = list(range(1, n + 1))
+    out = {"workload": f"{n} seeds x (G=4096 per sign, T=65, P=64, bfull, --normalize), {gens} generations"}
+    with da.Batch([prob] * n) as b:
+        for prec in ("fp64", "fp32", "bf16x3"):
+            b.run(seeds, 101, 0.0, track_stats=False, precision=prec)
+            b.continue_run(gens + 1, 0.0, track_stats=False, precision=prec)
+            prof = b.profile
+            ms = prof["evolve_ms"] / gens
+            out[prec] = {"kernel_ms_per_generation": ms, "genome_evals_per_s": P * n / ms * 1e3}
+    out["fp32_over_fp64"] = out["fp32"]["genome_evals_per_s"] / out["fp64"]["genome_evals_per_s"]
+    return out
+
+
+def plain_default_population(da, gens):
+    """The plain path at the reference CLI's default population (-P 512,
+    deac.cpp:1935-1938): one instance of the G=4096, T=65 workload, exact
+    FP64, `gens` timed generations after 20."""
+    t, f, s = load_isf()
+    P = 512
+    prob = da.make_problem(t, f, s, da.frequency_grid(G, OMEGA_MAX), TEMPERATURE, population_size=P, normalize=True,
+                           spectra_type="bfull")
+    with da.Engine(prob) as e:
+        e.run(1, 21, 0.0, track_stats=False)
+        o = e.run(1, gens + 1, 0.0, track_stats=False)
+    ms = o["profile"]["loop_ms"] / o["profile"]["generations"]
+    return {"workload": f"single instance G=4096, T=65, P={P}, seed 1, {gens} generations",
+            "ms_per_generation": ms, "genome_evals_per_s": P / ms * 1e3}
+
+
 def detailed_balance_batch(da, gens):
     """The detailed-balance variant (bdsf, with the -1st-moment term) on the
     seed-batch shape: 256 seeds of G=4096, T=65, P=32 for `gens` generations
@@ -430,6 +468,10 @@ def main():
                     help="timed generations of the detailed-balance seed batch (0: skip)")
     ap.add_argument("--small-pop-gens", type=int, default=300,
                     help="timed generations of the 512-seed P=8 batch, stacked and not (0: skip)")
+    ap.add_argument("--wide-pop-gens", type=int, default=300,
+                    help="timed generations of the 256-seed P=64 batch at each precision (0: skip)")
+    ap.add_argument("--plain-p512-gens", type=int, default=200,
+                    help="timed generations of the plain path at P=512 (0: skip)")
     ap.add_argument("--accuracy-gens", type=int, default=2000,
                     help="generations of the seed batch's FP64/FP32/BF16x3 accuracy comparison (0: skip)")
     ap.add_argument("--trajectory-gens", type=int, default=2000,
@@ -531,6 +573,8 @@ def main():
     accuracy = batch_precision_accuracy(da, prob, args.accuracy_gens) if args.accuracy_gens > 0 else None
     uneven = uneven_batch(da, args.uneven_gens) if args.uneven_gens > 0 else None
     small = small_population_batch(da, args.small_pop_gens) if args.small_pop_gens > 0 else None
+    wide = wide_population_batch(da, args.wide_pop_gens) if args.wide_pop_gens > 0 else None
+    plain512 = plain_default_population(da, args.plain_p512_gens) if args.plain_p512_gens > 0 else None
     bdsf = detailed_balance_batch(da, args.bdsf_gens) if args.bdsf_gens > 0 else None
     sweep = None if args.no_precision_sweep else precision_sweep(da)
     trajectories = precision_trajectories(da, args.trajectory_gens) if args.trajectory_gens > 0 else None
@@ -575,6 +619,8 @@ def main():
                    "batch_precision_accuracy": accuracy,
                    "uneven_batch": uneven,
                    "small_population_batch": small,
+                   "wide_population_batch": wide,
+                   "plain_fp64_default_population": plain512,
                    "detailed_balance_batch": bdsf,
                    "precision_sweep": sweep,
                    "precision_trajectories": trajectories},
