@@ -6,7 +6,7 @@ parent) at the mixed precision; the population is re-evaluated exactly in
 FP64 before the run, every refresh slice and at the end.  Bars, each written
 in its test:
   - the reported fitness is the exact FP64 chi^2 of the reported genome
-    (relative difference <= 1e-12; the north star asks 1e-6);
+    (relative difference <= 1e-12, well inside a 1e-6 requirement);
   - generation 0 is bit-identical to the FP64 batch (it is an exact FP64
     evaluation of the same initial population), and so is every returned
     RNG state (the draws do not depend on fitness values);
