@@ -165,10 +165,12 @@ def test_bdsf_batch_matches_oracle(name, slice_len, monkeypatch):
 # One case per register-tile shape of the fold kernel (2 RL + XR: RL
 # timeslices per lane, R = 64 RL / NM of them in the row blocks, XR extra
 # timeslices past R), for each member-tile count NM (P <= 16, 32, 64), with
-# odd and even genome lengths (the fused last position, partial chunks)
+# odd and even genome lengths (the fused last position, partial chunks) and
+# whole chunks of 128, 64 or 32 positions before the last
 FOLD_SHAPES = {
-    "nm1_rl1_x_T66_P16": dict(T=66, G=97, P=16),
-    "nm1_rl2_T100_P10": dict(T=100, G=64, P=10),
+    "nm1_rl1_x_T66_P16": dict(T=66, G=301, P=16),
+    "nm1_rl2_T100_P10": dict(T=100, G=257, P=10),
+    "nm1_rl2_T100_P10_one_chunk": dict(T=100, G=64, P=10),
     "nm2_rl1_T30_P24": dict(T=30, G=130, P=24),
     "nm2_rl1_x_T34_P24": dict(T=34, G=75, P=24),
     "nm2_rl2_T64_P20": dict(T=64, G=200, P=20),
