@@ -495,7 +495,7 @@ def test_batch_renormalization_matches_oracle(T, G, P, kw, temp, slice_len, monk
         assert o["rng"] == r["rng"] and o["generation"] == r["generation"]
 
 
-# mixed-kernel shapes (at most 32 members) where renormalization moves
+# mixed-kernel shapes where renormalization moves
 # generation 0's best fitness by more than 1e-3 (it barely does for the
 # hyperbolic model); the zero-temperature case has a first moment too
 MIXED_RENORM = [
@@ -503,6 +503,7 @@ MIXED_RENORM = [
     (65, 301, 32, dict(spectra_type="spbsf"), 1.0),
     (17, 90, 12, dict(spectra_type="spfsf", zero_temperature=True, first_moment=0.4), 0.0),
     (50, 150, 24, dict(spectra_type="spfsf"), 1.0),
+    (51, 128, 48, dict(spectra_type="spfsf"), 1.0),  # four member tiles
 ]
 
 
