@@ -221,15 +221,15 @@ def test_bdsf_batch_continues():
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
-@pytest.mark.parametrize("nfm", [True, False])
-def test_bdsf_mixed_batch(prec, nfm):
+@pytest.mark.parametrize("nfm,P", [(True, 24), (False, 24), (True, 48)])
+def test_bdsf_mixed_batch(prec, nfm, P):
     # detailed-balance instances at FP32 / BF16x3 K.S run on the mixed kernel
     # (delta form; the -1st-moment term from the mixed model ISF): the
     # initial fitness is the FP64 evaluation's, generation 0's best matches
     # the FP64 (fold) batch to the precision of the contraction, where the
     # -1st-moment term moves it by far more, and runs repeat bit for bit
-    print(f"[gpu] bdsf mixed batch {prec} nfm={nfm}", flush=True)
-    T, G, P = 65, 513, 24
+    print(f"[gpu] bdsf mixed batch {prec} nfm={nfm} P={P}", flush=True)
+    T, G = 65, 513
     tau = np.linspace(0.0, 0.4, T)
     isf = 0.6 * np.exp(-1.1 * tau) + 0.4 * np.exp(-0.3 * tau)
     err = 2e-3 * isf
