@@ -91,3 +91,32 @@ def test_noalloc_mixed_precision_and_refusals():
         # without statistics the same N needs no scratch (kept short here)
         st, got = _call(L.deac_batch_run_noalloc, b, seeds, 40, prec=0, track=0)
         assert st == 0
+
+
+def test_noalloc_fold_kernel_moments_and_stacked_populations():
+    # the detailed-balance fold kernel, moment terms on the mixed kernels at
+    # four member tiles, and FP64 populations stacked in one workgroup: the
+    # same bits as deac_batch_run, two calls in a row, the second smaller
+    print("[gpu] deac_batch_run_noalloc: bdsf, mixed moments, stacked", flush=True)
+    L = da.lib()
+    beta = 1.0 / 1.35
+
+    def prob(T, G, P, **kw):
+        t = np.linspace(0.0, beta / 2, T)
+        f = 0.6 * np.exp(-2.0 * t) + 0.4 * np.exp(-9.0 * t)
+        return da.make_problem(t, f, 1e-3 * f, da.frequency_grid(G, 40.0), 1.35, population_size=P, normalize=True,
+                               **kw)
+    probs = [prob(65, 200, 24, spectra_type="bdsf", use_negative_first_moment=True),
+             prob(40, 150, 48, first_moment=2.0), prob(33, 100, 6), prob(33, 100, 6)]
+    seeds = [3, 4, 5, 6]
+    with da.Batch(probs) as b:
+        sc = b.schedule
+        sc["pack"] = 4
+        b.schedule = sc
+        for prec in (0, 1):  # FP64 (fold kernel, stacked pair); FP32 (mixed kernels)
+            for N in (90, 40):
+                st, ref = _call(L.deac_batch_run, b, seeds, N, prec=prec)
+                assert st == 0, da._last_message()
+                st, got = _call(L.deac_batch_run_noalloc, b, seeds, N, prec=prec)
+                assert st == 0, da._last_message()
+                _same(got, ref)
